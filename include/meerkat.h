@@ -1,0 +1,186 @@
+/*
+ * meerkat.h - C ABI of libmeerkat_hip.so, the MI355X (gfx950) dynamic-graph
+ * library.
+ *
+ * The reference (Meerkat) is a header-only CUDA C++ template library; its
+ * "boundary" is the set of class methods its drivers and tests call.  Each
+ * entry point below replaces one of those methods and cites it (paths are
+ * relative to the reference tree).  The C++ classes in include/meerkat/ (one .hpp per class)
+ * restore the reference's class and method names on top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns an int status: MK_OK (0) or one of MK_ERR_*.
+ *    mk_last_error() returns the calling thread's last error text.
+ *  - Pointers named *_dev are device (HBM) pointers owned by the caller and
+ *    only read during the call; *_host pointers are host memory.
+ *  - Handles own their device state and are externally serialised (not
+ *    thread-safe); distinct handles may be driven from distinct threads.
+ *  - A pool must outlive every graph built on it, and a graph every
+ *    algorithm handle built on it (checked: destroy returns MK_ERR_STATE).
+ *  - Vertex ids are uint32 in [0, n); 0xFFFFFFFF and 0xFFFFFFFE are reserved
+ *    (empty / tombstone slab words).
+ *  - `stream` arguments are hipStream_t values passed as void* (NULL = the
+ *    null stream).  Graph mutations are ordered on that stream and complete
+ *    before the call returns.  Algorithm compute calls block until done and
+ *    report elapsed milliseconds (HIP events), like the reference's float
+ *    returns (include/Algorithms/impl/BFSImpl.cuh:644-661).
+ */
+#ifndef MEERKAT_H_
+#define MEERKAT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MK_OK 0
+#define MK_ERR_BAD_ARG 1   /* null handle/pointer, bad size or parameter */
+#define MK_ERR_POOL_FULL 2 /* slab pool exhausted; graph edge set unchanged */
+#define MK_ERR_HIP 3       /* a HIP runtime call failed */
+#define MK_ERR_STATE 4     /* call out of protocol order, or handle in use */
+#define MK_ERR_RANGE 5     /* vertex id >= n in a batch; graph unchanged */
+
+/* Per-edge outcome codes written by mk_graph_insert / mk_graph_delete. */
+#define MK_EDGE_UNCHANGED 0 /* insert: already present; delete: absent   */
+#define MK_EDGE_APPLIED 1   /* inserted as new / deleted                  */
+#define MK_EDGE_SELF_LOOP 2 /* skipped (InsertEdgeKernel.cuh:25,54)       */
+#define MK_EDGE_POOL_FULL 3 /* not inserted: pool exhausted (rolled back)  */
+
+typedef struct mk_pool mk_pool;
+typedef struct mk_graph mk_graph;
+typedef struct mk_bfs mk_bfs;
+typedef struct mk_sssp mk_sssp;
+typedef struct mk_pagerank mk_pagerank;
+typedef struct mk_tc mk_tc;
+typedef struct mk_cc mk_cc;
+typedef struct mk_msq mk_msq;
+
+/* Last error text of the calling thread ("" when none).  Replaces the
+ * reference's CHECK_ERROR print-and-exit (SlabHash macros used at e.g.
+ * include/Graph/SlabGraph.cuh:392-440). */
+const char *mk_last_error(void);
+/* ABI version, major*10000 + minor*100 + patch. */
+int mk_version(void);
+/* Device facts used by launch sizing: compute units and wavefront width. */
+int mk_device_info(int device, int *compute_units, int *wave_size);
+
+/* ---------------------------------------------------------------- A1 pool
+ * Slab pool: 256-byte slabs handed out from an HBM-resident bitmap.  Replaces
+ * AllocPolicy::DynamicAllocatorT (default-constructed at src/BFS.cu:53-55,
+ * held by reference in include/Graph/SlabGraph.cuh:369,620).  `bytes` = 0
+ * picks 1 GiB.  Stats report slabs in use and capacity. */
+int mk_pool_create(uint64_t bytes, int device, mk_pool **out);
+int mk_pool_destroy(mk_pool *pool);
+int mk_pool_stats(mk_pool *pool, uint64_t *used_slabs, uint64_t *total_slabs);
+
+/* --------------------------------------------------------------- A2 graph
+ * DynamicGraph(N, Allocator&, LoadFactor, VertexDegreeHints, DeviceIdx)
+ * (include/Graph/SlabGraph.cuh:382-466 weighted, 633-717 unweighted).
+ * weighted != 0 gives ConcurrentMap adjacency (dst -> uint32 weight), else
+ * ConcurrentSet.  Buckets per vertex = max(1, ceil(hint / (keys_per_slab *
+ * load_factor))) with 63 keys per set slab and 31 pairs per map slab.
+ * host_degree_hints may be NULL (one bucket per vertex). */
+int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
+                    const uint32_t *host_degree_hints, mk_graph **out);
+int mk_graph_destroy(mk_graph *g);
+
+/* DynamicGraph::InsertEdges(src, dst, n[, w])
+ * (include/Graph/SlabGraphImpl.cuh:6-34 -> device/InsertEdgeKernel.cuh:4-63).
+ * Self-loops are skipped.  An edge already present is left unchanged (its
+ * weight too).  w_dev may be NULL (weight 0, InsertEdgeKernel.cuh:24).  On
+ * MK_ERR_RANGE or MK_ERR_POOL_FULL the edge set is left as it was. */
+int mk_graph_insert(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_dev,
+                    const uint32_t *w_dev, uint32_t n, void *stream);
+/* DynamicGraph::DeleteEdges(src, dst, n)
+ * (include/Graph/SlabGraphImpl.cuh:36-62 -> device/DeleteEdgeKernel.cuh:4-26).
+ * A deleted key becomes a tombstone; later inserts reuse tombstoned slots. */
+int mk_graph_delete(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                    void *stream);
+/* Batched DynamicGraphContext::SearchEdge (include/Graph/warp/SearchEdge.cuh:
+ * 5-63, 118-142): found_dev[i] = 1 if (src[i], dst[i]) is present; for a
+ * weighted graph w_out_dev[i] (nullable) receives its weight. */
+int mk_graph_search(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                    uint8_t *found_dev, uint32_t *w_out_dev, void *stream);
+/* Device array of MK_EDGE_* codes for the most recent insert/delete batch.
+ * No reference counterpart: the reference's SlabHash insertKey returns this
+ * per lane (include/Graph/warp/InsertEdge.cuh:86-98) and discards it. */
+int mk_graph_last_status(mk_graph *g, const uint8_t **status_dev, uint32_t *n);
+
+/* Queries used by tests and algorithms.  VertexDegrees is the live
+ * out-degree array (SlabGraph.cuh:392-395). */
+int mk_graph_num_vertices(mk_graph *g, uint32_t *n);
+int mk_graph_num_edges(mk_graph *g, uint64_t *m);
+int mk_graph_is_weighted(mk_graph *g, int *weighted);
+int mk_graph_degrees(mk_graph *g, uint32_t *deg_host);
+/* GetVertexBucketsOffsets (SlabGraph.cuh:483-484), with n+1 entries. */
+int mk_graph_bucket_offsets(mk_graph *g, uint32_t *off_host);
+/* Per-vertex count of live keys and of tombstones found by walking every
+ * slab (the counting kernels of test/GraphTest.cu:58-200). */
+int mk_graph_count_slots(mk_graph *g, uint32_t *live_host, uint32_t *tomb_host);
+/* Adjacency as CSR on the host: row_off_host[n+1], col_host[m] (and
+ * w_host[m] when weighted and non-NULL), neighbours in slab order
+ * (the collection kernels of test/GraphTest.cu:202-483). */
+int mk_graph_export_csr(mk_graph *g, uint64_t *row_off_host, uint32_t *col_host, uint32_t *w_host);
+/* Head slabs, overflow slabs in use by this graph, and longest chain. */
+int mk_graph_slab_stats(mk_graph *g, uint64_t *head_slabs, uint64_t *overflow_slabs,
+                        uint32_t *max_chain);
+
+/* ------------------------------------------------------ A8-A10 BFS
+ * BFS(G, Src, N) (include/Algorithms/BFS.cuh:34-53).  Static
+ * (impl/BFSImpl.cuh:590-662), Incremental after InsertEdges with the same
+ * batch (:664-753), Decremental after DeleteEdges with the same batch
+ * (:755-846).  Distances are levels, UINT32_MAX when unreachable.  read
+ * returns real uint32 distances and parents (the reference's Distances()
+ * copies sizeof(float)*N bytes of a uint64 array, BFS.cuh:55-60). */
+int mk_bfs_create(mk_graph *g, uint32_t source, mk_bfs **out);
+int mk_bfs_destroy(mk_bfs *b);
+int mk_bfs_static(mk_bfs *b, float *ms);
+int mk_bfs_incremental(mk_bfs *b, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                       float *ms);
+int mk_bfs_decremental(mk_bfs *b, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                       float *ms);
+int mk_bfs_read(mk_bfs *b, uint32_t *dist_host, uint32_t *parent_host);
+/* Frontier iterations and adjacency slabs read by the last compute call. */
+int mk_bfs_last_stats(mk_bfs *b, uint32_t *iterations, uint64_t *slabs_read);
+
+/* ------------------------------------------------------ A11 SSSP
+ * SSSP(G, Src, N) (include/Algorithms/SSSP.cuh:34-69; impl/SSSPImpl.cuh
+ * Static :607-682, Incremental :684-778, Decremental :780-876) on a weighted
+ * graph.  Edge weights are read from the graph, which holds them; w_dev is
+ * accepted for signature parity and may be NULL.  Weights must be >= 1 for
+ * Decremental (a zero-weight cycle can carry a stale parent chain). */
+int mk_sssp_create(mk_graph *g, uint32_t source, mk_sssp **out);
+int mk_sssp_destroy(mk_sssp *s);
+int mk_sssp_static(mk_sssp *s, float *ms);
+int mk_sssp_incremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst_dev,
+                        const uint32_t *w_dev, uint32_t n, float *ms);
+int mk_sssp_decremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst_dev,
+                        const uint32_t *w_dev, uint32_t n, float *ms);
+int mk_sssp_read(mk_sssp *s, uint32_t *dist_host, uint32_t *parent_host);
+int mk_sssp_last_stats(mk_sssp *s, uint32_t *iterations, uint64_t *slabs_read);
+
+/* ------------------------------------------------------ workload generators
+ * Host-side, seeded, thread-count independent.  They replace the text edge
+ * lists and ".batch.<i>" files the reference drivers read (src/BFS.cu:23-36,
+ * 76-91).  mk_gen_rmat / mk_gen_er leave the sorted, de-duplicated,
+ * self-loop-free edge list in a per-thread buffer and return its length;
+ * mk_gen_fetch copies it out (weights 1 + draw mod 100, as src/SSSP.cu:52). */
+int mk_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                uint64_t seed, int undirected, uint64_t *out_edges);
+int mk_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint64_t *out_edges);
+int mk_gen_fetch(uint32_t *src_host, uint32_t *dst_host, uint32_t *w_host, uint64_t wseed);
+/* New edges absent from sorted_keys (src<<32|dst ascending) and distinct. */
+int mk_gen_insert_batch(const uint64_t *sorted_keys, uint64_t nkeys, uint32_t n, uint32_t count,
+                        uint64_t seed, uint32_t *src_host, uint32_t *dst_host, uint32_t *w_host,
+                        uint64_t wseed);
+/* Distinct edges sampled without replacement from sorted_keys. */
+int mk_gen_delete_batch(const uint64_t *sorted_keys, uint64_t nkeys, uint32_t count, uint64_t seed,
+                        uint32_t *src_host, uint32_t *dst_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEERKAT_H_ */

@@ -1,0 +1,102 @@
+"""Host-side mirror of the reference's algorithm classes over the C ABI.
+
+Each class keeps the reference's method names and protocol: mutate the graph
+first (``InsertEdges`` / ``DeleteEdges``), then call ``Incremental`` /
+``Decremental`` with the same batch; every compute call blocks and returns
+elapsed milliseconds (include/Algorithms/BFS.cuh:34-68, SSSP.cuh:34-69,
+PageRank.cuh:32-67, TC.cuh:28-40).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .graph import device_u32, dptr
+
+_vp = ctypes.c_void_p
+INF = 0xFFFFFFFF
+
+
+class _Traversal:
+    _prefix = None
+
+    def __init__(self, graph, source):
+        self.G = graph
+        self.source = int(source)
+        self._h = _vp()
+        _lib.call(f"{self._prefix}_create", graph.handle, self.source, ctypes.byref(self._h))
+
+    def Static(self, edges_n=0):
+        ms = ctypes.c_float()
+        _lib.call(f"{self._prefix}_static", self._h, ctypes.byref(ms))
+        return ms.value
+
+    def read(self):
+        """(distances, parents) as uint32 numpy arrays; INF = 0xFFFFFFFF."""
+        d = np.empty(self.G.n, np.uint32)
+        p = np.empty(self.G.n, np.uint32)
+        _lib.call(f"{self._prefix}_read", self._h, d.ctypes.data_as(_vp), p.ctypes.data_as(_vp))
+        return d, p
+
+    def Distances(self):
+        return self.read()[0]
+
+    def Parents(self):
+        return self.read()[1]
+
+    def last_stats(self):
+        it, slabs = ctypes.c_uint32(), ctypes.c_uint64()
+        _lib.call(f"{self._prefix}_last_stats", self._h, ctypes.byref(it), ctypes.byref(slabs))
+        return {"iterations": it.value, "slabs_read": slabs.value}
+
+    def close(self):
+        if self._h:
+            _lib.call(f"{self._prefix}_destroy", self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h:
+                getattr(_lib.lib(), f"{self._prefix}_destroy")(self._h)
+        except Exception:
+            pass
+
+
+class BFS(_Traversal):
+    """Levels from a source vertex (include/Algorithms/BFS.cuh:13-69)."""
+
+    _prefix = "mk_bfs"
+
+    def Incremental(self, src, dst):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_bfs_incremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+    def Decremental(self, src, dst):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_bfs_decremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+
+class SSSP(_Traversal):
+    """Shortest distances with uint32 weights (include/Algorithms/SSSP.cuh:11-69)."""
+
+    _prefix = "mk_sssp"
+
+    def Incremental(self, src, dst, weights=None):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        w = device_u32(weights, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_sssp_incremental", self._h, dptr(s), dptr(d), dptr(w), len(s),
+                  ctypes.byref(ms))
+        return ms.value
+
+    def Decremental(self, src, dst, weights=None):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        w = device_u32(weights, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_sssp_decremental", self._h, dptr(s), dptr(d), dptr(w), len(s),
+                  ctypes.byref(ms))
+        return ms.value

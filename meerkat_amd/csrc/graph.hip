@@ -1,0 +1,707 @@
+// Slab pool, dynamic graph construction and batched insert / delete /
+// search kernels, plus the graph inspection calls used by tests.
+//
+// Reference: include/Graph/SlabGraph.cuh (DynamicGraph ctor :382-466 and
+// :633-717), include/Graph/SlabGraphImpl.cuh (:6-62), device/Insert-
+// EdgeKernel.cuh, device/DeleteEdgeKernel.cuh, warp/*.cuh, and the absent
+// SlabHash submodule whose contract is restated in slabhash.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "slabhash.hpp"
+
+namespace mk {
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char *what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return MK_ERR_HIP;
+}
+
+int device_cus(int device) {
+  static int cached[64] = {0};
+  if (device < 0 || device >= 64) return 256;
+  if (cached[device] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cached[device] = v;
+  }
+  return cached[device];
+}
+
+int DevBuf::ensure(size_t need) {
+  if (need <= bytes && ptr) return MK_OK;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+  size_t want = std::max<size_t>(need, 256);
+  MK_HIP(hipMalloc(&ptr, want));
+  bytes = want;
+  return MK_OK;
+}
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+int HostBuf::ensure(size_t need) {
+  if (need <= bytes && ptr) return MK_OK;
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+  size_t want = std::max<size_t>(need, 256);
+  MK_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+  bytes = want;
+  return MK_OK;
+}
+void HostBuf::release() {
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+// ---------------------------------------------------------------- fills
+template <typename T> __global__ void k_fill(T *p, T v, uint64_t count) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+static unsigned fill_blocks(uint64_t count) {
+  uint64_t b = (count + kBlock - 1) / kBlock;
+  return unsigned(std::min<uint64_t>(std::max<uint64_t>(b, 1), 65536));
+}
+void fill_u32(uint32_t *p, uint32_t v, uint64_t count, hipStream_t s) {
+  if (count) k_fill<uint32_t><<<fill_blocks(count), kBlock, 0, s>>>(p, v, count);
+}
+void fill_u64(uint64_t *p, uint64_t v, uint64_t count, hipStream_t s) {
+  if (count) k_fill<uint64_t><<<fill_blocks(count), kBlock, 0, s>>>(p, v, count);
+}
+void fill_u8(uint8_t *p, uint8_t v, uint64_t count, hipStream_t s) {
+  if (count) k_fill<uint8_t><<<fill_blocks(count), kBlock, 0, s>>>(p, v, count);
+}
+
+// ---------------------------------------------------------------- kernels
+// owner[b] = the vertex whose bucket range holds b (upper_bound - 1).
+__global__ void k_bucket_owner(const uint32_t *off, uint32_t n, uint32_t total, uint32_t *owner) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= total) return;
+  uint32_t lo = 0, hi = n;  // find largest v with off[v] <= b
+  while (hi - lo > 1) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  owner[b] = lo;
+}
+
+// Range check of a batch: counters[0] += edges with an endpoint >= n.
+__global__ void k_validate(const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t nv,
+                           uint32_t *counters) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < n) bad = src[i] >= nv || dst[i] >= nv;
+  uint64_t m = ballot(bad);
+  if (m && lane_id() == ffs64(m)) atomicAdd(counters, uint32_t(__popcll(m)));
+}
+
+// Each wave takes `epw` (<= 64) consecutive edges, lane i holding edge i, and
+// applies them one after another with the whole wave (the reference's
+// per-warp work queue, include/Graph/warp/InsertEdge.cuh:66-103).
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_insert(GraphView G, PoolView P, const uint32_t *src, const uint32_t *dst, const uint32_t *wt,
+             uint32_t n, uint32_t epw, uint8_t *status, uint32_t *counters) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t base = uint64_t(wave) * epw;
+  if (base >= n) return;
+  const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  uint32_t s = 0, d = 0, w = 0;
+  bool act = false;
+  uint8_t st = MK_EDGE_UNCHANGED;
+  if (lane < cnt) {
+    s = src[base + lane];
+    d = dst[base + lane];
+    w = (MAP && wt) ? wt[base + lane] : 0u;
+    act = s != d;
+    if (!act) st = MK_EDGE_SELF_LOOP;
+  }
+  uint64_t pending = ballot(act);
+  uint32_t applied = 0, pool_fail = 0;
+  while (pending) {
+    const uint32_t j = ffs64(pending);
+    pending &= pending - 1;
+    const uint32_t sj = read_lane(s, j), dj = read_lane(d, j), wj = read_lane(w, j);
+    const int r = slab_insert<MAP>(G, P, sj, dj, wj, wave * 64u + j);
+    if (lane == j) {
+      if (r == 1) st = MK_EDGE_APPLIED;
+      else if (r < 0) st = MK_EDGE_POOL_FULL;  // rolled back by the host
+    }
+    if (r == 1) {
+      ++applied;
+      if (lane == 0) atomicAdd(G.vdeg + sj, 1u);
+    } else if (r < 0) {
+      ++pool_fail;
+    }
+  }
+  if (lane < cnt) status[base + lane] = st;
+  if (lane == 0) {
+    if (applied) atomicAdd(counters + 2, applied);
+    if (pool_fail) atomicAdd(counters + 1, pool_fail);
+  }
+}
+
+// Delete the batch edges whose status equals `only_status` (all edges when
+// only_status == 0xFF); records MK_EDGE_APPLIED / MK_EDGE_UNCHANGED.
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_delete(GraphView G, const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t epw,
+             uint8_t *status, uint32_t only_status, uint32_t *counters) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t base = uint64_t(wave) * epw;
+  if (base >= n) return;
+  const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  uint32_t s = 0, d = 0;
+  bool act = false;
+  uint8_t st = MK_EDGE_UNCHANGED;
+  if (lane < cnt) {
+    s = src[base + lane];
+    d = dst[base + lane];
+    if (only_status != 0xFFu) {
+      act = status[base + lane] == only_status;
+    } else {
+      act = s != d;
+      if (!act) st = MK_EDGE_SELF_LOOP;
+    }
+  }
+  uint64_t pending = ballot(act);
+  uint32_t applied = 0;
+  while (pending) {
+    const uint32_t j = ffs64(pending);
+    pending &= pending - 1;
+    const uint32_t sj = read_lane(s, j), dj = read_lane(d, j);
+    const int r = slab_delete<MAP>(G, sj, dj);
+    if (lane == j && r == 1) st = MK_EDGE_APPLIED;
+    if (r == 1) {
+      ++applied;
+      if (lane == 0) atomicSub(G.vdeg + sj, 1u);
+    }
+  }
+  if (only_status != 0xFFu) {
+    if (lane < cnt && act) status[base + lane] = MK_EDGE_UNCHANGED;
+  } else if (lane < cnt) {
+    status[base + lane] = st;
+  }
+  if (lane == 0 && applied) atomicAdd(counters + 2, applied);
+}
+
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_search(GraphView G, const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t epw,
+             uint8_t *found, uint32_t *wout) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t base = uint64_t(wave) * epw;
+  if (base >= n) return;
+  const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  uint32_t s = 0, d = 0;
+  bool act = false;
+  if (lane < cnt) {
+    s = src[base + lane];
+    d = dst[base + lane];
+    act = s < G.n && d < G.n;
+  }
+  uint8_t f = 0;
+  uint32_t wv = 0;
+  uint64_t pending = ballot(act);
+  while (pending) {
+    const uint32_t j = ffs64(pending);
+    pending &= pending - 1;
+    uint32_t wj = 0;
+    const int r = slab_search<MAP>(G, read_lane(s, j), read_lane(d, j), &wj);
+    if (lane == j) {
+      f = uint8_t(r);
+      wv = wj;
+    }
+  }
+  if (lane < cnt) {
+    found[base + lane] = f;
+    if (wout) wout[base + lane] = f ? wv : kEmpty;
+  }
+}
+
+// Per-bucket walk: live keys and tombstones added to the owner's counters;
+// overflow slab count and chain length maximum accumulated.
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_count_slots(GraphView G, uint32_t *live, uint32_t *tomb, unsigned long long *overflow,
+                  uint32_t *max_chain) {
+  const uint32_t lane = lane_id();
+  const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= G.total_buckets) return;
+  const uint64_t km = key_mask<MAP>();
+  const uint32_t *slab = G.heads + size_t(b) * kSlabWords;
+  uint32_t nl = 0, nt = 0, chain = 1;
+  for (;;) {
+    const uint32_t v = slab[lane];
+    nl += uint32_t(__popcll(ballot(v < kTomb) & km));
+    nt += uint32_t(__popcll(ballot(v == kTomb) & km));
+    const uint32_t nx = read_lane(v, kNextLane);
+    if (nx == kEmpty) break;
+    slab = slab_ptr(G.pool, nx);
+    ++chain;
+  }
+  if (lane == 0) {
+    const uint32_t u = G.bucket_owner[b];
+    if (live && nl) atomicAdd(live + u, nl);
+    if (tomb && nt) atomicAdd(tomb + u, nt);
+    if (overflow && chain > 1) atomicAdd(overflow, (unsigned long long)(chain - 1));
+    if (max_chain) atomicMax(max_chain, chain);
+  }
+}
+
+// Write each bucket's live keys (and values) at cursor[owner]++.
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_export(GraphView G, unsigned long long *cursor, uint32_t *col, uint32_t *wcol) {
+  const uint32_t lane = lane_id();
+  const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= G.total_buckets) return;
+  const uint64_t km = key_mask<MAP>();
+  const uint32_t u = G.bucket_owner[b];
+  const uint32_t *slab = G.heads + size_t(b) * kSlabWords;
+  for (;;) {
+    const uint32_t v = slab[lane];
+    const uint32_t val = MAP ? __shfl_down(v, 1, 64) : 0u;
+    const bool live = ((km >> lane) & 1ull) && v < kTomb;
+    const uint64_t m = ballot(live);
+    if (m) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(cursor + u, (unsigned long long)__popcll(m));
+      base = __shfl(base, 0, 64);
+      if (live) {
+        const uint64_t pos = base + lanes_below(m);
+        col[pos] = v;
+        if (MAP && wcol) wcol[pos] = val;
+      }
+    }
+    const uint32_t nx = read_lane(v, kNextLane);
+    if (nx == kEmpty) break;
+    slab = slab_ptr(G.pool, nx);
+  }
+}
+
+// Return every overflow slab of every chain to the pool.
+__global__ void __launch_bounds__(kBlock) k_release(GraphView G, PoolView P) {
+  const uint32_t lane = lane_id();
+  const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= G.total_buckets) return;
+  uint32_t nx = G.heads[size_t(b) * kSlabWords + kNextLane];
+  while (nx != kEmpty) {
+    const uint32_t next = slab_ptr(G.pool, nx)[kNextLane];
+    if (lane == 0) pool_free_lane(P, nx);
+    nx = next;
+  }
+}
+
+static uint32_t edges_per_wave(uint32_t n, int device) {
+  // enough waves to fill every CU ~16 deep, at most 64 edges (one per lane)
+  const uint64_t target_waves = uint64_t(device_cus(device)) * 16;
+  uint64_t epw = (n + target_waves - 1) / target_waves;
+  return uint32_t(std::min<uint64_t>(std::max<uint64_t>(epw, 1), 64));
+}
+
+}  // namespace mk
+
+using namespace mk;
+
+// ================================================================ C ABI
+extern "C" {
+
+const char *mk_last_error(void) { return g_err.c_str(); }
+int mk_version(void) { return 100; }
+
+int mk_device_info(int device, int *compute_units, int *wave_size) {
+  int cu = 0, ws = 0;
+  MK_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
+  MK_HIP(hipDeviceGetAttribute(&ws, hipDeviceAttributeWarpSize, device));
+  if (compute_units) *compute_units = cu;
+  if (wave_size) *wave_size = ws;
+  return MK_OK;
+}
+
+int mk_pool_create(uint64_t bytes, int device, mk_pool **out) {
+  if (!out) return fail(MK_ERR_BAD_ARG, "mk_pool_create: out is NULL");
+  *out = nullptr;
+  if (bytes == 0) bytes = 1ull << 30;
+  uint64_t slabs = bytes / (kSlabWords * 4);
+  slabs = (slabs / kBlockSlabs) * kBlockSlabs;
+  if (slabs == 0) slabs = kBlockSlabs;
+  if (slabs > 0xFFFF0000ull) return fail(MK_ERR_BAD_ARG, "mk_pool_create: pool above 2^32 slabs");
+  int wave = 0;
+  MK_HIP(hipSetDevice(device));
+  MK_HIP(hipDeviceGetAttribute(&wave, hipDeviceAttributeWarpSize, device));
+  if (wave != 64) return fail(MK_ERR_HIP, "mk_pool_create: device wavefront is not 64 lanes");
+  mk_pool *p = new mk_pool();
+  p->device = device;
+  p->total_slabs = slabs;
+  p->view.nblocks = uint32_t(slabs / kBlockSlabs);
+  p->view.total_slabs = uint32_t(slabs);
+  hipError_t e = hipMalloc(&p->view.slabs, slabs * kSlabWords * 4);
+  if (e == hipSuccess) e = hipMalloc(&p->view.bitmap, slabs / 8);
+  if (e == hipSuccess) e = hipMalloc(&p->view.counters, 64);
+  if (e != hipSuccess) {
+    (void)hipFree(p->view.slabs);
+    (void)hipFree(p->view.bitmap);
+    delete p;
+    return hip_fail(e, "mk_pool_create: hipMalloc");
+  }
+  fill_u32(p->view.bitmap, 0u, slabs / 32, nullptr);
+  fill_u32(p->view.counters, 0u, 16, nullptr);
+  MK_HIP(hipDeviceSynchronize());
+  *out = p;
+  return MK_OK;
+}
+
+int mk_pool_destroy(mk_pool *p) {
+  if (!p) return fail(MK_ERR_BAD_ARG, "mk_pool_destroy: NULL pool");
+  if (p->graphs) return fail(MK_ERR_STATE, "mk_pool_destroy: graphs still use this pool");
+  (void)hipSetDevice(p->device);
+  (void)hipFree(p->view.slabs);
+  (void)hipFree(p->view.bitmap);
+  (void)hipFree(p->view.counters);
+  delete p;
+  return MK_OK;
+}
+
+int mk_pool_stats(mk_pool *p, uint64_t *used, uint64_t *total) {
+  if (!p) return fail(MK_ERR_BAD_ARG, "mk_pool_stats: NULL pool");
+  uint32_t c[2] = {0, 0};
+  MK_HIP(hipSetDevice(p->device));
+  MK_HIP(hipMemcpy(c, p->view.counters, sizeof(c), hipMemcpyDeviceToHost));
+  if (used) *used = c[0];
+  if (total) *total = p->total_slabs;
+  return MK_OK;
+}
+
+int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
+                    const uint32_t *hints, mk_graph **out) {
+  if (!out) return fail(MK_ERR_BAD_ARG, "mk_graph_create: out is NULL");
+  *out = nullptr;
+  if (!pool) return fail(MK_ERR_BAD_ARG, "mk_graph_create: NULL pool");
+  if (n == 0 || n >= kTomb) return fail(MK_ERR_BAD_ARG, "mk_graph_create: n out of range");
+  if (!(load_factor > 0.0) || load_factor > 64.0)
+    return fail(MK_ERR_BAD_ARG, "mk_graph_create: load_factor must be in (0, 64]");
+  MK_HIP(hipSetDevice(pool->device));
+  const double per = (weighted ? kMapKeysPerSlab : kSetKeysPerSlab) * load_factor;
+  std::vector<uint32_t> off(size_t(n) + 1);
+  uint64_t total = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    off[v] = uint32_t(total);
+    uint64_t nb = hints ? uint64_t(std::ceil(double(hints[v]) / per)) : 1;
+    total += nb ? nb : 1;
+    if (total >= 0xFFFFFFF0ull) return fail(MK_ERR_BAD_ARG, "mk_graph_create: >2^32 buckets");
+  }
+  off[n] = uint32_t(total);
+  mk_graph *g = new mk_graph();
+  g->pool = pool;
+  g->device = pool->device;
+  g->n = n;
+  g->weighted = weighted ? 1 : 0;
+  g->load_factor = load_factor;
+  g->total_buckets = uint32_t(total);
+  hipError_t e = hipMalloc(&g->heads, total * kSlabWords * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->bucket_off, (size_t(n) + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->bucket_owner, total * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->vdeg, size_t(n) * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->counters, 64);
+  if (e == hipSuccess)
+    e = hipMemcpy(g->bucket_off, off.data(), (size_t(n) + 1) * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(g->heads);
+    (void)hipFree(g->bucket_off);
+    (void)hipFree(g->bucket_owner);
+    (void)hipFree(g->vdeg);
+    (void)hipFree(g->counters);
+    delete g;
+    return hip_fail(e, "mk_graph_create: allocation");
+  }
+  fill_u32(g->heads, kEmpty, total * kSlabWords, nullptr);
+  fill_u32(g->vdeg, 0u, n, nullptr);
+  fill_u32(g->counters, 0u, 16, nullptr);
+  k_bucket_owner<<<blocks_for_threads(total), kBlock>>>(g->bucket_off, n, uint32_t(total),
+                                                       g->bucket_owner);
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    set_error(std::string("mk_graph_create: init: ") + hipGetErrorString(e));
+    return MK_ERR_HIP;
+  }
+  g->bucket_off_host.swap(off);
+  pool->graphs++;
+  *out = g;
+  return MK_OK;
+}
+
+int mk_graph_destroy(mk_graph *g) {
+  if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_destroy: NULL graph");
+  if (g->users) return fail(MK_ERR_STATE, "mk_graph_destroy: algorithm handles still attached");
+  (void)hipSetDevice(g->device);
+  k_release<<<blocks_for_waves(g->total_buckets), kBlock>>>(g->view(), g->pool->view);
+  (void)hipDeviceSynchronize();
+  (void)hipFree(g->heads);
+  (void)hipFree(g->bucket_off);
+  (void)hipFree(g->bucket_owner);
+  (void)hipFree(g->vdeg);
+  (void)hipFree(g->counters);
+  g->status.release();
+  g->host_scratch.release();
+  g->pool->graphs--;
+  delete g;
+  return MK_OK;
+}
+
+// Shared front half of insert / delete: argument checks, range validation.
+static int batch_prologue(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
+                          hipStream_t s, const char *who) {
+  if (!g) return fail(MK_ERR_BAD_ARG, std::string(who) + ": NULL graph");
+  if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, std::string(who) + ": NULL batch");
+  MK_HIP(hipSetDevice(g->device));
+  if (int r = g->status.ensure(std::max<size_t>(n, 1))) return r;
+  if (int r = g->host_scratch.ensure(64)) return r;
+  g->status_n = n;
+  fill_u32(g->counters, 0u, 4, s);
+  if (n == 0) return MK_OK;
+  k_validate<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters);
+  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
+  MK_HIP(hipStreamSynchronize(s));
+  const uint32_t bad = g->host_scratch.as<uint32_t>()[0];
+  if (bad) {
+    fill_u8(g->status.as<uint8_t>(), MK_EDGE_UNCHANGED, n, s);
+    MK_HIP(hipStreamSynchronize(s));
+    return fail(MK_ERR_RANGE, std::string(who) + ": " + std::to_string(bad) +
+                                  " edges have a vertex id >= n; graph unchanged");
+  }
+  return MK_OK;
+}
+
+int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                    uint32_t n, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
+  if (n == 0) return MK_OK;
+  const uint32_t epw = edges_per_wave(n, g->device);
+  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
+  uint8_t *st = g->status.as<uint8_t>();
+  if (g->weighted)
+    k_insert<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src, dst,
+                                                              w, n, epw, st, g->counters);
+  else
+    k_insert<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
+                                                               dst, nullptr, n, epw, st,
+                                                               g->counters);
+  MK_HIP(hipGetLastError());
+  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
+  MK_HIP(hipStreamSynchronize(s));
+  const uint32_t pool_fail = g->host_scratch.as<uint32_t>()[1];
+  if (pool_fail) {
+    // roll back: delete what this batch inserted, restoring the edge set
+    if (g->weighted)
+      k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                                MK_EDGE_APPLIED, g->counters);
+    else
+      k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                                 MK_EDGE_APPLIED, g->counters);
+    MK_HIP(hipStreamSynchronize(s));
+    return fail(MK_ERR_POOL_FULL, "mk_graph_insert: slab pool exhausted (" +
+                                      std::to_string(pool_fail) +
+                                      " edges); batch rolled back, edge set unchanged");
+  }
+  return MK_OK;
+}
+
+int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
+                    void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
+  if (n == 0) return MK_OK;
+  const uint32_t epw = edges_per_wave(n, g->device);
+  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
+  uint8_t *st = g->status.as<uint8_t>();
+  if (g->weighted)
+    k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                              0xFFu, g->counters);
+  else
+    k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                               0xFFu, g->counters);
+  MK_HIP(hipGetLastError());
+  MK_HIP(hipStreamSynchronize(s));
+  return MK_OK;
+}
+
+int mk_graph_search(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
+                    uint8_t *found, uint32_t *wout, void *stream) {
+  if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_search: NULL graph");
+  if (n && (!src || !dst || !found)) return fail(MK_ERR_BAD_ARG, "mk_graph_search: NULL buffer");
+  if (n == 0) return MK_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  MK_HIP(hipSetDevice(g->device));
+  const uint32_t epw = edges_per_wave(n, g->device);
+  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
+  if (g->weighted)
+    k_search<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, found,
+                                                              wout);
+  else
+    k_search<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, found,
+                                                               wout);
+  MK_HIP(hipGetLastError());
+  MK_HIP(hipStreamSynchronize(s));
+  return MK_OK;
+}
+
+int mk_graph_last_status(mk_graph *g, const uint8_t **status, uint32_t *n) {
+  if (!g || !status || !n) return fail(MK_ERR_BAD_ARG, "mk_graph_last_status: NULL argument");
+  *status = g->status.as<uint8_t>();
+  *n = g->status_n;
+  return MK_OK;
+}
+
+int mk_graph_num_vertices(mk_graph *g, uint32_t *n) {
+  if (!g || !n) return fail(MK_ERR_BAD_ARG, "mk_graph_num_vertices: NULL argument");
+  *n = g->n;
+  return MK_OK;
+}
+
+int mk_graph_is_weighted(mk_graph *g, int *w) {
+  if (!g || !w) return fail(MK_ERR_BAD_ARG, "mk_graph_is_weighted: NULL argument");
+  *w = g->weighted;
+  return MK_OK;
+}
+
+int mk_graph_degrees(mk_graph *g, uint32_t *deg) {
+  if (!g || !deg) return fail(MK_ERR_BAD_ARG, "mk_graph_degrees: NULL argument");
+  MK_HIP(hipSetDevice(g->device));
+  MK_HIP(hipMemcpy(deg, g->vdeg, size_t(g->n) * 4, hipMemcpyDeviceToHost));
+  return MK_OK;
+}
+
+int mk_graph_num_edges(mk_graph *g, uint64_t *m) {
+  if (!g || !m) return fail(MK_ERR_BAD_ARG, "mk_graph_num_edges: NULL argument");
+  std::vector<uint32_t> deg(g->n);
+  if (int r = mk_graph_degrees(g, deg.data())) return r;
+  uint64_t t = 0;
+  for (uint32_t d : deg) t += d;
+  *m = t;
+  return MK_OK;
+}
+
+int mk_graph_bucket_offsets(mk_graph *g, uint32_t *off) {
+  if (!g || !off) return fail(MK_ERR_BAD_ARG, "mk_graph_bucket_offsets: NULL argument");
+  std::memcpy(off, g->bucket_off_host.data(), (size_t(g->n) + 1) * 4);
+  return MK_OK;
+}
+
+static int count_slots(mk_graph *g, uint32_t *live_dev, uint32_t *tomb_dev,
+                       unsigned long long *overflow_dev, uint32_t *max_dev) {
+  if (g->weighted)
+    k_count_slots<true><<<blocks_for_waves(g->total_buckets), kBlock>>>(g->view(), live_dev,
+                                                                       tomb_dev, overflow_dev,
+                                                                       max_dev);
+  else
+    k_count_slots<false><<<blocks_for_waves(g->total_buckets), kBlock>>>(g->view(), live_dev,
+                                                                        tomb_dev, overflow_dev,
+                                                                        max_dev);
+  MK_HIP(hipGetLastError());
+  return MK_OK;
+}
+
+int mk_graph_count_slots(mk_graph *g, uint32_t *live, uint32_t *tomb) {
+  if (!g || (!live && !tomb)) return fail(MK_ERR_BAD_ARG, "mk_graph_count_slots: NULL argument");
+  MK_HIP(hipSetDevice(g->device));
+  DevBuf b;
+  if (int r = b.ensure(size_t(g->n) * 8)) return r;
+  uint32_t *l = b.as<uint32_t>(), *t = l + g->n;
+  fill_u32(l, 0u, uint64_t(g->n) * 2, nullptr);
+  int r = count_slots(g, l, t, nullptr, nullptr);
+  if (r == MK_OK) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && live) e = hipMemcpy(live, l, size_t(g->n) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && tomb) e = hipMemcpy(tomb, t, size_t(g->n) * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = hip_fail(e, "mk_graph_count_slots");
+  }
+  b.release();
+  return r;
+}
+
+int mk_graph_slab_stats(mk_graph *g, uint64_t *heads, uint64_t *overflow, uint32_t *max_chain) {
+  if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_slab_stats: NULL graph");
+  MK_HIP(hipSetDevice(g->device));
+  DevBuf b;
+  if (int r = b.ensure(16)) return r;
+  unsigned long long *ov = b.as<unsigned long long>();
+  uint32_t *mx = reinterpret_cast<uint32_t *>(ov + 1);
+  MK_HIP(hipMemset(b.ptr, 0, 16));
+  int r = count_slots(g, nullptr, nullptr, ov, mx);
+  unsigned long long host[2] = {0, 0};
+  if (r == MK_OK) {
+    hipError_t e = hipMemcpy(host, b.ptr, 16, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = hip_fail(e, "mk_graph_slab_stats");
+  }
+  b.release();
+  if (heads) *heads = g->total_buckets;
+  if (overflow) *overflow = host[0];
+  if (max_chain) *max_chain = uint32_t(host[1]);
+  return r;
+}
+
+int mk_graph_export_csr(mk_graph *g, uint64_t *row_off, uint32_t *col, uint32_t *wcol) {
+  if (!g || !row_off) return fail(MK_ERR_BAD_ARG, "mk_graph_export_csr: NULL argument");
+  MK_HIP(hipSetDevice(g->device));
+  std::vector<uint32_t> live(g->n);
+  if (int r = mk_graph_count_slots(g, live.data(), nullptr)) return r;
+  row_off[0] = 0;
+  for (uint32_t v = 0; v < g->n; ++v) row_off[v + 1] = row_off[v] + live[v];
+  const uint64_t m = row_off[g->n];
+  if (!col) return MK_OK;
+  DevBuf cur, c, w;
+  int r = cur.ensure(size_t(g->n) * 8);
+  if (!r) r = c.ensure(std::max<uint64_t>(m, 1) * 4);
+  if (!r && g->weighted && wcol) r = w.ensure(std::max<uint64_t>(m, 1) * 4);
+  if (!r) {
+    hipError_t e =
+        hipMemcpy(cur.ptr, row_off, size_t(g->n) * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      if (g->weighted)
+        k_export<true><<<blocks_for_waves(g->total_buckets), kBlock>>>(
+            g->view(), cur.as<unsigned long long>(), c.as<uint32_t>(),
+            wcol ? w.as<uint32_t>() : nullptr);
+      else
+        k_export<false><<<blocks_for_waves(g->total_buckets), kBlock>>>(
+            g->view(), cur.as<unsigned long long>(), c.as<uint32_t>(), nullptr);
+      e = hipDeviceSynchronize();
+    }
+    if (e == hipSuccess && m) e = hipMemcpy(col, c.ptr, m * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && m && g->weighted && wcol)
+      e = hipMemcpy(wcol, w.ptr, m * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = hip_fail(e, "mk_graph_export_csr");
+  }
+  cur.release();
+  c.release();
+  w.release();
+  return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// Internal definitions shared by the HIP translation units of
+// libmeerkat_hip.so: slab layout, bucket hashing, wavefront helpers, the
+// device views passed to kernels by value, and the host handle structs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "meerkat.h"
+
+namespace mk {
+
+// ---------------------------------------------------------------- slabs
+// A slab is 64 x uint32 = 256 B: one dword per lane of a wave64, read with a
+// single coalesced load.  Lane 63 holds the index of the next slab in the
+// pool (EMPTY = end of chain).
+//   set slab: lanes 0..62 hold keys (63 keys).
+//   map slab: even lanes 0..60 hold keys, lane k+1 holds key k's value
+//             (31 pairs); lane 62 is unused.
+// Words equal to EMPTY are free; TOMB marks a deleted key (reusable by a
+// later insert kernel, never by a concurrent one).
+constexpr uint32_t kSlabWords = 64;
+constexpr uint32_t kNextLane = 63;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kTomb = 0xFFFFFFFEu;
+constexpr uint64_t kSetKeyMask = 0x7FFFFFFFFFFFFFFFull;  // lanes 0..62
+constexpr uint64_t kMapKeyMask = 0x1555555555555555ull;  // lanes 0,2,..,60
+constexpr uint32_t kSetKeysPerSlab = 63;
+constexpr uint32_t kMapKeysPerSlab = 31;
+
+// Pool geometry: a block is 2048 slabs whose 2048 bitmap bits are 64 words,
+// one per lane, so a wave scans a whole block with one load.
+constexpr uint32_t kBlockSlabs = 2048;
+constexpr uint32_t kBlockWords = 64;
+
+// 64-bit packed distance: (distance << 32) | parent; a 64-bit atomicMin is
+// a lexicographic (distance, parent) minimum (include/Algorithms/
+// definitions.cuh:12-17).
+constexpr uint64_t kInfInf = 0xFFFFFFFFFFFFFFFFull;
+constexpr uint32_t kInf = 0xFFFFFFFFu;
+
+__host__ __device__ inline uint64_t pack(uint32_t d, uint32_t p) {
+  return (uint64_t(d) << 32) | p;
+}
+__host__ __device__ inline uint32_t dist_of(uint64_t x) { return uint32_t(x >> 32); }
+__host__ __device__ inline uint32_t parent_of(uint64_t x) { return uint32_t(x); }
+
+// Bucket of `key` among the `nb` buckets of a vertex: a 32-bit finaliser
+// (lowbias32 of C. Wellons' hash-prospector) and a multiply-shift range map.
+__host__ __device__ inline uint32_t key_hash(uint32_t k) {
+  k ^= k >> 16;
+  k *= 0x7FEB352Du;
+  k ^= k >> 15;
+  k *= 0x846CA68Bu;
+  k ^= k >> 16;
+  return k;
+}
+__host__ __device__ inline uint32_t bucket_in(uint32_t key, uint32_t nb) {
+  return uint32_t((uint64_t(key_hash(key)) * nb) >> 32);
+}
+
+// ---------------------------------------------------------------- views
+struct PoolView {
+  uint32_t *slabs;     // total_slabs * 64 words
+  uint32_t *bitmap;    // total_slabs / 32 words, bit set = slab in use
+  uint32_t *counters;  // [0] slabs in use, [1] failed allocations
+  uint32_t nblocks;
+  uint32_t total_slabs;
+};
+
+struct GraphView {
+  uint32_t *heads;               // total_buckets * 64 words
+  uint32_t *pool;                // pool slab base
+  const uint32_t *bucket_off;    // n + 1
+  const uint32_t *bucket_owner;  // total_buckets
+  uint32_t *vdeg;                // live out-degree per vertex
+  uint32_t n;
+  uint32_t total_buckets;
+  uint32_t weighted;
+};
+
+#ifdef __HIP_DEVICE_COMPILE__
+#define MK_DEV __device__ __forceinline__
+#else
+#define MK_DEV __device__ inline
+#endif
+
+MK_DEV uint32_t lane_id() { return __lane_id(); }
+
+MK_DEV uint64_t ballot(bool p) { return __ballot(p); }
+
+MK_DEV uint32_t read_lane(uint32_t v, uint32_t l) {
+  return uint32_t(__builtin_amdgcn_readlane(int(v), int(l)));
+}
+MK_DEV uint32_t first_lane_val(uint32_t v) { return uint32_t(__builtin_amdgcn_readfirstlane(int(v))); }
+
+// Number of set bits of `mask` below this lane.
+MK_DEV uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+MK_DEV uint32_t ffs64(uint64_t m) { return uint32_t(__ffsll((unsigned long long)m) - 1); }
+
+// Loads of words that other waves may be writing in the same kernel: agent-
+// scope relaxed atomics, so a retry never re-reads a stale non-coherent copy.
+MK_DEV uint32_t coherent_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MK_DEV uint64_t coherent_load64(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MK_DEV void coherent_store(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+MK_DEV uint32_t *slab_ptr(uint32_t *pool, uint32_t idx) {
+  return pool + size_t(idx) * kSlabWords;
+}
+
+// Wave-aggregated append of the lanes with `pred` to queue[*count...].
+// Returns nothing; lanes with pred write their value.
+MK_DEV void wave_push(bool pred, uint32_t value, uint32_t *queue, uint32_t *count) {
+  uint64_t m = ballot(pred);
+  if (m == 0) return;
+  uint32_t base = 0;
+  uint32_t leader = ffs64(m);
+  if (lane_id() == leader) base = atomicAdd(count, uint32_t(__popcll(m)));
+  base = read_lane(base, leader);
+  if (pred) queue[base + lanes_below(m)] = value;
+}
+
+// Sum of a uint32 over the wave (all lanes must be active).
+MK_DEV uint32_t wave_sum_u32(uint32_t v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+MK_DEV double wave_sum_f64(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+MK_DEV float wave_sum_f32(float v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace mk
+
+// ---------------------------------------------------------------- host
+namespace mk {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+int device_cus(int device);
+
+#define MK_HIP(call)                                                      \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) return ::mk::hip_fail(e_, #call);               \
+  } while (0)
+
+// A grow-only device buffer reused across calls (no allocation per call).
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Pinned host scratch for small device->host reads.
+struct HostBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+}  // namespace mk
+
+struct mk_pool {
+  int device = 0;
+  mk::PoolView view{};
+  uint64_t total_slabs = 0;
+  int graphs = 0;
+};
+
+struct mk_graph {
+  mk_pool *pool = nullptr;
+  int device = 0;
+  uint32_t n = 0;
+  int weighted = 0;
+  double load_factor = 0.7;
+  uint32_t total_buckets = 0;
+  uint32_t *heads = nullptr;
+  uint32_t *bucket_off = nullptr;
+  uint32_t *bucket_owner = nullptr;
+  uint32_t *vdeg = nullptr;
+  uint32_t *counters = nullptr;  // [0] range errors [1] pool failures [2] applied
+  std::vector<uint32_t> bucket_off_host;
+  mk::DevBuf status;             // uint8 per edge of the last batch
+  uint32_t status_n = 0;
+  mk::HostBuf host_scratch;
+  int users = 0;                 // algorithm handles attached
+  mk::GraphView view() const {
+    mk::GraphView v;
+    v.heads = heads;
+    v.pool = pool->view.slabs;
+    v.bucket_off = bucket_off;
+    v.bucket_owner = bucket_owner;
+    v.vdeg = vdeg;
+    v.n = n;
+    v.total_buckets = total_buckets;
+    v.weighted = uint32_t(weighted);
+    return v;
+  }
+};
+
+// Launch-shape helpers.
+namespace mk {
+constexpr int kBlock = 256;  // 4 waves
+inline unsigned blocks_for_threads(uint64_t threads) {
+  uint64_t b = (threads + kBlock - 1) / kBlock;
+  if (b == 0) b = 1;
+  return unsigned(b);
+}
+inline unsigned blocks_for_waves(uint64_t waves) { return blocks_for_threads(waves * 64); }
+
+// Device-wide fill (used instead of hipMemsetAsync so callers may capture).
+void fill_u32(uint32_t *p, uint32_t v, uint64_t count, hipStream_t s);
+void fill_u64(uint64_t *p, uint64_t v, uint64_t count, hipStream_t s);
+void fill_u8(uint8_t *p, uint8_t v, uint64_t count, hipStream_t s);
+}  // namespace mk

@@ -1,0 +1,210 @@
+"""Host-side mirror of the reference's graph API over the C ABI.
+
+``SlabPool`` stands for ``AllocPolicy::DynamicAllocatorT`` and
+``DynamicGraph`` for ``DynamicGraph<EdgePolicy, IsWeighted>``
+(include/Graph/SlabGraph.cuh:343-506, 594-752): same construction
+arguments (vertex count, allocator, load factor, degree hints, device) and
+the same batched ``InsertEdges`` / ``DeleteEdges`` entry points, which take
+device arrays.  Device arrays are torch tensors (int32 or uint32 on the GPU);
+numpy arrays are copied to the device first.  torch is only the carrier of
+device memory here; all work runs in libmeerkat_hip.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+_vp = ctypes.c_void_p
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def device_u32(a, device=0):
+    """A device int32 tensor holding the bits of a uint32 array (or the tensor
+    itself if it already lives on the GPU)."""
+    torch = _torch()
+    if a is None:
+        return None
+    if isinstance(a, torch.Tensor):
+        if not a.is_cuda:
+            a = a.to(f"cuda:{device}")
+        if a.dtype not in (torch.int32, torch.uint32):
+            raise TypeError("device arrays must be int32/uint32")
+        return a.contiguous()
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return torch.from_numpy(a.view(np.int32)).to(f"cuda:{device}")
+
+
+def dptr(t):
+    return None if t is None else _vp(t.data_ptr())
+
+
+class SlabPool:
+    """HBM-resident pool of 256-byte slabs with a bitmap allocator."""
+
+    def __init__(self, nbytes=0, device=0):
+        self._h = _vp()
+        self.device = device
+        _lib.call("mk_pool_create", nbytes, device, ctypes.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stats(self):
+        used, total = ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.call("mk_pool_stats", self._h, ctypes.byref(used), ctypes.byref(total))
+        return used.value, total.value
+
+    def close(self):
+        if self._h:
+            _lib.call("mk_pool_destroy", self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().mk_pool_destroy(self._h)
+        except Exception:
+            pass
+
+
+class DynamicGraph:
+    """Per-vertex slab-hash adjacency: a set (unweighted) or a map from
+    destination to uint32 weight (weighted)."""
+
+    def __init__(self, n, pool, load_factor=0.7, degree_hints=None, weighted=False, device=0):
+        self.n = int(n)
+        self.pool = pool
+        self.weighted = bool(weighted)
+        self.device = device
+        self._h = _vp()
+        hints = None
+        if degree_hints is not None:
+            hints = np.ascontiguousarray(degree_hints, dtype=np.uint32)
+            if len(hints) != self.n:
+                raise ValueError("degree_hints must have n entries")
+        self._hints = hints
+        _lib.call("mk_graph_create", pool.handle, self.n, int(self.weighted), float(load_factor),
+                  None if hints is None else hints.ctypes.data_as(_vp), ctypes.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---------------------------------------------------------- mutation
+    def InsertEdges(self, src, dst, weights=None, stream=None):
+        s, d = device_u32(src, self.device), device_u32(dst, self.device)
+        w = device_u32(weights, self.device)
+        _lib.call("mk_graph_insert", self._h, dptr(s), dptr(d), dptr(w), len(s), stream)
+        return s, d, w
+
+    def DeleteEdges(self, src, dst, stream=None):
+        s, d = device_u32(src, self.device), device_u32(dst, self.device)
+        _lib.call("mk_graph_delete", self._h, dptr(s), dptr(d), len(s), stream)
+        return s, d
+
+    insert_edges = InsertEdges
+    delete_edges = DeleteEdges
+
+    def last_status(self):
+        """MK_EDGE_* code of every edge of the last insert/delete, as numpy."""
+        p, n = _vp(), ctypes.c_uint32()
+        _lib.call("mk_graph_last_status", self._h, ctypes.byref(p), ctypes.byref(n))
+        host = np.empty(n.value, np.uint8)
+        if n.value:
+            _copy_d2h(host, p.value, n.value)
+        return host
+
+    def SearchEdges(self, src, dst):
+        torch = _torch()
+        s, d = device_u32(src, self.device), device_u32(dst, self.device)
+        found = torch.empty(len(s), dtype=torch.uint8, device=s.device)
+        w = torch.empty(len(s), dtype=torch.int32, device=s.device) if self.weighted else None
+        _lib.call("mk_graph_search", self._h, dptr(s), dptr(d), len(s), dptr(found), dptr(w),
+                  None)
+        f = found.cpu().numpy().astype(bool)
+        return (f, w.cpu().numpy().view(np.uint32)) if self.weighted else f
+
+    # ---------------------------------------------------------- queries
+    def num_edges(self):
+        m = ctypes.c_uint64()
+        _lib.call("mk_graph_num_edges", self._h, ctypes.byref(m))
+        return m.value
+
+    def degrees(self):
+        out = np.empty(self.n, np.uint32)
+        _lib.call("mk_graph_degrees", self._h, out.ctypes.data_as(_vp))
+        return out
+
+    def GetVertexBucketsOffsets(self):
+        out = np.empty(self.n + 1, np.uint32)
+        _lib.call("mk_graph_bucket_offsets", self._h, out.ctypes.data_as(_vp))
+        return out
+
+    def GetBucketsPerVertex(self):
+        return np.diff(self.GetVertexBucketsOffsets())
+
+    def count_slots(self):
+        live = np.empty(self.n, np.uint32)
+        tomb = np.empty(self.n, np.uint32)
+        _lib.call("mk_graph_count_slots", self._h, live.ctypes.data_as(_vp),
+                  tomb.ctypes.data_as(_vp))
+        return live, tomb
+
+    def export_csr(self):
+        """(row_off uint64[n+1], col uint32[m], w uint32[m] or None), each row
+        sorted by destination."""
+        row = np.empty(self.n + 1, np.uint64)
+        _lib.call("mk_graph_export_csr", self._h, row.ctypes.data_as(_vp), None, None)
+        m = int(row[-1])
+        col = np.empty(max(m, 1), np.uint32)
+        w = np.empty(max(m, 1), np.uint32) if self.weighted else None
+        _lib.call("mk_graph_export_csr", self._h, row.ctypes.data_as(_vp),
+                  col.ctypes.data_as(_vp), None if w is None else w.ctypes.data_as(_vp))
+        col, w = col[:m], (None if w is None else w[:m])
+        # rows come out in slab order; sort each row by destination
+        owner = np.repeat(np.arange(self.n, dtype=np.uint64), np.diff(row).astype(np.int64))
+        order = np.lexsort((col, owner))
+        col = col[order]
+        if w is not None:
+            w = w[order]
+        return row, col, w
+
+    def slab_stats(self):
+        h, o, mx = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+        _lib.call("mk_graph_slab_stats", self._h, ctypes.byref(h), ctypes.byref(o),
+                  ctypes.byref(mx))
+        return {"head_slabs": h.value, "overflow_slabs": o.value, "max_chain": mx.value}
+
+    def close(self):
+        if self._h:
+            _lib.call("mk_graph_destroy", self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().mk_graph_destroy(self._h)
+        except Exception:
+            pass
+
+
+def _copy_d2h(host, dev_addr, nbytes):
+    """Copy nbytes of device memory at dev_addr into a numpy array."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemcpy.restype = ctypes.c_int
+    rc = _hip.hipMemcpy(host.ctypes.data_as(_vp), _vp(dev_addr), nbytes, 2)  # device->host
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+
+
+_hip = None

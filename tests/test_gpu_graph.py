@@ -1,0 +1,181 @@
+"""GPU parity of the slab pool and slab-hash adjacency against the oracle.
+
+Restates the reference's test/GraphTest.cu cases (DegreeCountTest :647-714,
+ElementTest :716-871, PartialDelete :873-956, FullDelete :958-1007,
+WGraphTest :1111-1215) on seeded generated graphs, plus the semantics the
+reference leaves to its absent SlabHash submodule: duplicate suppression
+under concurrency, tombstone reuse, pool exhaustion and range errors.
+"""
+import numpy as np
+import pytest
+
+import meerkat_amd as mk
+import oracle
+from meerkat_amd import _lib, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def build(n, s, d, w=None, hints=True, weighted=False, pool_bytes=1 << 28, lf=0.7):
+    pool = mk.SlabPool(pool_bytes)
+    h = gen.degree_hints(n, s) if hints else None
+    g = mk.DynamicGraph(n, pool, lf, h, weighted=weighted)
+    if len(s):
+        g.InsertEdges(s, d, w)
+    return pool, g
+
+
+def assert_same_adjacency(g, ref):
+    row, col, w = g.export_csr()
+    assert np.array_equal(row, ref.row_off)
+    assert np.array_equal(col, ref.col)
+    if ref.w is not None:
+        assert np.array_equal(w, ref.w)
+
+
+@pytest.fixture(scope="module")
+def rmat12(gpu):
+    n, s, d, _ = gen.rmat(12, 16, seed=1)
+    return n, s, d
+
+
+def test_degree_and_elements_with_duplicates_and_self_loops(gpu):
+    # raw numpy-drawn edges with duplicates and self-loops (golden fixture)
+    from conftest import load_golden
+
+    z = load_golden("dense_directed")
+    n = int(z["n"])
+    ref = oracle.csr(n, z["raw_src"], z["raw_dst"])
+    pool, g = build(n, z["raw_src"], z["raw_dst"], hints=False)
+    live, tomb = g.count_slots()
+    assert np.array_equal(live, ref.degrees())  # DegreeCountTest
+    assert np.array_equal(g.degrees(), ref.degrees())
+    assert tomb.sum() == 0
+    assert_same_adjacency(g, ref)  # ElementTest
+    st = g.last_status()
+    self_loops = z["raw_src"] == z["raw_dst"]
+    assert (st[self_loops] == _lib.EDGE_SELF_LOOP).all()
+    assert (st == _lib.EDGE_APPLIED).sum() == ref.m
+
+
+@pytest.mark.parametrize("hints", [True, False])
+def test_partial_then_full_delete(rmat12, hints):
+    n, s, d = rmat12
+    ref = oracle.csr(n, s, d)
+    pool, g = build(n, s, d, hints=hints)
+    assert_same_adjacency(g, ref)
+    rng = np.random.default_rng(45)
+    mark = rng.random(len(s)) < 0.45  # GraphTest.cu:520-529 marks ~45%
+    g.DeleteEdges(s[mark], d[mark])
+    assert (g.last_status() == _lib.EDGE_APPLIED).all()
+    rest = oracle.csr(n, s[~mark], d[~mark])
+    assert_same_adjacency(g, rest)  # PartialDelete
+    assert np.array_equal(g.degrees(), rest.degrees())
+    g.DeleteEdges(s[~mark], d[~mark])
+    live, tomb = g.count_slots()  # FullDelete: tombstones == original degree
+    assert live.sum() == 0
+    assert np.array_equal(tomb, ref.degrees())
+    assert g.num_edges() == 0
+    # deleting again changes nothing
+    g.DeleteEdges(s[:100], d[:100])
+    assert (g.last_status() == _lib.EDGE_UNCHANGED).all()
+
+
+def test_tombstones_are_reused(rmat12):
+    n, s, d = rmat12
+    pool, g = build(n, s, d, hints=False)
+    before = g.slab_stats()
+    used_before, _ = pool.stats()
+    g.DeleteEdges(s, d)
+    g.InsertEdges(s, d)
+    assert g.slab_stats()["overflow_slabs"] == before["overflow_slabs"]
+    assert pool.stats()[0] == used_before
+    assert_same_adjacency(g, oracle.csr(n, s, d))
+
+
+def test_concurrent_duplicates_inserted_once(gpu):
+    n, s, d, _ = gen.rmat(11, 16, seed=9)
+    reps = 8
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(len(s) * reps)
+    S = np.tile(s, reps)[perm]
+    D = np.tile(d, reps)[perm]
+    for hints in (True, False):
+        pool, g = build(n, S, D, hints=hints)
+        assert (g.last_status() == _lib.EDGE_APPLIED).sum() == len(s)
+        assert np.array_equal(g.degrees(), oracle.csr(n, s, d).degrees())
+        assert_same_adjacency(g, oracle.csr(n, s, d))
+
+
+def test_weighted_pairs_and_keep_first_weight(gpu):
+    n, s, d, w = gen.rmat(11, 16, seed=3, weighted=True)
+    ref = oracle.csr(n, s, d, w)
+    pool, g = build(n, s, d, w, weighted=True)
+    assert_same_adjacency(g, ref)  # WGraphTest.ElementTest
+    live, _ = g.count_slots()
+    assert np.array_equal(live, ref.degrees())  # WGraphTest.DegreeCountTest
+    # re-inserting existing edges with other weights changes nothing
+    g.InsertEdges(s[:500], d[:500], (w[:500] % 7) + 200)
+    assert (g.last_status() == _lib.EDGE_UNCHANGED).all()
+    assert_same_adjacency(g, ref)
+    found, ww = g.SearchEdges(s[:1000], d[:1000])
+    assert found.all() and np.array_equal(ww, w[:1000])
+    missing = gen.insert_batch(gen.keys_of(s, d), n, 300, seed=4)
+    found, ww = g.SearchEdges(missing[0], missing[1])
+    assert not found.any() and (ww == 0xFFFFFFFF).all()
+
+
+def test_long_chains_hub_without_hints(gpu):
+    n = 20000
+    hub = np.zeros(15000, np.uint32)
+    nbr = np.arange(1, 15001, dtype=np.uint32)
+    pool, g = build(n, hub, nbr, hints=False)
+    st = g.slab_stats()
+    assert st["max_chain"] == int(np.ceil(15000 / 63))
+    assert pool.stats()[0] == st["overflow_slabs"]
+    assert_same_adjacency(g, oracle.csr(n, hub, nbr))
+    g.close()
+    assert pool.stats()[0] == 0  # destroy returns every overflow slab
+
+
+def test_pool_exhaustion_leaves_edge_set_unchanged(gpu):
+    n = 200000
+    pool, g = build(n, np.zeros(0, np.uint32), np.zeros(0, np.uint32), hints=False,
+                    pool_bytes=2048 * 256)
+    first = np.arange(1, 1001, dtype=np.uint32)
+    g.InsertEdges(np.zeros(1000, np.uint32), first)
+    before = g.export_csr()
+    hub = np.zeros(150000, np.uint32)
+    nbr = np.arange(1, 150001, dtype=np.uint32)
+    with pytest.raises(mk.MeerkatError) as ei:
+        g.InsertEdges(hub, nbr)
+    assert ei.value.code == _lib.MK_ERR_POOL_FULL
+    after = g.export_csr()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert g.num_edges() == 1000
+
+
+def test_out_of_range_batch_rejected(rmat12):
+    n, s, d = rmat12
+    pool, g = build(n, s, d)
+    bad_s = np.array([1, 2, n], np.uint32)
+    bad_d = np.array([3, 4, 5], np.uint32)
+    with pytest.raises(mk.MeerkatError) as ei:
+        g.InsertEdges(bad_s, bad_d)
+    assert ei.value.code == _lib.MK_ERR_RANGE
+    with pytest.raises(mk.MeerkatError) as ei:
+        g.DeleteEdges(bad_d, bad_s)
+    assert ei.value.code == _lib.MK_ERR_RANGE
+    assert_same_adjacency(g, oracle.csr(n, s, d))
+
+
+def test_empty_batches_and_handle_order(gpu):
+    pool, g = build(100, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    g.InsertEdges(np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    g.DeleteEdges(np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    assert g.num_edges() == 0
+    with pytest.raises(mk.MeerkatError) as ei:
+        pool.close()  # a graph still uses it
+    assert ei.value.code == _lib.MK_ERR_STATE
+    g.close()
+    pool.close()

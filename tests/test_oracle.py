@@ -1,0 +1,272 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked against something that does not share its
+code: the golden fixtures (scipy / dense linear algebra on numpy-drawn
+graphs), scipy.sparse.csgraph on generated graphs, closed forms, and brute
+force on tiny graphs.  The mutation tests at the bottom show the checks are
+sharp: a plausible mistake planted in an oracle result (on the CPU) fails
+them.
+"""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as cg
+
+import oracle
+from conftest import GOLDEN_NAMES, load_golden
+
+INF = 0xFFFFFFFF
+
+
+def sym(src, dst):
+    return np.concatenate([src, dst]), np.concatenate([dst, src])
+
+
+# ------------------------------------------------------------------ golden
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_csr_build_matches_golden_dedup(name):
+    z = load_golden(name)
+    n = int(z["n"])
+    g = oracle.csr(n, z["raw_src"], z["raw_dst"], z["raw_w"])
+    expect = np.unique(z["src"].astype(np.uint64) << np.uint64(32) | z["dst"].astype(np.uint64))
+    assert np.array_equal(g.keys(), expect)
+    # first occurrence's weight kept for duplicates
+    order = np.argsort(z["src"].astype(np.uint64) << np.uint64(32) | z["dst"].astype(np.uint64))
+    assert np.array_equal(g.w, z["w"][order])
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_bfs_golden(name):
+    z = load_golden(name)
+    g = oracle.csr(int(z["n"]), z["src"], z["dst"])
+    src = int(z["source"])
+    assert np.array_equal(oracle.bfs(g, src), z["bfs"])
+    assert np.array_equal(oracle.bfs_par(g, src, threads=4), z["bfs"])
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_dijkstra_golden(name):
+    z = load_golden(name)
+    g = oracle.csr(int(z["n"]), z["src"], z["dst"], z["w"])
+    assert np.array_equal(oracle.dijkstra(g, int(z["source"])), z["sssp"])
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_pagerank_golden(name):
+    z = load_golden(name)
+    n = int(z["n"])
+    g = oracle.csr(n, z["src"], z["dst"])
+    pr, it, delta = oracle.pagerank(oracle.transpose(g), g.degrees(), 0.85, 1e-15, 10000)
+    assert delta <= 1e-15
+    assert np.abs(pr - z["pagerank"]).sum() < 1e-12
+    assert abs(pr.sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_tc_golden(name):
+    z = load_golden(name)
+    n = int(z["n"])
+    s, d = sym(z["src"], z["dst"])
+    g = oracle.csr(n, s, d)
+    assert oracle.tc(g) == int(z["tc"])
+    assert oracle.tc(g, threads=3) == int(z["tc"])
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_cc_golden(name):
+    z = load_golden(name)
+    n = int(z["n"])
+    g = oracle.csr(n, z["src"], z["dst"])
+    labels, k = oracle.cc(g)
+    assert np.array_equal(labels, z["cc"])
+    assert k == int(z["cc_count"])
+
+
+# ------------------------------------------------------------------ scipy on generated graphs
+def rmat_graph(scale, seed, weighted=False, undirected=False, ef=8):
+    import meerkat_amd.gen as gen
+
+    return gen.rmat(scale, ef, seed=seed, weighted=weighted, undirected=undirected)
+
+
+def test_bfs_vs_scipy_rmat():
+    n, s, d, _ = rmat_graph(12, 1)
+    g = oracle.csr(n, s, d)
+    A = sp.csr_matrix((np.ones(g.m), g.col, g.row_off.astype(np.int64)), shape=(n, n))
+    ref = cg.shortest_path(A, unweighted=True, indices=0)
+    ref = np.where(np.isinf(ref), INF, ref).astype(np.uint32)
+    assert np.array_equal(oracle.bfs(g, 0), ref)
+    assert np.array_equal(oracle.bfs_par(g, 0), ref)
+
+
+def test_dijkstra_vs_scipy_rmat():
+    n, s, d, w = rmat_graph(12, 3, weighted=True)
+    g = oracle.csr(n, s, d, w)
+    A = sp.csr_matrix((g.w.astype(np.float64), g.col, g.row_off.astype(np.int64)), shape=(n, n))
+    ref = cg.dijkstra(A, indices=0)
+    ref = np.where(np.isinf(ref), INF, ref).astype(np.uint32)
+    assert np.array_equal(oracle.dijkstra(g, 0), ref)
+
+
+def test_cc_vs_scipy_rmat():
+    n, s, d, _ = rmat_graph(12, 7, undirected=True, ef=2)
+    g = oracle.csr(n, s, d)
+    labels, k = oracle.cc(g)
+    A = sp.csr_matrix((np.ones(g.m), g.col, g.row_off.astype(np.int64)), shape=(n, n))
+    k2, lab = cg.connected_components(A, directed=False)
+    assert k == k2
+    # same partition, and every label is the component minimum
+    for c in range(k2):
+        members = np.nonzero(lab == c)[0]
+        assert (labels[members] == members.min()).all()
+
+
+def test_tc_vs_trace_rmat():
+    n, s, d, _ = rmat_graph(9, 5, undirected=True)
+    g = oracle.csr(n, s, d)
+    A = np.zeros((n, n))
+    A[np.repeat(np.arange(n), np.diff(g.row_off).astype(np.int64)), g.col] = 1
+    assert oracle.tc(g) == int(round(np.trace(A @ A @ A) / 6))
+
+
+# ------------------------------------------------------------------ closed forms / brute force
+def test_bfs_path_and_star():
+    n = 50
+    g = oracle.csr(n, np.arange(n - 1), np.arange(1, n))
+    assert np.array_equal(oracle.bfs(g, 0), np.arange(n, dtype=np.uint32))
+    d = oracle.bfs(g, 10)
+    assert (d[:10] == INF).all() and np.array_equal(d[10:], np.arange(n - 10))
+    star = oracle.csr(n, np.zeros(n - 1, np.uint32), np.arange(1, n))
+    assert np.array_equal(oracle.bfs(star, 0), np.r_[0, np.ones(n - 1)].astype(np.uint32))
+
+
+def test_dijkstra_brute_force_tiny():
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        n = 7
+        m = 14
+        s = rng.integers(0, n, m).astype(np.uint32)
+        d = rng.integers(0, n, m).astype(np.uint32)
+        w = rng.integers(1, 9, m).astype(np.uint32)
+        g = oracle.csr(n, s, d, w)
+        # Floyd-Warshall
+        D = np.full((n, n), np.inf)
+        np.fill_diagonal(D, 0)
+        for u in range(n):
+            for j in range(int(g.row_off[u]), int(g.row_off[u + 1])):
+                D[u, g.col[j]] = min(D[u, g.col[j]], g.w[j])
+        for k in range(n):
+            D = np.minimum(D, D[:, [k]] + D[[k], :])
+        ref = np.where(np.isinf(D[0]), INF, D[0]).astype(np.uint32)
+        assert np.array_equal(oracle.dijkstra(g, 0), ref), trial
+
+
+def test_pagerank_cycle_uniform_and_all_dangling():
+    n = 37
+    g = oracle.csr(n, np.arange(n), (np.arange(n) + 1) % n)
+    pr, _, _ = oracle.pagerank(oracle.transpose(g), g.degrees(), 0.85, 1e-15, 1000)
+    assert np.allclose(pr, 1.0 / n, rtol=0, atol=1e-15)
+    empty = oracle.csr(n, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    pr, it, _ = oracle.pagerank(oracle.transpose(empty), empty.degrees(), 0.85, 1e-15, 1000)
+    assert np.allclose(pr, 1.0 / n, atol=1e-16) and it == 1
+
+
+def test_pagerank_two_node_closed_form():
+    # 0 -> 1 only: vertex 1 dangling.  Fixed point by hand:
+    # x0 = (1-d)/2 + d*x1/2 ; x1 = (1-d)/2 + d*x0 + d*x1/2
+    d = 0.85
+    g = oracle.csr(2, np.array([0], np.uint32), np.array([1], np.uint32))
+    pr, _, _ = oracle.pagerank(oracle.transpose(g), g.degrees(), d, 1e-16, 10000)
+    A = np.array([[1.0, -d / 2], [-d, 1 - d / 2]])
+    x = np.linalg.solve(A, np.full(2, (1 - d) / 2))
+    assert np.allclose(pr, x, atol=1e-15)
+
+
+def test_pagerank_stopping_rule_matches_reference_loop():
+    # the reference loop is `while (Delta > ErrorMargin && Iterations < MaxIter)`
+    z = load_golden("dense_directed")
+    g = oracle.csr(int(z["n"]), z["src"], z["dst"])
+    _, it, delta = oracle.pagerank(oracle.transpose(g), g.degrees(), 0.85, 1e-3, 100)
+    _, it2, delta2 = oracle.pagerank(oracle.transpose(g), g.degrees(), 0.85, 1e-3, it - 1)
+    assert delta <= 1e-3 and delta2 > 1e-3
+    _, it3, _ = oracle.pagerank(oracle.transpose(g), g.degrees(), 0.85, 0.0, 5)
+    assert it3 == 5
+
+
+def test_tc_complete_and_brute_force():
+    n = 9
+    pairs = [(a, b) for a in range(n) for b in range(n) if a != b]
+    s, d = np.array(pairs, np.uint32).T
+    assert oracle.tc(oracle.csr(n, s, d)) == 84  # C(9,3)
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        n = 12
+        m = 30
+        s = rng.integers(0, n, m).astype(np.uint32)
+        d = rng.integers(0, n, m).astype(np.uint32)
+        ss, dd = sym(s, d)
+        g = oracle.csr(n, ss, dd)
+        E = set(zip(g.keys() >> np.uint64(32), g.keys() & np.uint64(0xFFFFFFFF)))
+        E = {(int(a), int(b)) for a, b in E}
+        brute = sum(1 for a, b, c in itertools.combinations(range(n), 3)
+                    if (a, b) in E and (b, c) in E and (a, c) in E)
+        assert oracle.tc(g) == brute
+
+
+def test_cc_brute_force_and_min_labels():
+    rng = np.random.default_rng(9)
+    for _ in range(10):
+        n = 30
+        m = 20
+        s = rng.integers(0, n, m).astype(np.uint32)
+        d = rng.integers(0, n, m).astype(np.uint32)
+        g = oracle.csr(n, s, d)
+        labels, k = oracle.cc(g)
+        # flood fill on the undirected version
+        adj = [set() for _ in range(n)]
+        for a, b in zip(s, d):
+            adj[a].add(int(b))
+            adj[b].add(int(a))
+        seen = [-1] * n
+        comps = 0
+        for v in range(n):
+            if seen[v] >= 0:
+                continue
+            comps += 1
+            stack = [v]
+            seen[v] = v
+            while stack:
+                x = stack.pop()
+                for y in adj[x]:
+                    if seen[y] < 0:
+                        seen[y] = v
+                        stack.append(y)
+        assert k == comps
+        assert np.array_equal(labels, np.array(seen, np.uint32))
+
+
+# ------------------------------------------------------------------ mutation checks
+def test_planted_mistakes_are_caught():
+    z = load_golden("dense_directed")
+    n = int(z["n"])
+    g = oracle.csr(n, z["src"], z["dst"], z["w"])
+    # PageRank without the dangling term, or with the wrong damping, misses
+    # the fixed point by far more than the 1e-6 parity tolerance
+    gin = oracle.transpose(g)
+    z2 = load_golden("sparse_directed")
+    g2 = oracle.csr(int(z2["n"]), z2["src"], z2["dst"])
+    deg2 = g2.degrees()
+    pr_wrong_d, _, _ = oracle.pagerank(oracle.transpose(g2), deg2, 0.80, 1e-15, 10000)
+    assert np.abs(pr_wrong_d - z2["pagerank"]).sum() > 1e-3
+    fake_deg = deg2.copy()
+    fake_deg[fake_deg == 0] = 1  # drops the dangling redistribution
+    pr_no_dangling, _, _ = oracle.pagerank(oracle.transpose(g2), fake_deg, 0.85, 1e-15, 10000)
+    assert np.abs(pr_no_dangling - z2["pagerank"]).sum() > 1e-3
+    # an off-by-one level or a unit weight instead of the edge weight differ
+    lv = oracle.bfs(oracle.csr(n, z["src"], z["dst"]), int(z["source"]))
+    assert not np.array_equal(np.where(lv == INF, lv, lv + 1), z["bfs"])
+    gunit = oracle.csr(n, z["src"], z["dst"], np.ones(len(z["src"]), np.uint32))
+    assert not np.array_equal(oracle.dijkstra(gunit, int(z["source"])), z["sssp"])
+    del gin
