@@ -142,8 +142,10 @@ int mk_bfs_incremental(mk_bfs *b, const uint32_t *src_dev, const uint32_t *dst_d
 int mk_bfs_decremental(mk_bfs *b, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
                        float *ms);
 int mk_bfs_read(mk_bfs *b, uint32_t *dist_host, uint32_t *parent_host);
-/* Frontier iterations and adjacency slabs read by the last compute call. */
-int mk_bfs_last_stats(mk_bfs *b, uint32_t *iterations, uint64_t *slabs_read);
+/* Frontier iterations, adjacency slabs and live keys (edges) read by the
+ * last compute call; edges/ms gives its traversed-edges rate. */
+int mk_bfs_last_stats(mk_bfs *b, uint32_t *iterations, uint64_t *slabs_read,
+                      uint64_t *edges_read);
 
 /* ------------------------------------------------------ A11 SSSP
  * SSSP(G, Src, N) (include/Algorithms/SSSP.cuh:34-69; impl/SSSPImpl.cuh
@@ -159,7 +161,20 @@ int mk_sssp_incremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst
 int mk_sssp_decremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst_dev,
                         const uint32_t *w_dev, uint32_t n, float *ms);
 int mk_sssp_read(mk_sssp *s, uint32_t *dist_host, uint32_t *parent_host);
-int mk_sssp_last_stats(mk_sssp *s, uint32_t *iterations, uint64_t *slabs_read);
+int mk_sssp_last_stats(mk_sssp *s, uint32_t *iterations, uint64_t *slabs_read,
+                       uint64_t *edges_read);
+
+/* ------------------------------------------------------ kernel profiling
+ * No reference counterpart (the reference times whole calls with
+ * cudaEvent, BFSImpl.cuh:644-661).  When enabled, every instrumented kernel
+ * launch is bracketed by HIP events on its own stream; totals (launches,
+ * milliseconds, algorithmic bytes) accumulate per kernel name until reset.
+ * mk_profile_count snapshots the names; mk_profile_get reads entry i. */
+int mk_profile_enable(int on);
+int mk_profile_reset(void);
+int mk_profile_count(int *count);
+int mk_profile_get(int i, const char **name, uint64_t *launches, double *total_ms,
+                   double *algo_bytes);
 
 /* ------------------------------------------------------ workload generators
  * Host-side, seeded, thread-count independent.  They replace the text edge

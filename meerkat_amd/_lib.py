@@ -79,14 +79,18 @@ _PROTOS = {
     "mk_bfs_incremental": [vp, vp, vp, u32, P(f32)],
     "mk_bfs_decremental": [vp, vp, vp, u32, P(f32)],
     "mk_bfs_read": [vp, vp, vp],
-    "mk_bfs_last_stats": [vp, P(u32), P(u64)],
+    "mk_bfs_last_stats": [vp, P(u32), P(u64), P(u64)],
     "mk_sssp_create": [vp, u32, P(vp)],
     "mk_sssp_destroy": [vp],
     "mk_sssp_static": [vp, P(f32)],
     "mk_sssp_incremental": [vp, vp, vp, vp, u32, P(f32)],
     "mk_sssp_decremental": [vp, vp, vp, vp, u32, P(f32)],
     "mk_sssp_read": [vp, vp, vp],
-    "mk_sssp_last_stats": [vp, P(u32), P(u64)],
+    "mk_sssp_last_stats": [vp, P(u32), P(u64), P(u64)],
+    "mk_profile_enable": [i32],
+    "mk_profile_reset": [],
+    "mk_profile_count": [P(i32)],
+    "mk_profile_get": [i32, P(ctypes.c_char_p), P(u64), P(f64), P(f64)],
     "mk_gen_rmat": [u32, u32, f64, f64, f64, u64, i32, P(u64)],
     "mk_gen_er": [u32, u64, u64, P(u64)],
     "mk_gen_fetch": [vp, vp, vp, u64],

@@ -45,9 +45,10 @@ class _Traversal:
         return self.read()[1]
 
     def last_stats(self):
-        it, slabs = ctypes.c_uint32(), ctypes.c_uint64()
-        _lib.call(f"{self._prefix}_last_stats", self._h, ctypes.byref(it), ctypes.byref(slabs))
-        return {"iterations": it.value, "slabs_read": slabs.value}
+        it, slabs, edges = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.call(f"{self._prefix}_last_stats", self._h, ctypes.byref(it), ctypes.byref(slabs),
+                  ctypes.byref(edges))
+        return {"iterations": it.value, "slabs_read": slabs.value, "edges_read": edges.value}
 
     def close(self):
         if self._h:

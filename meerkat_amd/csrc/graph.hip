@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 
+#include "profile.hpp"
 #include "slabhash.hpp"
 
 namespace mk {
@@ -506,13 +507,18 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
   uint8_t *st = g->status.as<uint8_t>();
-  if (g->weighted)
-    k_insert<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src, dst,
-                                                              w, n, epw, st, g->counters);
-  else
-    k_insert<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
-                                                               dst, nullptr, n, epw, st,
-                                                               g->counters);
+  {
+    // per edge: batch entry, one slab probe, key (+value) write, degree
+    // atomic, status byte
+    ProfScope prof(s, "k_insert", double(n) * (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1));
+    if (g->weighted)
+      k_insert<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
+                                                                dst, w, n, epw, st, g->counters);
+    else
+      k_insert<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
+                                                                 dst, nullptr, n, epw, st,
+                                                                 g->counters);
+  }
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
   MK_HIP(hipStreamSynchronize(s));
@@ -541,12 +547,17 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
   uint8_t *st = g->status.as<uint8_t>();
-  if (g->weighted)
-    k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                              0xFFu, g->counters);
-  else
-    k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                               0xFFu, g->counters);
+  {
+    // per edge: batch entry, one slab probe, tombstone write, degree atomic,
+    // status byte
+    ProfScope prof(s, "k_delete", double(n) * (8 + 256 + 4 + 8 + 1));
+    if (g->weighted)
+      k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                                0xFFu, g->counters);
+    else
+      k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                                 0xFFu, g->counters);
+  }
   MK_HIP(hipGetLastError());
   MK_HIP(hipStreamSynchronize(s));
   return MK_OK;

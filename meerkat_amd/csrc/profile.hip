@@ -1,0 +1,134 @@
+// Kernel timing registry behind mk_profile_* (see profile.hpp).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mk_internal.hpp"
+#include "profile.hpp"
+
+namespace mk {
+namespace {
+
+struct Pending {
+  hipEvent_t a, b;
+  const char *name;
+  double bytes;
+};
+struct Total {
+  uint64_t launches = 0;
+  double ms = 0.0;
+  double bytes = 0.0;
+};
+
+std::atomic<bool> g_on{false};
+std::mutex g_mu;
+std::vector<Pending> g_pending;
+std::vector<hipEvent_t> g_free;
+std::map<std::string, Total> g_totals;
+std::vector<std::string> g_names;  // stable storage for returned names
+
+hipEvent_t take_event() {
+  if (!g_free.empty()) {
+    hipEvent_t e = g_free.back();
+    g_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void resolve_locked() {
+  for (auto &p : g_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      Total &t = g_totals[p.name];
+      t.launches++;
+      t.ms += ms;
+      t.bytes += p.bytes;
+    }
+    g_free.push_back(p.a);
+    g_free.push_back(p.b);
+  }
+  g_pending.clear();
+}
+
+}  // namespace
+
+bool profiling_on() { return g_on.load(std::memory_order_relaxed); }
+
+hipEvent_t prof_begin(hipStream_t s) {
+  if (!profiling_on()) return nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipEvent_t e = take_event();
+  if (e && hipEventRecord(e, s) != hipSuccess) {
+    g_free.push_back(e);
+    return nullptr;
+  }
+  return e;
+}
+
+void prof_end(hipEvent_t start, hipStream_t s, const char *name, double algo_bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipEvent_t e = take_event();
+  if (!e || hipEventRecord(e, s) != hipSuccess) {
+    g_free.push_back(start);
+    if (e) g_free.push_back(e);
+    return;
+  }
+  g_pending.push_back({start, e, name, algo_bytes});
+  if (g_pending.size() > 4096) resolve_locked();
+}
+
+void prof_add_bytes(const char *name, double algo_bytes) {
+  if (!profiling_on()) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_totals[name].bytes += algo_bytes;
+}
+
+}  // namespace mk
+
+using namespace mk;
+
+extern "C" {
+
+int mk_profile_enable(int on) {
+  g_on.store(on != 0);
+  return MK_OK;
+}
+
+int mk_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  resolve_locked();
+  g_totals.clear();
+  return MK_OK;
+}
+
+int mk_profile_count(int *count) {
+  if (!count) return fail(MK_ERR_BAD_ARG, "mk_profile_count: NULL count");
+  std::lock_guard<std::mutex> lk(g_mu);
+  resolve_locked();
+  g_names.clear();
+  for (auto &kv : g_totals) g_names.push_back(kv.first);
+  *count = int(g_names.size());
+  return MK_OK;
+}
+
+int mk_profile_get(int i, const char **name, uint64_t *launches, double *total_ms,
+                   double *algo_bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (i < 0 || size_t(i) >= g_names.size())
+    return fail(MK_ERR_BAD_ARG, "mk_profile_get: index out of range (call mk_profile_count)");
+  const Total &t = g_totals[g_names[i]];
+  if (name) *name = g_names[i].c_str();
+  if (launches) *launches = t.launches;
+  if (total_ms) *total_ms = t.ms;
+  if (algo_bytes) *algo_bytes = t.bytes;
+  return MK_OK;
+}
+
+}  // extern "C"

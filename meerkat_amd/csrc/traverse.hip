@@ -32,6 +32,7 @@
 #include <cstring>
 #include <vector>
 
+#include "profile.hpp"
 #include "slabhash.hpp"
 
 namespace mk {
@@ -47,7 +48,8 @@ struct TravView {
   uint2 *q_large[2];
   uint32_t *cnt_small;             // [2]
   unsigned long long *cnt_large;   // [2]: (entries << 32) | bucket items
-  unsigned long long *slabs_read;  // statistics
+  unsigned long long *slabs_read;  // statistics: slabs and live keys read
+  unsigned long long *edges_read;
 };
 
 // Append v to the next frontier `nx` (lanes with `push`).
@@ -65,7 +67,7 @@ MK_DEV void enqueue(const GraphView &G, const TravView &S, uint32_t nx, bool pus
 // Relax every edge of one bucket chain of u (distance du).
 template <bool MAP, bool WEIGHTS>
 MK_DEV uint32_t relax_bucket(const GraphView &G, const TravView &S, uint32_t nx, uint32_t epoch,
-                             uint32_t u, uint32_t du, uint32_t b) {
+                             uint32_t u, uint32_t du, uint32_t b, uint32_t &edges) {
   const uint32_t lane = lane_id();
   const uint64_t km = key_mask<MAP>();
   const bool key_lane = (km >> lane) & 1ull;
@@ -74,6 +76,7 @@ MK_DEV uint32_t relax_bucket(const GraphView &G, const TravView &S, uint32_t nx,
   for (;;) {
     const uint32_t v = slab[lane];
     ++slabs;
+    edges += uint32_t(__popcll(ballot(key_lane && v < kTomb)));
     const uint32_t wv = MAP ? __shfl_down(v, 1, 64) : 1u;
     bool push = false;
     if (key_lane && v < kTomb) {
@@ -96,7 +99,7 @@ MK_DEV uint32_t relax_bucket(const GraphView &G, const TravView &S, uint32_t nx,
 // out-neighbours whose packed parent is x.
 template <bool MAP>
 MK_DEV uint32_t invalidate_bucket(const GraphView &G, const TravView &S, uint32_t nx, uint32_t x,
-                                  uint32_t b) {
+                                  uint32_t b, uint32_t &edges) {
   const uint32_t lane = lane_id();
   const uint64_t km = key_mask<MAP>();
   const bool key_lane = (km >> lane) & 1ull;
@@ -105,6 +108,7 @@ MK_DEV uint32_t invalidate_bucket(const GraphView &G, const TravView &S, uint32_
   for (;;) {
     const uint32_t v = slab[lane];
     ++slabs;
+    edges += uint32_t(__popcll(ballot(key_lane && v < kTomb)));
     bool push = false;
     if (key_lane && v < kTomb) {
       const uint64_t cur = coherent_load64(S.dist + v);
@@ -139,7 +143,7 @@ __global__ void __launch_bounds__(kBlock)
     S.cnt_small[c] = 0;
     S.cnt_large[c] = 0ull;
   }
-  uint32_t slabs = 0;
+  uint32_t slabs = 0, edges = 0;
   const uint2 *ql = S.q_large[c];
   for (uint32_t t0 = gw * kLargeChunk; t0 < large_items; t0 += W * kLargeChunk) {
     uint32_t lo = 0, hi = n_large;  // last entry with start <= t0
@@ -157,9 +161,9 @@ __global__ void __launch_bounds__(kBlock)
       const uint32_t b = G.bucket_off[u] + (t - ent.y);
       if (MODE == 0) {
         const uint32_t du = dist_of(coherent_load64(S.dist + u));
-        if (du != kInf) slabs += relax_bucket<MAP, WEIGHTS>(G, S, nx, epoch, u, du, b);
+        if (du != kInf) slabs += relax_bucket<MAP, WEIGHTS>(G, S, nx, epoch, u, du, b, edges);
       } else {
-        slabs += invalidate_bucket<MAP>(G, S, nx, u, b);
+        slabs += invalidate_bucket<MAP>(G, S, nx, u, b, edges);
       }
     }
   }
@@ -171,12 +175,15 @@ __global__ void __launch_bounds__(kBlock)
       const uint32_t du = dist_of(coherent_load64(S.dist + u));
       if (du == kInf) continue;
       for (uint32_t b = b0; b < b1; ++b)
-        slabs += relax_bucket<MAP, WEIGHTS>(G, S, nx, epoch, u, du, b);
+        slabs += relax_bucket<MAP, WEIGHTS>(G, S, nx, epoch, u, du, b, edges);
     } else {
-      for (uint32_t b = b0; b < b1; ++b) slabs += invalidate_bucket<MAP>(G, S, nx, u, b);
+      for (uint32_t b = b0; b < b1; ++b) slabs += invalidate_bucket<MAP>(G, S, nx, u, b, edges);
     }
   }
-  if (lane_id() == 0 && slabs) atomicAdd(S.slabs_read, (unsigned long long)slabs);
+  if (lane_id() == 0 && slabs) {
+    atomicAdd(S.slabs_read, (unsigned long long)slabs);
+    atomicAdd(S.edges_read, (unsigned long long)edges);
+  }
 }
 
 // Relax the batch edges (u, v): thread per edge for BFS.
@@ -281,7 +288,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
   const uint64_t km = key_mask<MAP>();
   const bool key_lane = (km >> lane) & 1ull;
-  uint32_t slabs = 0;
+  uint32_t slabs = 0, edges = 0;
   for (uint32_t b = gw; b < G.total_buckets; b += W) {
     const uint32_t u = G.bucket_owner[b];
     if (inv_bit(S.invbits, u)) continue;
@@ -291,6 +298,7 @@ __global__ void __launch_bounds__(kBlock)
     for (;;) {
       const uint32_t v = slab[lane];
       ++slabs;
+      edges += uint32_t(__popcll(ballot(key_lane && v < kTomb)));
       const uint32_t wv = MAP ? __shfl_down(v, 1, 64) : 1u;
       bool push = false;
       if (key_lane && v < kTomb && inv_bit(S.invbits, v)) {
@@ -307,7 +315,10 @@ __global__ void __launch_bounds__(kBlock)
       slab = slab_ptr(G.pool, nxt);
     }
   }
-  if (lane == 0 && slabs) atomicAdd(S.slabs_read, (unsigned long long)slabs);
+  if (lane == 0 && slabs) {
+    atomicAdd(S.slabs_read, (unsigned long long)slabs);
+    atomicAdd(S.edges_read, (unsigned long long)edges);
+  }
 }
 
 __global__ void k_set_source(uint64_t *dist, uint32_t source, uint32_t *q_small,
@@ -331,6 +342,7 @@ struct Counts {
   uint32_t pad[2];
   unsigned long long large[2];
   unsigned long long slabs;
+  unsigned long long edges;
 };
 
 struct Traversal {
@@ -349,6 +361,7 @@ struct Traversal {
   uint32_t epoch = 0;
   uint32_t last_iters = 0;
   uint64_t last_slabs = 0;
+  uint64_t last_edges = 0;
   int max_waves = 0;
 
   TravView view() const {
@@ -363,6 +376,7 @@ struct Traversal {
     v.cnt_small = cnt->small;
     v.cnt_large = cnt->large;
     v.slabs_read = &cnt->slabs;
+    v.edges_read = &cnt->edges;
     return v;
   }
 
@@ -440,6 +454,7 @@ struct Traversal {
     const unsigned grid = blocks_for_waves(want);
     const GraphView G = g->view();
     const TravView S = view();
+    ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
     if (g->weighted) {
       if (weights)
         k_expand<true, true, MODE><<<grid, kBlock, 0, stream>>>(G, S, c, ep, ns, nl, items);
@@ -450,16 +465,34 @@ struct Traversal {
     }
   }
 
+  // Algorithmic bytes of one expand step, from the counters before (p) and
+  // after (h) it; `c` is the side it wrote.  256 B per slab read, 8 B per
+  // neighbour's packed distance, 24 B per frontier vertex (entry, bucket
+  // offsets, its distance), 16 B per vertex pushed (atomic, stamp, entry).
+  template <int MODE> void account(const Counts &p, const Counts &h, uint32_t c) {
+    const uint32_t src_side = c ^ 1u;
+    const double frontier = double(p.small[src_side]) + double(p.large[src_side] >> 32);
+    const double pushed = double(h.small[c]) + double(h.large[c] >> 32);
+    const double bytes = 256.0 * double(h.slabs - p.slabs) + 8.0 * double(h.edges - p.edges) +
+                         24.0 * frontier + 16.0 * pushed;
+    prof_add_bytes(MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", bytes);
+  }
+
   // Iterate frontier steps starting from side `c` until the frontier is
   // empty.  Returns the final side.
   template <int MODE> int iterate(uint32_t &c, uint32_t &iters) {
     // a label-correcting run settles within n frontier steps (Bellman-Ford
     // bound); more means corrupted state, reported instead of spinning
     const uint64_t limit = uint64_t(iters) + uint64_t(g->n) + 2;
+    bool launched = false;
+    Counts prev{};
     for (;;) {
       if (iters > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
       if (int r = read_counts()) return r;
       const Counts h = *host.as<Counts>();
+      if (launched && profiling_on()) account<MODE>(prev, h, c);
+      prev = h;
+      launched = true;
       if (h.small[c] == 0 && (h.large[c] >> 32) == 0) break;
       const uint32_t nx = c ^ 1u;
       launch_expand<MODE>(c, MODE == 0 ? next_epoch() : 0u, h);
@@ -474,7 +507,7 @@ struct Traversal {
     (void)ms;
     MK_HIP(hipSetDevice(g->device));
     // graph mutations were issued on the caller's stream and completed
-    fill_u64(reinterpret_cast<uint64_t *>(&cnt->slabs), 0ull, 1, stream);
+    fill_u64(reinterpret_cast<uint64_t *>(&cnt->slabs), 0ull, 2, stream);
     MK_HIP(hipEventRecord(ev0, stream));
     return MK_OK;
   }
@@ -487,6 +520,7 @@ struct Traversal {
     if (ms) *ms = t;
     if (int r = read_counts()) return r;
     last_slabs = host.as<Counts>()->slabs;
+    last_edges = host.as<Counts>()->edges;
     last_iters = iters;
     return MK_OK;
   }
@@ -511,6 +545,7 @@ struct Traversal {
     if (n) {
       const uint32_t ep = next_epoch();
       const GraphView G = g->view();
+      ProfScope prof(stream, "k_seed_batch", 40.0 * n);
       if (weights) {
         const uint64_t target = uint64_t(max_waves);
         uint32_t epw = uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (n + target - 1) / target)));
@@ -535,8 +570,11 @@ struct Traversal {
     clear_side(1);
     if (n) {
       const GraphView G = g->view();
-      k_invalidate_batch<<<blocks_for_threads(n), kBlock, 0, stream>>>(G, view(), src, dst, n, 0,
-                                                                      source);
+      {
+        ProfScope prof(stream, "k_invalidate_batch", 24.0 * n);
+        k_invalidate_batch<<<blocks_for_threads(n), kBlock, 0, stream>>>(G, view(), src, dst, n,
+                                                                        0, source);
+      }
       MK_HIP(hipGetLastError());
       // propagate invalidation down the tree; count what was invalidated
       uint32_t inv_steps = 0;
@@ -552,15 +590,33 @@ struct Traversal {
         c = 0;
         const unsigned grid = blocks_for_waves(
             std::min<uint64_t>(g->total_buckets, uint64_t(max_waves) * 4));
-        if (g->weighted) {
-          if (weights)
-            k_reseed<true, true><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
-          else
-            k_reseed<true, false><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
-        } else {
-          k_reseed<false, false><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
+        Counts before{};
+        if (profiling_on()) {
+          if (int r = read_counts()) return r;
+          before = *host.as<Counts>();
+        }
+        {
+          ProfScope prof(stream, "k_reseed", 0.0);
+          if (g->weighted) {
+            if (weights)
+              k_reseed<true, true><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
+            else
+              k_reseed<true, false><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
+          } else {
+            k_reseed<false, false><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
+          }
         }
         MK_HIP(hipGetLastError());
+        if (profiling_on()) {
+          // 256 B per slab, 16 B per bucket (owner, invalid bit, distance),
+          // 4 B per key (invalid bit), 16 B per re-seeded vertex
+          if (int r = read_counts()) return r;
+          const Counts h = *host.as<Counts>();
+          prof_add_bytes("k_reseed", 256.0 * double(h.slabs - before.slabs) +
+                                         16.0 * double(g->total_buckets) +
+                                         4.0 * double(h.edges - before.edges) +
+                                         16.0 * (double(h.small[0]) + double(h.large[0] >> 32)));
+        }
         fill_u32(invbits, 0u, (uint64_t(g->n) + 31) / 32, stream);
         // the re-seeded vertices are on side 0
         iters = inv_steps + 1;
@@ -647,10 +703,11 @@ int mk_bfs_read(mk_bfs *b, uint32_t *d, uint32_t *p) {
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_read: NULL handle");
   return b->t.read(d, p);
 }
-int mk_bfs_last_stats(mk_bfs *b, uint32_t *iters, uint64_t *slabs) {
+int mk_bfs_last_stats(mk_bfs *b, uint32_t *iters, uint64_t *slabs, uint64_t *edges) {
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_last_stats: NULL handle");
   if (iters) *iters = b->t.last_iters;
   if (slabs) *slabs = b->t.last_slabs;
+  if (edges) *edges = b->t.last_edges;
   return MK_OK;
 }
 
@@ -678,10 +735,11 @@ int mk_sssp_read(mk_sssp *s, uint32_t *d, uint32_t *p) {
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_read: NULL handle");
   return s->t.read(d, p);
 }
-int mk_sssp_last_stats(mk_sssp *s, uint32_t *iters, uint64_t *slabs) {
+int mk_sssp_last_stats(mk_sssp *s, uint32_t *iters, uint64_t *slabs, uint64_t *edges) {
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_last_stats: NULL handle");
   if (iters) *iters = s->t.last_iters;
   if (slabs) *slabs = s->t.last_slabs;
+  if (edges) *edges = s->t.last_edges;
   return MK_OK;
 }
 

@@ -55,13 +55,16 @@ def split_keys(keys):
 
 
 def insert_batch(sorted_keys, n, count, seed, weighted=False, wseed=7):
-    """`count` distinct new edges absent from `sorted_keys`."""
-    sorted_keys = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+    """`count` distinct new edges absent from `sorted_keys` (None: no check
+    against existing edges)."""
+    if sorted_keys is not None:
+        sorted_keys = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
     src = np.empty(count, np.uint32)
     dst = np.empty(count, np.uint32)
     w = np.empty(count, np.uint32) if weighted else None
-    _lib.call("mk_gen_insert_batch", _p(sorted_keys), len(sorted_keys), n, count, seed,
-              _p(src), _p(dst), _p(w), wseed)
+    _lib.call("mk_gen_insert_batch", _p(sorted_keys),
+              0 if sorted_keys is None else len(sorted_keys), n, count, seed, _p(src), _p(dst),
+              _p(w), wseed)
     return src, dst, w
 
 
