@@ -484,6 +484,8 @@ static int batch_prologue(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   if (int r = g->status.ensure(std::max<size_t>(n, 1))) return r;
   if (int r = g->host_scratch.ensure(64)) return r;
   g->status_n = n;
+  g->last_op = 0;
+  g->last_src = g->last_dst = g->last_w = nullptr;
   fill_u32(g->counters, 0u, 4, s);
   if (n == 0) return MK_OK;
   k_validate<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters);
@@ -536,6 +538,10 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
                                       std::to_string(pool_fail) +
                                       " edges); batch rolled back, edge set unchanged");
   }
+  g->last_op = 1;
+  g->last_src = src;
+  g->last_dst = dst;
+  g->last_w = w;
   return MK_OK;
 }
 
@@ -560,6 +566,9 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   }
   MK_HIP(hipGetLastError());
   MK_HIP(hipStreamSynchronize(s));
+  g->last_op = 2;
+  g->last_src = src;
+  g->last_dst = dst;
   return MK_OK;
 }
 

@@ -205,6 +205,12 @@ struct mk_graph {
   std::vector<uint32_t> bucket_off_host;
   mk::DevBuf status;             // uint8 per edge of the last batch
   uint32_t status_n = 0;
+  // identity of the last mutation batch, so an algorithm called with the
+  // same batch can use its per-edge status (1 = insert, 2 = delete)
+  int last_op = 0;
+  const void *last_src = nullptr;
+  const void *last_dst = nullptr;
+  const void *last_w = nullptr;
   mk::HostBuf host_scratch;
   int users = 0;                 // algorithm handles attached
   mk::GraphView view() const {

@@ -9,21 +9,28 @@
 // atomicMin, so the stored pair is the lexicographic minimum.
 //
 // Differences in method (same results):
-//  - The frontier holds vertices, not edges.  Vertices with at most
-//    kSmallBuckets buckets are expanded by one wave each; larger ones are
-//    queued with a reserved range of "bucket work items" (one 64-bit
-//    atomicAdd returns the entry index and the range start together, so
-//    entries are sorted by range start) and expanded bucket-by-bucket by all
-//    waves, which balances hub vertices.
+//  - The frontier holds vertices, not edges.  A wave takes up to 64 frontier
+//    vertices at once (their queue entries, bucket ranges and distances are
+//    read by all lanes in parallel) and then loads their bucket slabs four at
+//    a time, so several independent slab loads are in flight per wave.
+//    Vertices with more than kSmallBuckets buckets are queued separately
+//    with a reserved range of "bucket work items" (one 64-bit atomicAdd
+//    returns the entry index and the range start together, so entries are
+//    sorted by range start) and expanded bucket-by-bucket by all waves, which
+//    balances hub vertices.
 //  - Duplicate pushes within an iteration are dropped with a per-vertex
-//    epoch stamp instead of the reference's Queued flags, so nothing is
-//    reset between iterations.
+//    epoch stamp instead of the reference's Queued flags.
+//  - Frontier counters are triple-buffered in device memory: step i reads
+//    slot i%3, appends to slot (i+1)%3 and clears slot (i+2)%3, so the host
+//    launches several steps back to back and reads the counters only to
+//    detect the empty frontier (steps past it are no-ops).
 //  - Decremental invalidation walks the shortest-path tree downward from the
 //    deleted tree edges (children of x are out-neighbours whose packed parent
 //    is x), O(edges of invalidated vertices), instead of every vertex walking
-//    its ancestor chain.  Re-seeding scans every bucket once for edges from a
+//    its ancestor chain.  Re-seeding scans the graph once for edges from a
 //    valid vertex into an invalidated one, as the reference's frontier build
-//    does (BFSImpl.cuh:406-440), filtered through an L2-resident bitmap.
+//    does (BFSImpl.cuh:406-440), but reads only buckets of reached vertices
+//    with live edges and tests targets against an L2-resident bitmap.
 //  - Every frontier buffer is sized for n vertices once, at create time
 //    (the reference sizes them from the batch and can overflow).
 #include <hip/hip_runtime.h>
@@ -39,6 +46,19 @@ namespace mk {
 
 constexpr uint32_t kSmallBuckets = 2;
 constexpr uint32_t kLargeChunk = 8;  // bucket work items per wave step
+constexpr uint32_t kBatch = 4;       // slab loads in flight per wave
+
+// Device counters.  Slots rotate per frontier step; the rest accumulate
+// over one compute call.
+struct Counts {
+  uint32_t small[3];
+  uint32_t steps;                 // steps that found a non-empty frontier
+  unsigned long long large[3];    // (entries << 32) | bucket items
+  unsigned long long slabs;       // slabs read
+  unsigned long long edges;       // live keys read
+  unsigned long long visited;     // frontier vertices expanded
+  unsigned long long pushed;      // vertices appended to a frontier
+};
 
 struct TravView {
   uint64_t *dist;
@@ -46,40 +66,40 @@ struct TravView {
   uint32_t *invbits;
   uint32_t *q_small[2];
   uint2 *q_large[2];
-  uint32_t *cnt_small;             // [2]
-  unsigned long long *cnt_large;   // [2]: (entries << 32) | bucket items
-  unsigned long long *slabs_read;  // statistics: slabs and live keys read
-  unsigned long long *edges_read;
+  Counts *cnt;
 };
 
-// Append v to the next frontier `nx` (lanes with `push`).
-MK_DEV void enqueue(const GraphView &G, const TravView &S, uint32_t nx, bool push, uint32_t v) {
+// Append v to queue side `side`, counter slot `slot` (lanes with `push`).
+// Wave-collective.
+MK_DEV void enqueue(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                    bool push, uint32_t v) {
   uint32_t nb = 0;
   if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
   const bool small = push && nb <= kSmallBuckets;
-  wave_push(small, v, S.q_small[nx], S.cnt_small + nx);
+  const uint64_t pm = ballot(push);
+  if (pm == 0) return;
+  wave_push(small, v, S.q_small[side], &S.cnt->small[slot]);
   if (push && !small) {
-    const unsigned long long r = atomicAdd(S.cnt_large + nx, (1ull << 32) | nb);
-    S.q_large[nx][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+    const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
+    S.q_large[side][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
   }
+  if (lane_id() == ffs64(pm)) atomicAdd(&S.cnt->pushed, (unsigned long long)__popcll(pm));
 }
 
-// Relax every edge of one bucket chain of u (distance du).
-template <bool MAP, bool WEIGHTS>
-MK_DEV uint32_t relax_bucket(const GraphView &G, const TravView &S, uint32_t nx, uint32_t epoch,
-                             uint32_t u, uint32_t du, uint32_t b, uint32_t &edges) {
+MK_DEV bool inv_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> (v & 31u)) & 1u; }
+
+// Per-slab work of the three passes.  `v` is this lane's word of one slab of
+// vertex u (distance du).  Wave-collective; returns live keys seen.
+template <bool MAP, bool WEIGHTS, int MODE>
+MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                          uint32_t epoch, uint32_t u, uint32_t du, uint32_t v) {
   const uint32_t lane = lane_id();
-  const uint64_t km = key_mask<MAP>();
-  const bool key_lane = (km >> lane) & 1ull;
-  const uint32_t *slab = G.heads + size_t(b) * kSlabWords;
-  uint32_t slabs = 0;
-  for (;;) {
-    const uint32_t v = slab[lane];
-    ++slabs;
-    edges += uint32_t(__popcll(ballot(key_lane && v < kTomb)));
-    const uint32_t wv = MAP ? __shfl_down(v, 1, 64) : 1u;
-    bool push = false;
-    if (key_lane && v < kTomb) {
+  const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
+  const bool live = key_lane && v < kTomb;
+  const uint32_t wv = MAP ? __shfl_down(v, 1, 64) : 1u;
+  bool push = false;
+  if (MODE == 0) {  // relax u -> v
+    if (live) {
       const uint32_t D = du + (WEIGHTS ? wv : 1u);
       if (D >= du && D < dist_of(S.dist[v])) {
         const uint64_t old = atomicMin((unsigned long long *)(S.dist + v),
@@ -87,32 +107,10 @@ MK_DEV uint32_t relax_bucket(const GraphView &G, const TravView &S, uint32_t nx,
         if (D < dist_of(old)) push = atomicExch(S.stamp + v, epoch) != epoch;
       }
     }
-    enqueue(G, S, nx, push, v);
-    const uint32_t nxt = read_lane(v, kNextLane);
-    if (nxt == kEmpty) break;
-    slab = slab_ptr(G.pool, nxt);
-  }
-  return slabs;
-}
-
-// Invalidate the children of invalidated vertex x found in one bucket chain:
-// out-neighbours whose packed parent is x.
-template <bool MAP>
-MK_DEV uint32_t invalidate_bucket(const GraphView &G, const TravView &S, uint32_t nx, uint32_t x,
-                                  uint32_t b, uint32_t &edges) {
-  const uint32_t lane = lane_id();
-  const uint64_t km = key_mask<MAP>();
-  const bool key_lane = (km >> lane) & 1ull;
-  const uint32_t *slab = G.heads + size_t(b) * kSlabWords;
-  uint32_t slabs = 0;
-  for (;;) {
-    const uint32_t v = slab[lane];
-    ++slabs;
-    edges += uint32_t(__popcll(ballot(key_lane && v < kTomb)));
-    bool push = false;
-    if (key_lane && v < kTomb) {
+  } else if (MODE == 1) {  // invalidate tree children of u
+    if (live) {
       const uint64_t cur = coherent_load64(S.dist + v);
-      if (cur != kInfInf && parent_of(cur) == x) {
+      if (cur != kInfInf && parent_of(cur) == u) {
         if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                       (unsigned long long)kInfInf) == cur) {
           atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
@@ -120,31 +118,87 @@ MK_DEV uint32_t invalidate_bucket(const GraphView &G, const TravView &S, uint32_
         }
       }
     }
-    enqueue(G, S, nx, push, v);
-    const uint32_t nxt = read_lane(v, kNextLane);
-    if (nxt == kEmpty) break;
-    slab = slab_ptr(G.pool, nxt);
+  } else {  // re-seed invalidated v from valid u
+    if (live && inv_bit(S.invbits, v)) {
+      const uint32_t D = du + (WEIGHTS ? wv : 1u);
+      if (D >= du) {
+        const uint64_t old = atomicMin((unsigned long long *)(S.dist + v),
+                                       (unsigned long long)pack(D, u));
+        if (D < dist_of(old)) push = atomicExch(S.stamp + v, epoch) != epoch;
+      }
+    }
   }
-  return slabs;
+  enqueue(G, S, side, slot, push, v);
+  return uint32_t(__popcll(ballot(live)));
 }
 
-// One frontier step.  MODE 0 relaxes out-edges of every frontier vertex;
-// MODE 1 propagates invalidation to tree children.
+// Work of a whole chain given its first slab's word `v` (already loaded).
+template <bool MAP, bool WEIGHTS, int MODE>
+MK_DEV void chain_work(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                       uint32_t epoch, uint32_t u, uint32_t du, uint32_t v, uint32_t &slabs,
+                       uint32_t &edges) {
+  for (;;) {
+    ++slabs;
+    edges += slab_work<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, u, du, v);
+    const uint32_t nx = read_lane(v, kNextLane);
+    if (nx == kEmpty) break;
+    v = slab_ptr(G.pool, nx)[lane_id()];
+  }
+}
+
+// Process the buckets named by lanes of `m`: lane j owns bucket bk (its
+// vertex uj with distance dj).  Slabs are loaded kBatch at a time.
+template <bool MAP, bool WEIGHTS, int MODE>
+MK_DEV void batched_buckets(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                            uint32_t epoch, uint64_t m, uint32_t bk, uint32_t uj, uint32_t dj,
+                            uint32_t &slabs, uint32_t &edges) {
+  const uint32_t lane = lane_id();
+  while (m) {
+    uint32_t js[kBatch];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBatch; ++k) {
+      js[k] = m ? ffs64(m) : 0u;
+      if (m) {
+        m &= m - 1;
+        c = k + 1;
+      }
+    }
+    uint32_t v[kBatch];
+#pragma unroll
+    for (uint32_t k = 0; k < kBatch; ++k)
+      v[k] = k < c ? G.heads[size_t(read_lane(bk, js[k])) * kSlabWords + lane] : kEmpty;
+#pragma unroll
+    for (uint32_t k = 0; k < kBatch; ++k)
+      if (k < c)
+        chain_work<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, read_lane(uj, js[k]),
+                                       read_lane(dj, js[k]), v[k], slabs, edges);
+  }
+}
+
+// One frontier step: read queue side `side`, counter slot `slot`; append to
+// side^1 / slot+1; clear slot+2.  MODE 0 relaxes, MODE 1 invalidates.
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
-    k_expand(GraphView G, TravView S, uint32_t c, uint32_t epoch, uint32_t n_small,
-             uint32_t n_large, uint32_t large_items) {
+    k_expand(GraphView G, TravView S, uint32_t side, uint32_t slot, uint32_t epoch) {
+  const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t nx = c ^ 1u;
-  // side c's counters were read by the host before launch; clear them now so
-  // they are ready for the step after this one
+  const uint32_t nslot = slot == 2 ? 0 : slot + 1;
+  const uint32_t cslot = nslot == 2 ? 0 : nslot + 1;
+  const uint32_t nside = side ^ 1u;
+  const uint32_t n_small = S.cnt->small[slot];
+  const unsigned long long lg = S.cnt->large[slot];
+  const uint32_t n_large = uint32_t(lg >> 32), large_items = uint32_t(lg);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.cnt_small[c] = 0;
-    S.cnt_large[c] = 0ull;
+    S.cnt->small[cslot] = 0;
+    S.cnt->large[cslot] = 0ull;
+    if (n_small + n_large) atomicAdd(&S.cnt->steps, 1u);
   }
-  uint32_t slabs = 0, edges = 0;
-  const uint2 *ql = S.q_large[c];
+  if (n_small + n_large == 0) return;
+  uint32_t slabs = 0, edges = 0, visited = 0;
+  // hub vertices: bucket work items spread over all waves
+  const uint2 *ql = S.q_large[side];
   for (uint32_t t0 = gw * kLargeChunk; t0 < large_items; t0 += W * kLargeChunk) {
     uint32_t lo = 0, hi = n_large;  // last entry with start <= t0
     while (hi - lo > 1) {
@@ -152,110 +206,135 @@ __global__ void __launch_bounds__(kBlock)
       if (ql[mid].y <= t0) lo = mid;
       else hi = mid;
     }
+    // lane k < chunk takes work item t0 + k
+    const uint32_t t = t0 + lane;
+    bool act = lane < kLargeChunk && t < large_items;
     uint32_t e = lo;
-    const uint32_t t1 = min(t0 + kLargeChunk, large_items);
-    for (uint32_t t = t0; t < t1; ++t) {
+    if (act)
       while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
+    uint32_t u = 0, b = 0, du = 0;
+    if (act) {
       const uint2 ent = ql[e];
-      const uint32_t u = ent.x;
-      const uint32_t b = G.bucket_off[u] + (t - ent.y);
-      if (MODE == 0) {
-        const uint32_t du = dist_of(coherent_load64(S.dist + u));
-        if (du != kInf) slabs += relax_bucket<MAP, WEIGHTS>(G, S, nx, epoch, u, du, b, edges);
-      } else {
-        slabs += invalidate_bucket<MAP>(G, S, nx, u, b, edges);
-      }
+      u = ent.x;
+      b = G.bucket_off[u] + (t - ent.y);
+      du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
+      if (MODE == 0 && du == kInf) act = false;
     }
+    batched_buckets<MAP, WEIGHTS, MODE>(G, S, nside, nslot, epoch, ballot(act), b, u, du, slabs,
+                                        edges);
   }
-  const uint32_t *qs = S.q_small[c];
-  for (uint32_t i = gw; i < n_small; i += W) {
-    const uint32_t u = qs[i];
-    const uint32_t b0 = G.bucket_off[u], b1 = G.bucket_off[u + 1];
-    if (MODE == 0) {
-      const uint32_t du = dist_of(coherent_load64(S.dist + u));
-      if (du == kInf) continue;
-      for (uint32_t b = b0; b < b1; ++b)
-        slabs += relax_bucket<MAP, WEIGHTS>(G, S, nx, epoch, u, du, b, edges);
-    } else {
-      for (uint32_t b = b0; b < b1; ++b) slabs += invalidate_bucket<MAP>(G, S, nx, u, b, edges);
+  // small vertices: 64 per wave step
+  const uint32_t *qs = S.q_small[side];
+  for (uint32_t i0 = gw * 64; i0 < n_small; i0 += W * 64) {
+    const uint32_t i = i0 + lane;
+    bool act = i < n_small;
+    uint32_t u = 0, b0 = 0, nb = 0, du = 0;
+    if (act) {
+      u = qs[i];
+      b0 = G.bucket_off[u];
+      nb = G.bucket_off[u + 1] - b0;
+      du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
+      if (MODE == 0 && du == kInf) act = false;
     }
+    visited += uint32_t(__popcll(ballot(act)));
+    batched_buckets<MAP, WEIGHTS, MODE>(G, S, nside, nslot, epoch, ballot(act), b0, u, du, slabs,
+                                        edges);
+    // further buckets (at most kSmallBuckets - 1 for queued vertices)
+    for (uint32_t k = 1; ballot(act && nb > k); ++k)
+      batched_buckets<MAP, WEIGHTS, MODE>(G, S, nside, nslot, epoch, ballot(act && nb > k),
+                                          b0 + k, u, du, slabs, edges);
   }
-  if (lane_id() == 0 && slabs) {
-    atomicAdd(S.slabs_read, (unsigned long long)slabs);
-    atomicAdd(S.edges_read, (unsigned long long)edges);
+  if (lane == 0) {
+    if (slabs) atomicAdd(&S.cnt->slabs, (unsigned long long)slabs);
+    if (edges) atomicAdd(&S.cnt->edges, (unsigned long long)edges);
+    if (visited) atomicAdd(&S.cnt->visited, (unsigned long long)visited);
   }
 }
 
-// Relax the batch edges (u, v): thread per edge for BFS.
+// Re-seed invalidated vertices from valid in-neighbours: waves stream over
+// all buckets 64 at a time; a bucket's slabs are read only if its owner is
+// reached, valid and has live edges.
+template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock)
-    k_seed_batch_unit(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
-                      uint32_t n, uint32_t nx, uint32_t epoch) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool push = false;
-  uint32_t v = 0;
-  if (i < n) {
-    const uint32_t u = src[i];
-    v = dst[i];
-    if (u != v && u < G.n && v < G.n) {
-      const uint32_t du = dist_of(S.dist[u]);
-      const uint32_t D = du + 1u;
-      if (du != kInf && D < dist_of(S.dist[v])) {
-        const uint64_t old = atomicMin((unsigned long long *)(S.dist + v),
-                                       (unsigned long long)pack(D, u));
-        if (D < dist_of(old)) push = atomicExch(S.stamp + v, epoch) != epoch;
-      }
-    }
-  }
-  enqueue(G, S, nx, push, v);
-}
-
-// Relax the batch edges with the weight stored in the graph (the graph is
-// authoritative if the batch repeated an existing edge): wave-serial search.
-__global__ void __launch_bounds__(kBlock)
-    k_seed_batch_weighted(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
-                          uint32_t n, uint32_t epw, uint32_t nx, uint32_t epoch) {
+    k_reseed(GraphView G, TravView S, uint32_t epoch) {
   const uint32_t lane = lane_id();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t base = uint64_t(wave) * epw;
-  if (base >= n) return;
-  const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
-  uint32_t u = 0, v = 0, wgt = 0;
-  bool act = false;
-  if (lane < cnt) {
-    u = src[base + lane];
-    v = dst[base + lane];
-    act = u != v && u < G.n && v < G.n && dist_of(S.dist[u]) != kInf;
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  uint32_t slabs = 0, edges = 0;
+  for (uint64_t b0 = uint64_t(gw) * 64; b0 < G.total_buckets; b0 += uint64_t(W) * 64) {
+    const uint64_t b = b0 + lane;
+    bool act = b < G.total_buckets;
+    uint32_t u = 0, du = kInf;
+    if (act) {
+      u = G.bucket_owner[b];
+      act = G.vdeg[u] != 0;
+      if (act) {
+        du = dist_of(S.dist[u]);
+        act = du != kInf && !inv_bit(S.invbits, u);
+      }
+    }
+    batched_buckets<MAP, WEIGHTS, 2>(G, S, 0, 0, epoch, ballot(act), uint32_t(b), u, du, slabs,
+                                     edges);
   }
-  bool have = false;
-  uint64_t pending = ballot(act);
-  while (pending) {
-    const uint32_t j = ffs64(pending);
-    pending &= pending - 1;
-    uint32_t wj = 0;
-    const int r = slab_search<true>(G, read_lane(u, j), read_lane(v, j), &wj);
-    if (lane == j) {
-      have = r == 1;
-      wgt = wj;
+  if (lane == 0) {
+    if (slabs) atomicAdd(&S.cnt->slabs, (unsigned long long)slabs);
+    if (edges) atomicAdd(&S.cnt->edges, (unsigned long long)edges);
+  }
+}
+
+// Relax the batch edges (u, v) into side 0 / slot 0.  Unit weights: thread
+// per edge.  Weighted: the weight is the batch's when this batch was the
+// graph's last insert and the edge was applied (`status`, may be NULL), and
+// is searched in the graph otherwise (the graph keeps an existing edge's
+// weight), one wave-serial search per such edge.
+template <bool MAP, bool WEIGHTS>
+__global__ void __launch_bounds__(kBlock)
+    k_seed_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
+                 const uint32_t *wts, const uint8_t *status, uint32_t n, uint32_t epoch) {
+  const uint32_t lane = lane_id();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t u = 0, v = 0, wgt = 1;
+  bool act = false, need = false;
+  if (i < n) {
+    u = src[i];
+    v = dst[i];
+    act = u != v && u < G.n && v < G.n && dist_of(S.dist[u]) != kInf;
+    if (act && WEIGHTS) {
+      if (status && wts && status[i] == MK_EDGE_APPLIED) wgt = wts[i];
+      else need = true;
+    }
+  }
+  if (WEIGHTS) {
+    uint64_t pending = ballot(need);
+    while (pending) {
+      const uint32_t j = ffs64(pending);
+      pending &= pending - 1;
+      uint32_t wj = 0;
+      const int r = slab_search<MAP>(G, read_lane(u, j), read_lane(v, j), &wj);
+      if (lane == j) {
+        if (r == 1) wgt = wj;
+        else act = false;  // not in the graph: nothing to relax
+      }
     }
   }
   bool push = false;
-  if (have) {
+  if (act) {
     const uint32_t du = dist_of(S.dist[u]);
-    const uint32_t D = du + wgt;
+    const uint32_t D = du + (WEIGHTS ? wgt : 1u);
     if (D >= du && D < dist_of(S.dist[v])) {
       const uint64_t old = atomicMin((unsigned long long *)(S.dist + v),
                                      (unsigned long long)pack(D, u));
       if (D < dist_of(old)) push = atomicExch(S.stamp + v, epoch) != epoch;
     }
   }
-  enqueue(G, S, nx, push, v);
+  enqueue(G, S, 0, 0, push, v);
 }
 
 // Decremental seed: batch edge (u, v) was deleted; if it was v's tree edge,
 // v becomes invalid (Invalidate, BFSImpl.cuh:3-15).
 __global__ void __launch_bounds__(kBlock)
     k_invalidate_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
-                       uint32_t n, uint32_t nx, uint32_t source) {
+                       uint32_t n, uint32_t source) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool push = false;
   uint32_t v = 0;
@@ -273,59 +352,22 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
   }
-  enqueue(G, S, nx, push, v);
+  enqueue(G, S, 0, 0, push, v);
 }
 
-MK_DEV bool inv_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> (v & 31u)) & 1u; }
-
-// Re-seed invalidated vertices from valid in-neighbours: one wave per bucket
-// over the whole graph, edges kept only if the target is invalidated.
-template <bool MAP, bool WEIGHTS>
-__global__ void __launch_bounds__(kBlock)
-    k_reseed(GraphView G, TravView S, uint32_t nx, uint32_t epoch) {
-  const uint32_t lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
-  const uint64_t km = key_mask<MAP>();
-  const bool key_lane = (km >> lane) & 1ull;
-  uint32_t slabs = 0, edges = 0;
-  for (uint32_t b = gw; b < G.total_buckets; b += W) {
-    const uint32_t u = G.bucket_owner[b];
-    if (inv_bit(S.invbits, u)) continue;
-    const uint32_t du = dist_of(S.dist[u]);
-    if (du == kInf) continue;
-    const uint32_t *slab = G.heads + size_t(b) * kSlabWords;
-    for (;;) {
-      const uint32_t v = slab[lane];
-      ++slabs;
-      edges += uint32_t(__popcll(ballot(key_lane && v < kTomb)));
-      const uint32_t wv = MAP ? __shfl_down(v, 1, 64) : 1u;
-      bool push = false;
-      if (key_lane && v < kTomb && inv_bit(S.invbits, v)) {
-        const uint32_t D = du + (WEIGHTS ? wv : 1u);
-        if (D >= du) {
-          const uint64_t old = atomicMin((unsigned long long *)(S.dist + v),
-                                         (unsigned long long)pack(D, u));
-          if (D < dist_of(old)) push = atomicExch(S.stamp + v, epoch) != epoch;
-        }
-      }
-      enqueue(G, S, nx, push, v);
-      const uint32_t nxt = read_lane(v, kNextLane);
-      if (nxt == kEmpty) break;
-      slab = slab_ptr(G.pool, nxt);
-    }
-  }
-  if (lane == 0 && slabs) {
-    atomicAdd(S.slabs_read, (unsigned long long)slabs);
-    atomicAdd(S.edges_read, (unsigned long long)edges);
-  }
-}
-
-__global__ void k_set_source(uint64_t *dist, uint32_t source, uint32_t *q_small,
-                             uint32_t *cnt_small) {
+// Start a static run: the source at distance 0 is the frontier (side 0,
+// slot 0), in the queue its bucket count selects.
+__global__ void k_set_source(GraphView G, uint64_t *dist, uint32_t source, uint32_t *q_small,
+                             uint2 *q_large, Counts *cnt) {
   dist[source] = pack(0, source);
-  q_small[0] = source;
-  cnt_small[0] = 1;
+  const uint32_t nb = G.bucket_off[source + 1] - G.bucket_off[source];
+  if (nb <= kSmallBuckets) {
+    q_small[0] = source;
+    cnt->small[0] = 1;
+  } else {
+    q_large[0] = make_uint2(source, 0u);
+    cnt->large[0] = (1ull << 32) | nb;
+  }
 }
 
 __global__ void k_unpack(const uint64_t *dist, uint32_t n, uint32_t *d, uint32_t *p) {
@@ -337,14 +379,6 @@ __global__ void k_unpack(const uint64_t *dist, uint32_t n, uint32_t *d, uint32_t
 }
 
 // ---------------------------------------------------------------- host
-struct Counts {
-  uint32_t small[2];
-  uint32_t pad[2];
-  unsigned long long large[2];
-  unsigned long long slabs;
-  unsigned long long edges;
-};
-
 struct Traversal {
   mk_graph *g = nullptr;
   uint32_t source = 0;
@@ -373,10 +407,7 @@ struct Traversal {
     v.q_small[1] = qs[1];
     v.q_large[0] = ql[0];
     v.q_large[1] = ql[1];
-    v.cnt_small = cnt->small;
-    v.cnt_large = cnt->large;
-    v.slabs_read = &cnt->slabs;
-    v.edges_read = &cnt->edges;
+    v.cnt = cnt;
     return v;
   }
 
@@ -401,8 +432,8 @@ struct Traversal {
     fill_u64(dist, kInfInf, n, stream);
     fill_u32(stamp, 0u, n, stream);
     fill_u32(invbits, 0u, (n + 31) / 32, stream);
-    fill_u32(reinterpret_cast<uint32_t *>(cnt), 0u, sizeof(Counts) / 4, stream);
-    k_set_source<<<1, 1, 0, stream>>>(dist, source, qs[0], cnt->small);
+    clear_counts();
+    k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
     MK_HIP(hipStreamSynchronize(stream));
     max_waves = device_cus(g->device) * 16;
     return MK_OK;
@@ -426,204 +457,191 @@ struct Traversal {
   }
 
   uint32_t next_epoch() {
-    if (++epoch == 0xFFFFFFFFu) {  // wrap: clear stamps once per 4G iterations
+    if (++epoch == 0xFFFFFFFFu) {  // wrap: clear stamps once per 4G steps
       fill_u32(stamp, 0u, g->n, stream);
       epoch = 1;
     }
     return epoch;
   }
 
-  // Clear frontier side `c` counters.
-  void clear_side(uint32_t c) {
-    fill_u32(&cnt->small[c], 0u, 1, stream);
-    fill_u64(reinterpret_cast<uint64_t *>(&cnt->large[c]), 0ull, 1, stream);
+  // Zero every counter (statistics included).
+  void clear_counts() {
+    fill_u32(reinterpret_cast<uint32_t *>(cnt), 0u, sizeof(Counts) / 4, stream);
+  }
+  // Zero the frontier slots only.
+  void clear_slots() {
+    fill_u32(cnt->small, 0u, 3, stream);
+    fill_u64(reinterpret_cast<uint64_t *>(cnt->large), 0ull, 3, stream);
   }
 
-  int read_counts() {
+  int read_counts(Counts &h) {
     MK_HIP(hipMemcpyAsync(host.ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
     MK_HIP(hipStreamSynchronize(stream));
+    h = *host.as<Counts>();
     return MK_OK;
   }
 
-  template <int MODE> void launch_expand(uint32_t c, uint32_t ep, const Counts &h) {
-    const uint32_t ns = h.small[c];
-    const uint32_t nl = uint32_t(h.large[c] >> 32);
-    const uint32_t items = uint32_t(h.large[c]);
-    uint64_t want = std::max<uint64_t>(ns, (uint64_t(items) + kLargeChunk - 1) / kLargeChunk);
-    want = std::min<uint64_t>(std::max<uint64_t>(want, 1), uint64_t(max_waves));
-    const unsigned grid = blocks_for_waves(want);
+  template <int MODE> void launch_expand(uint32_t side, uint32_t slot, uint32_t ep) {
+    const unsigned grid = blocks_for_waves(uint64_t(max_waves));
     const GraphView G = g->view();
     const TravView S = view();
     ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
     if (g->weighted) {
       if (weights)
-        k_expand<true, true, MODE><<<grid, kBlock, 0, stream>>>(G, S, c, ep, ns, nl, items);
+        k_expand<true, true, MODE><<<grid, kBlock, 0, stream>>>(G, S, side, slot, ep);
       else
-        k_expand<true, false, MODE><<<grid, kBlock, 0, stream>>>(G, S, c, ep, ns, nl, items);
+        k_expand<true, false, MODE><<<grid, kBlock, 0, stream>>>(G, S, side, slot, ep);
     } else {
-      k_expand<false, false, MODE><<<grid, kBlock, 0, stream>>>(G, S, c, ep, ns, nl, items);
+      k_expand<false, false, MODE><<<grid, kBlock, 0, stream>>>(G, S, side, slot, ep);
     }
   }
 
-  // Algorithmic bytes of one expand step, from the counters before (p) and
-  // after (h) it; `c` is the side it wrote.  256 B per slab read, 8 B per
-  // neighbour's packed distance, 24 B per frontier vertex (entry, bucket
-  // offsets, its distance), 16 B per vertex pushed (atomic, stamp, entry).
-  template <int MODE> void account(const Counts &p, const Counts &h, uint32_t c) {
-    const uint32_t src_side = c ^ 1u;
-    const double frontier = double(p.small[src_side]) + double(p.large[src_side] >> 32);
-    const double pushed = double(h.small[c]) + double(h.large[c] >> 32);
-    const double bytes = 256.0 * double(h.slabs - p.slabs) + 8.0 * double(h.edges - p.edges) +
-                         24.0 * frontier + 16.0 * pushed;
-    prof_add_bytes(MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", bytes);
-  }
-
-  // Iterate frontier steps starting from side `c` until the frontier is
-  // empty.  Returns the final side.
-  template <int MODE> int iterate(uint32_t &c, uint32_t &iters) {
-    // a label-correcting run settles within n frontier steps (Bellman-Ford
-    // bound); more means corrupted state, reported instead of spinning
-    const uint64_t limit = uint64_t(iters) + uint64_t(g->n) + 2;
-    bool launched = false;
-    Counts prev{};
-    for (;;) {
-      if (iters > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
-      if (int r = read_counts()) return r;
-      const Counts h = *host.as<Counts>();
-      if (launched && profiling_on()) account<MODE>(prev, h, c);
-      prev = h;
-      launched = true;
-      if (h.small[c] == 0 && (h.large[c] >> 32) == 0) break;
-      const uint32_t nx = c ^ 1u;
-      launch_expand<MODE>(c, MODE == 0 ? next_epoch() : 0u, h);
+  // Run frontier steps from queue side 0 / slot 0 until the frontier is
+  // empty.  Steps are launched in groups without waiting; the counters are
+  // read after each group.  Algorithmic bytes (profiling) per group: 256 B
+  // per slab, 8 B per key (neighbour distance), 24 B per expanded vertex
+  // (entry, bucket range, distance), 16 B per push (atomic, stamp, entry).
+  template <int MODE> int iterate(Counts &h) {
+    uint32_t side = 0, slot = 0;
+    uint32_t group = 2;
+    uint64_t launched = 0;
+    const uint64_t limit = uint64_t(g->n) + 8;
+    Counts prev;
+    if (int r = read_counts(prev)) return r;
+    h = prev;
+    while (h.small[slot] != 0 || (h.large[slot] >> 32) != 0) {
+      if (launched > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
+      for (uint32_t k = 0; k < group; ++k) {
+        launch_expand<MODE>(side, slot, MODE == 0 ? next_epoch() : 0u);
+        side ^= 1u;
+        slot = slot == 2 ? 0 : slot + 1;
+        ++launched;
+      }
       MK_HIP(hipGetLastError());
-      c = nx;
-      ++iters;
+      if (int r = read_counts(h)) return r;
+      if (profiling_on()) {
+        const double bytes = 256.0 * double(h.slabs - prev.slabs) +
+                             8.0 * double(h.edges - prev.edges) +
+                             24.0 * double(h.visited - prev.visited) +
+                             16.0 * double(h.pushed - prev.pushed);
+        prof_add_bytes(MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", bytes);
+      }
+      prev = h;
+      group = std::min<uint32_t>(group * 2, 8);
     }
     return MK_OK;
   }
 
-  int begin(float *ms) {
-    (void)ms;
+  int begin() {
     MK_HIP(hipSetDevice(g->device));
-    // graph mutations were issued on the caller's stream and completed
-    fill_u64(reinterpret_cast<uint64_t *>(&cnt->slabs), 0ull, 2, stream);
+    clear_counts();
     MK_HIP(hipEventRecord(ev0, stream));
     return MK_OK;
   }
 
-  int end(float *ms, uint32_t iters) {
+  int end(float *ms) {
     MK_HIP(hipEventRecord(ev1, stream));
     MK_HIP(hipEventSynchronize(ev1));
     float t = 0.f;
     MK_HIP(hipEventElapsedTime(&t, ev0, ev1));
     if (ms) *ms = t;
-    if (int r = read_counts()) return r;
-    last_slabs = host.as<Counts>()->slabs;
-    last_edges = host.as<Counts>()->edges;
-    last_iters = iters;
+    Counts h;
+    if (int r = read_counts(h)) return r;
+    last_slabs = h.slabs;
+    last_edges = h.edges;
+    last_iters = h.steps;
     return MK_OK;
   }
 
   int run_static(float *ms) {
-    if (int r = begin(ms)) return r;
+    if (int r = begin()) return r;
     fill_u64(dist, kInfInf, g->n, stream);
-    clear_side(0);
-    clear_side(1);
-    k_set_source<<<1, 1, 0, stream>>>(dist, source, qs[0], &cnt->small[0]);
-    uint32_t c = 0, iters = 0;
-    if (int r = iterate<0>(c, iters)) return r;
-    return end(ms, iters);
+    k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
+    Counts h;
+    if (int r = iterate<0>(h)) return r;
+    return end(ms);
   }
 
-  int run_incremental(const uint32_t *src, const uint32_t *dst, uint32_t n, float *ms) {
+  int run_incremental(const uint32_t *src, const uint32_t *dst, const uint32_t *w, uint32_t n,
+                      float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "incremental: NULL batch");
-    if (int r = begin(ms)) return r;
-    uint32_t c = 0, iters = 0;
-    clear_side(0);
-    clear_side(1);
+    if (int r = begin()) return r;
     if (n) {
       const uint32_t ep = next_epoch();
       const GraphView G = g->view();
-      ProfScope prof(stream, "k_seed_batch", 40.0 * n);
-      if (weights) {
-        const uint64_t target = uint64_t(max_waves);
-        uint32_t epw = uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (n + target - 1) / target)));
-        const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
-        k_seed_batch_weighted<<<blocks_for_waves(waves), kBlock, 0, stream>>>(G, view(), src, dst,
-                                                                            n, epw, 0, ep);
-      } else {
-        k_seed_batch_unit<<<blocks_for_threads(n), kBlock, 0, stream>>>(G, view(), src, dst, n, 0,
-                                                                       ep);
+      // the insert's per-edge status applies when this is the batch it saw
+      const bool same = g->last_op == 1 && g->last_src == src && g->last_dst == dst &&
+                        g->status_n == n && g->last_w == w;
+      const uint8_t *st = same ? g->status.as<uint8_t>() : nullptr;
+      {
+        ProfScope prof(stream, "k_seed_batch", 24.0 * n);
+        if (weights)
+          k_seed_batch<true, true><<<blocks_for_threads(n), kBlock, 0, stream>>>(G, view(), src,
+                                                                                dst, w, st, n, ep);
+        else if (g->weighted)
+          k_seed_batch<true, false><<<blocks_for_threads(n), kBlock, 0, stream>>>(
+              G, view(), src, dst, nullptr, nullptr, n, ep);
+        else
+          k_seed_batch<false, false><<<blocks_for_threads(n), kBlock, 0, stream>>>(
+              G, view(), src, dst, nullptr, nullptr, n, ep);
       }
       MK_HIP(hipGetLastError());
-      if (int r = iterate<0>(c, iters)) return r;
+      Counts h;
+      if (int r = iterate<0>(h)) return r;
     }
-    return end(ms, iters);
+    return end(ms);
   }
 
   int run_decremental(const uint32_t *src, const uint32_t *dst, uint32_t n, float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "decremental: NULL batch");
-    if (int r = begin(ms)) return r;
-    uint32_t c = 0, iters = 0;
-    clear_side(0);
-    clear_side(1);
+    if (int r = begin()) return r;
     if (n) {
       const GraphView G = g->view();
       {
         ProfScope prof(stream, "k_invalidate_batch", 24.0 * n);
         k_invalidate_batch<<<blocks_for_threads(n), kBlock, 0, stream>>>(G, view(), src, dst, n,
-                                                                        0, source);
+                                                                        source);
       }
       MK_HIP(hipGetLastError());
-      // propagate invalidation down the tree; count what was invalidated
-      uint32_t inv_steps = 0;
-      if (int r = read_counts()) return r;
-      const Counts h0 = *host.as<Counts>();
-      const bool any = h0.small[0] != 0 || (h0.large[0] >> 32) != 0;
-      if (any) {
-        if (int r = iterate<1>(c, inv_steps)) return r;
-        // re-seed from valid in-neighbours over the whole graph
+      // propagate invalidation down the tree
+      Counts h;
+      if (int r = iterate<1>(h)) return r;
+      if (h.pushed != 0) {
+        // re-seed from valid in-neighbours; the re-seeded vertices start a
+        // fresh frontier at side 0 / slot 0
         const uint32_t ep = next_epoch();
-        clear_side(0);
-        clear_side(1);
-        c = 0;
-        const unsigned grid = blocks_for_waves(
-            std::min<uint64_t>(g->total_buckets, uint64_t(max_waves) * 4));
-        Counts before{};
-        if (profiling_on()) {
-          if (int r = read_counts()) return r;
-          before = *host.as<Counts>();
-        }
+        clear_slots();
+        const uint64_t slabs0 = h.slabs, edges0 = h.edges, pushed0 = h.pushed;
+        const unsigned grid = blocks_for_waves(std::min<uint64_t>(
+            (uint64_t(g->total_buckets) + 63) / 64, uint64_t(max_waves) * 2));
         {
           ProfScope prof(stream, "k_reseed", 0.0);
           if (g->weighted) {
             if (weights)
-              k_reseed<true, true><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
+              k_reseed<true, true><<<grid, kBlock, 0, stream>>>(G, view(), ep);
             else
-              k_reseed<true, false><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
+              k_reseed<true, false><<<grid, kBlock, 0, stream>>>(G, view(), ep);
           } else {
-            k_reseed<false, false><<<grid, kBlock, 0, stream>>>(G, view(), 0, ep);
+            k_reseed<false, false><<<grid, kBlock, 0, stream>>>(G, view(), ep);
           }
         }
         MK_HIP(hipGetLastError());
         if (profiling_on()) {
-          // 256 B per slab, 16 B per bucket (owner, invalid bit, distance),
-          // 4 B per key (invalid bit), 16 B per re-seeded vertex
-          if (int r = read_counts()) return r;
-          const Counts h = *host.as<Counts>();
-          prof_add_bytes("k_reseed", 256.0 * double(h.slabs - before.slabs) +
-                                         16.0 * double(g->total_buckets) +
-                                         4.0 * double(h.edges - before.edges) +
-                                         16.0 * (double(h.small[0]) + double(h.large[0] >> 32)));
+          // 20 B per bucket (owner, degree, distance, invalid bit of the
+          // owner), 256 B per slab read, 4 B per key (invalid bit), 16 B per
+          // re-seeded vertex
+          Counts a;
+          if (int r = read_counts(a)) return r;
+          prof_add_bytes("k_reseed", 20.0 * double(g->total_buckets) +
+                                         256.0 * double(a.slabs - slabs0) +
+                                         4.0 * double(a.edges - edges0) +
+                                         16.0 * double(a.pushed - pushed0));
         }
         fill_u32(invbits, 0u, (uint64_t(g->n) + 31) / 32, stream);
-        // the re-seeded vertices are on side 0
-        iters = inv_steps + 1;
-        if (int r = iterate<0>(c, iters)) return r;
+        if (int r = iterate<0>(h)) return r;
       }
     }
-    return end(ms, iters);
+    return end(ms);
   }
 
   int read(uint32_t *d, uint32_t *p) {
@@ -692,7 +710,7 @@ int mk_bfs_static(mk_bfs *b, float *ms) {
 int mk_bfs_incremental(mk_bfs *b, const uint32_t *src, const uint32_t *dst, uint32_t n,
                        float *ms) {
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_incremental: NULL handle");
-  return b->t.run_incremental(src, dst, n, ms);
+  return b->t.run_incremental(src, dst, nullptr, n, ms);
 }
 int mk_bfs_decremental(mk_bfs *b, const uint32_t *src, const uint32_t *dst, uint32_t n,
                        float *ms) {
@@ -721,9 +739,8 @@ int mk_sssp_static(mk_sssp *s, float *ms) {
 }
 int mk_sssp_incremental(mk_sssp *s, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
                         uint32_t n, float *ms) {
-  (void)w;
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_incremental: NULL handle");
-  return s->t.run_incremental(src, dst, n, ms);
+  return s->t.run_incremental(src, dst, w, n, ms);
 }
 int mk_sssp_decremental(mk_sssp *s, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
                         uint32_t n, float *ms) {

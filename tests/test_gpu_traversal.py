@@ -197,3 +197,36 @@ def test_sssp_rejects_unweighted_graph(gpu):
     g = mk.DynamicGraph(10, pool)
     with pytest.raises(mk.MeerkatError):
         mk.SSSP(g, 0)
+
+
+def test_sssp_incremental_uses_graph_weight_for_existing_edges(gpu):
+    # the batch repeats existing edges with smaller weights: the graph keeps
+    # the original weights, so the relaxation must use those (status path),
+    # and a copy of the batch (different pointers) must give the same result
+    # through the search path
+    n, s, d, w = gen.rmat(12, 16, seed=8, weighted=True)
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
+    g.InsertEdges(s, d, w)
+    es = EdgeSet(s, d, w)
+    sp = mk.SSSP(g, 0)
+    sp.Static()
+    bs, bd, bw = gen.insert_batch(es.keys, n, 3000, seed=77, weighted=True)
+    rs, rd = s[:2000], d[:2000]
+    rw = np.ones(2000, np.uint32)  # smaller than the stored weights
+    S = np.r_[bs, rs]
+    D = np.r_[bd, rd]
+    Wt = np.r_[bw, rw]
+    s_dev, d_dev, w_dev = g.InsertEdges(S, D, Wt)
+    es.insert(bs, bd, bw)
+    sp.Incremental(s_dev, d_dev, w_dev)  # same device batch: status path
+    check_sssp(sp, es, n, 0)
+    sp2 = mk.SSSP(g, 0)
+    sp2.Static()
+    bs2, bd2, bw2 = gen.insert_batch(es.keys, n, 2000, seed=78, weighted=True)
+    g.InsertEdges(bs2, bd2, bw2)
+    es.insert(bs2, bd2, bw2)
+    sp.Incremental(bs2, bd2, bw2)  # fresh device copies: search path
+    sp2.Static()
+    check_sssp(sp, es, n, 0)
+    assert np.array_equal(sp.Distances(), sp2.Distances())
