@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--batch", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=9)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mirror", action="store_true",
+                    help="no in-edge mirror: decremental re-seeds by scanning every edge")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -80,9 +82,9 @@ def main():
     del keys
     log(f"update stream staged on host in {time.perf_counter() - t0:.1f}s")
 
-    hints = gen.degree_hints(n, s)
     pool = mk.SlabPool(16 << 30)
-    g = mk.DynamicGraph(n, pool, 0.7, hints, weighted=True)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True,
+                        reverse=not args.no_mirror, in_degree_hints=gen.degree_hints(n, d))
     S, D, W = mk.device_u32(s), mk.device_u32(d), mk.device_u32(w)
     torch.cuda.synchronize()
     tb = time.perf_counter()
@@ -212,7 +214,7 @@ def main():
                    "scale": args.scale, "edge_factor": args.edge_factor, "vertices": n,
                    "edges": m, "weights": "uniform [1,100]", "batch_edges": args.batch,
                    "batch_mix": "50% delete / 50% insert", "source": 0,
-                   "parallelism": "single GPU"},
+                   "in_edge_mirror": not args.no_mirror, "parallelism": "single GPU"},
         "update_only_edges_per_s": round(args.batch * K / upd.sum(), 1),
         "sssp_incremental_gteps": round(incr_gteps, 3),
         "sssp_decremental_gteps": round(decr_gteps, 3),

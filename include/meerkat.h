@@ -85,6 +85,20 @@ int mk_pool_stats(mk_pool *pool, uint64_t *used_slabs, uint64_t *total_slabs);
 int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
                     const uint32_t *host_degree_hints, mk_graph **out);
 int mk_graph_destroy(mk_graph *g);
+/* Flags of mk_graph_create_ex. */
+#define MK_GRAPH_WEIGHTED 1u /* ConcurrentMap adjacency (dst -> weight)        */
+#define MK_GRAPH_REVERSE 2u  /* also keep an in-edge mirror (dst -> src [, w]) */
+/* DynamicGraph constructor with options.  MK_GRAPH_REVERSE keeps a second
+ * slab-hash graph of in-edges on the same pool, updated by every
+ * mk_graph_insert / mk_graph_delete; decremental BFS/SSSP then re-seed from
+ * in-neighbours instead of scanning every edge.  in_hints sizes the mirror
+ * (NULL: out_hints).  No reference counterpart (the reference builds a
+ * separate reverse DynamicGraph for PageRank, include/Algorithms/
+ * PageRank.cuh:11, and updates it by hand, src/PageRank.cu:137). */
+int mk_graph_create_ex(mk_pool *pool, uint32_t n, uint32_t flags, double load_factor,
+                       const uint32_t *out_hints, const uint32_t *in_hints, mk_graph **out);
+/* The in-edge mirror of g (read-only: inspection calls only). */
+int mk_graph_reverse(mk_graph *g, mk_graph **rev);
 
 /* DynamicGraph::InsertEdges(src, dst, n[, w])
  * (include/Graph/SlabGraphImpl.cuh:6-34 -> device/InsertEdgeKernel.cuh:4-63).

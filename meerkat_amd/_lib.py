@@ -60,6 +60,8 @@ _PROTOS = {
     "mk_pool_destroy": [vp],
     "mk_pool_stats": [vp, P(u64), P(u64)],
     "mk_graph_create": [vp, u32, i32, f64, vp, P(vp)],
+    "mk_graph_create_ex": [vp, u32, u32, f64, vp, vp, P(vp)],
+    "mk_graph_reverse": [vp, P(vp)],
     "mk_graph_destroy": [vp],
     "mk_graph_insert": [vp, vp, vp, vp, u32, vp],
     "mk_graph_delete": [vp, vp, vp, u32, vp],

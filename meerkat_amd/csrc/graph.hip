@@ -399,14 +399,14 @@ int mk_pool_stats(mk_pool *p, uint64_t *used, uint64_t *total) {
   return MK_OK;
 }
 
-int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
-                    const uint32_t *hints, mk_graph **out) {
-  if (!out) return fail(MK_ERR_BAD_ARG, "mk_graph_create: out is NULL");
+static int graph_alloc(mk_pool *pool, uint32_t n, int weighted, double load_factor,
+                       const uint32_t *hints, mk_graph **out) {
+  if (!out) return fail(MK_ERR_BAD_ARG, "graph create: out is NULL");
   *out = nullptr;
-  if (!pool) return fail(MK_ERR_BAD_ARG, "mk_graph_create: NULL pool");
-  if (n == 0 || n >= kTomb) return fail(MK_ERR_BAD_ARG, "mk_graph_create: n out of range");
+  if (!pool) return fail(MK_ERR_BAD_ARG, "graph create: NULL pool");
+  if (n == 0 || n >= kTomb) return fail(MK_ERR_BAD_ARG, "graph create: n out of range");
   if (!(load_factor > 0.0) || load_factor > 64.0)
-    return fail(MK_ERR_BAD_ARG, "mk_graph_create: load_factor must be in (0, 64]");
+    return fail(MK_ERR_BAD_ARG, "graph create: load_factor must be in (0, 64]");
   MK_HIP(hipSetDevice(pool->device));
   const double per = (weighted ? kMapKeysPerSlab : kSetKeysPerSlab) * load_factor;
   std::vector<uint32_t> off(size_t(n) + 1);
@@ -415,7 +415,7 @@ int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
     off[v] = uint32_t(total);
     uint64_t nb = hints ? uint64_t(std::ceil(double(hints[v]) / per)) : 1;
     total += nb ? nb : 1;
-    if (total >= 0xFFFFFFF0ull) return fail(MK_ERR_BAD_ARG, "mk_graph_create: >2^32 buckets");
+    if (total >= 0xFFFFFFF0ull) return fail(MK_ERR_BAD_ARG, "graph create: >2^32 buckets");
   }
   off[n] = uint32_t(total);
   mk_graph *g = new mk_graph();
@@ -439,7 +439,7 @@ int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
     (void)hipFree(g->vdeg);
     (void)hipFree(g->counters);
     delete g;
-    return hip_fail(e, "mk_graph_create: allocation");
+    return hip_fail(e, "graph create: allocation");
   }
   fill_u32(g->heads, kEmpty, total * kSlabWords, nullptr);
   fill_u32(g->vdeg, 0u, n, nullptr);
@@ -448,7 +448,7 @@ int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
                                                        g->bucket_owner);
   e = hipDeviceSynchronize();
   if (e != hipSuccess) {
-    set_error(std::string("mk_graph_create: init: ") + hipGetErrorString(e));
+    set_error(std::string("graph create: init: ") + hipGetErrorString(e));
     return MK_ERR_HIP;
   }
   g->bucket_off_host.swap(off);
@@ -457,10 +457,49 @@ int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
   return MK_OK;
 }
 
+
+int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
+                    const uint32_t *hints, mk_graph **out) {
+  return graph_alloc(pool, n, weighted ? 1 : 0, load_factor, hints, out);
+}
+
+int mk_graph_create_ex(mk_pool *pool, uint32_t n, uint32_t flags, double load_factor,
+                       const uint32_t *out_hints, const uint32_t *in_hints, mk_graph **out) {
+  if (flags & ~uint32_t(MK_GRAPH_WEIGHTED | MK_GRAPH_REVERSE))
+    return fail(MK_ERR_BAD_ARG, "mk_graph_create_ex: unknown flag bits");
+  const int weighted = (flags & MK_GRAPH_WEIGHTED) ? 1 : 0;
+  if (int r = graph_alloc(pool, n, weighted, load_factor, out_hints, out)) return r;
+  if (flags & MK_GRAPH_REVERSE) {
+    mk_graph *rev = nullptr;
+    int r = graph_alloc(pool, n, weighted, load_factor, in_hints ? in_hints : out_hints, &rev);
+    if (r) {
+      mk_graph_destroy(*out);
+      *out = nullptr;
+      return r;
+    }
+    rev->mirror_of = *out;
+    (*out)->rev = rev;
+  }
+  return MK_OK;
+}
+
+int mk_graph_reverse(mk_graph *g, mk_graph **rev) {
+  if (!g || !rev) return fail(MK_ERR_BAD_ARG, "mk_graph_reverse: NULL argument");
+  *rev = g->rev;
+  if (!g->rev) return fail(MK_ERR_STATE, "mk_graph_reverse: graph has no in-edge mirror");
+  return MK_OK;
+}
+
 int mk_graph_destroy(mk_graph *g) {
   if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_destroy: NULL graph");
   if (g->users) return fail(MK_ERR_STATE, "mk_graph_destroy: algorithm handles still attached");
+  if (g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_destroy: an in-edge mirror is owned by its graph");
   (void)hipSetDevice(g->device);
+  if (g->rev) {
+    g->rev->mirror_of = nullptr;
+    mk_graph_destroy(g->rev);
+    g->rev = nullptr;
+  }
   k_release<<<blocks_for_waves(g->total_buckets), kBlock>>>(g->view(), g->pool->view);
   (void)hipDeviceSynchronize();
   (void)hipFree(g->heads);
@@ -501,38 +540,74 @@ static int batch_prologue(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   return MK_OK;
 }
 
-int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
-                    uint32_t n, void *stream) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
-  if (n == 0) return MK_OK;
+// Launch the insert kernel on graph g (forward or mirror) with its status
+// buffer and counters.
+static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
+                          const uint32_t *w, uint32_t n, hipStream_t s, const char *name) {
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
   uint8_t *st = g->status.as<uint8_t>();
-  {
-    // per edge: batch entry, one slab probe, key (+value) write, degree
-    // atomic, status byte
-    ProfScope prof(s, "k_insert", double(n) * (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1));
-    if (g->weighted)
-      k_insert<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
-                                                                dst, w, n, epw, st, g->counters);
-    else
-      k_insert<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
-                                                                 dst, nullptr, n, epw, st,
-                                                                 g->counters);
-  }
+  // per edge: batch entry, one slab probe, key (+value) write, degree atomic,
+  // status byte
+  ProfScope prof(s, name, double(n) * (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1));
+  if (g->weighted)
+    k_insert<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src, dst,
+                                                              w, n, epw, st, g->counters);
+  else
+    k_insert<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
+                                                               dst, nullptr, n, epw, st,
+                                                               g->counters);
+}
+
+// Launch the delete kernel on graph g; only_status != 0xFF restricts it to
+// edges whose status equals it (roll-back of an insert).
+static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
+                          hipStream_t s, uint32_t only_status, const char *name) {
+  const uint32_t epw = edges_per_wave(n, g->device);
+  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
+  uint8_t *st = g->status.as<uint8_t>();
+  // per edge: batch entry, one slab probe, tombstone write, degree atomic,
+  // status byte
+  ProfScope prof(s, name, double(n) * (8 + 256 + 4 + 8 + 1));
+  if (g->weighted)
+    k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                              only_status, g->counters);
+  else
+    k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
+                                                               only_status, g->counters);
+}
+
+// Status buffer and counters of the in-edge mirror for a batch of n edges.
+static int mirror_prologue(mk_graph *g, uint32_t n, hipStream_t s) {
+  if (!g->rev) return MK_OK;
+  if (int r = g->rev->status.ensure(std::max<size_t>(n, 1))) return r;
+  if (int r = g->rev->host_scratch.ensure(64)) return r;
+  g->rev->status_n = n;
+  fill_u32(g->rev->counters, 0u, 4, s);
+  return MK_OK;
+}
+
+int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                    uint32_t n, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_insert: mirror is read-only");
+  if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
+  if (n == 0) return MK_OK;
+  if (int r = mirror_prologue(g, n, s)) return r;
+  launch_insert(g, src, dst, w, n, s, "k_insert");
+  if (g->rev) launch_insert(g->rev, dst, src, w, n, s, "k_insert_mirror");
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
+  if (g->rev)
+    MK_HIP(hipMemcpyAsync(g->rev->host_scratch.ptr, g->rev->counters, 16, hipMemcpyDeviceToHost,
+                          s));
   MK_HIP(hipStreamSynchronize(s));
-  const uint32_t pool_fail = g->host_scratch.as<uint32_t>()[1];
+  uint32_t pool_fail = g->host_scratch.as<uint32_t>()[1];
+  if (g->rev) pool_fail += g->rev->host_scratch.as<uint32_t>()[1];
   if (pool_fail) {
     // roll back: delete what this batch inserted, restoring the edge set
-    if (g->weighted)
-      k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                                MK_EDGE_APPLIED, g->counters);
-    else
-      k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                                 MK_EDGE_APPLIED, g->counters);
+    launch_delete(g, src, dst, n, s, MK_EDGE_APPLIED, "k_insert_rollback");
+    if (g->rev) launch_delete(g->rev, dst, src, n, s, MK_EDGE_APPLIED, "k_insert_rollback");
     MK_HIP(hipStreamSynchronize(s));
     return fail(MK_ERR_POOL_FULL, "mk_graph_insert: slab pool exhausted (" +
                                       std::to_string(pool_fail) +
@@ -548,22 +623,12 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
 int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
                     void *stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_delete: mirror is read-only");
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
   if (n == 0) return MK_OK;
-  const uint32_t epw = edges_per_wave(n, g->device);
-  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
-  uint8_t *st = g->status.as<uint8_t>();
-  {
-    // per edge: batch entry, one slab probe, tombstone write, degree atomic,
-    // status byte
-    ProfScope prof(s, "k_delete", double(n) * (8 + 256 + 4 + 8 + 1));
-    if (g->weighted)
-      k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                                0xFFu, g->counters);
-    else
-      k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                                 0xFFu, g->counters);
-  }
+  if (int r = mirror_prologue(g, n, s)) return r;
+  launch_delete(g, src, dst, n, s, 0xFFu, "k_delete");
+  if (g->rev) launch_delete(g->rev, dst, src, n, s, 0xFFu, "k_delete_mirror");
   MK_HIP(hipGetLastError());
   MK_HIP(hipStreamSynchronize(s));
   g->last_op = 2;

@@ -213,6 +213,8 @@ struct mk_graph {
   const void *last_w = nullptr;
   mk::HostBuf host_scratch;
   int users = 0;                 // algorithm handles attached
+  mk_graph *rev = nullptr;       // in-edge mirror (MK_GRAPH_REVERSE), or null
+  mk_graph *mirror_of = nullptr; // set on a mirror: the graph that owns it
   mk::GraphView view() const {
     mk::GraphView v;
     v.heads = heads;

@@ -48,17 +48,50 @@ constexpr uint32_t kSmallBuckets = 2;
 constexpr uint32_t kLargeChunk = 8;  // bucket work items per wave step
 constexpr uint32_t kBatch = 4;       // slab loads in flight per wave
 
-// Device counters.  Slots rotate per frontier step; the rest accumulate
-// over one compute call.
+constexpr uint32_t kWaveQ = 512;     // LDS staging entries per wave
+constexpr uint32_t kStatSlots = 64;  // statistics spread over this many slots
+enum { kStatSlabs = 0, kStatEdges = 1, kStatVisited = 2, kStatPushed = 3 };
+
+// Device counters.  Frontier slots rotate per step; the statistics
+// accumulate over one compute call, each block adding to slot
+// blockIdx % kStatSlots so no single address is contended.
 struct Counts {
   uint32_t small[3];
   uint32_t steps;                 // steps that found a non-empty frontier
   unsigned long long large[3];    // (entries << 32) | bucket items
-  unsigned long long slabs;       // slabs read
-  unsigned long long edges;       // live keys read
-  unsigned long long visited;     // frontier vertices expanded
-  unsigned long long pushed;      // vertices appended to a frontier
+  unsigned long long stats[kStatSlots][4];
 };
+
+// Host-side totals of a Counts snapshot.
+struct Totals {
+  uint32_t small[3];
+  uint32_t steps;
+  unsigned long long large[3];
+  unsigned long long slabs, edges, visited, pushed;
+};
+
+MK_DEV void add_stat(Counts *c, uint32_t which, uint32_t v) {
+  if (v) atomicAdd(&c->stats[blockIdx.x % kStatSlots][which], (unsigned long long)v);
+}
+
+// Per-wave frontier staging in LDS: pushes are appended to the wave's LDS
+// buffer and copied out with one global atomicAdd per kWaveQ entries.
+struct WaveQ {
+  uint32_t *buf;
+  uint32_t n;       // wave-uniform fill
+  uint32_t pushed;  // wave-uniform statistics
+};
+
+MK_DEV void wq_flush(WaveQ &q, uint32_t *queue, uint32_t *count) {
+  if (q.n == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(count, q.n);
+  base = read_lane(base, 0);
+  for (uint32_t i = lane_id(); i < q.n; i += 64) queue[base + i] = q.buf[i];
+  __builtin_amdgcn_wave_barrier();
+  q.n = 0;
+}
 
 struct TravView {
   uint64_t *dist;
@@ -69,21 +102,57 @@ struct TravView {
   Counts *cnt;
 };
 
-// Append v to queue side `side`, counter slot `slot` (lanes with `push`).
+// Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
+// small vertices through the wave's LDS staging, hub vertices directly.
 // Wave-collective.
-MK_DEV void enqueue(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
-                    bool push, uint32_t v) {
+MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
+                    uint32_t slot, bool push, uint32_t v) {
+  const uint64_t pm = ballot(push);
+  if (pm == 0) return;
   uint32_t nb = 0;
   if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
   const bool small = push && nb <= kSmallBuckets;
-  const uint64_t pm = ballot(push);
-  if (pm == 0) return;
-  wave_push(small, v, S.q_small[side], &S.cnt->small[slot]);
+  const uint64_t sm = ballot(small);
+  const uint32_t c = uint32_t(__popcll(sm));
+  if (q.n + c > kWaveQ) wq_flush(q, S.q_small[side], &S.cnt->small[slot]);
+  if (small) q.buf[q.n + lanes_below(sm)] = v;
+  q.n += c;
   if (push && !small) {
     const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
     S.q_large[side][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
   }
-  if (lane_id() == ffs64(pm)) atomicAdd(&S.cnt->pushed, (unsigned long long)__popcll(pm));
+  q.pushed += uint32_t(__popcll(pm));
+}
+
+// Block-aggregated append for kernels with one item per thread: one global
+// atomicAdd per block.  Called by every thread of the block.
+template <uint32_t NW>
+MK_DEV void block_enqueue(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                          bool push, uint32_t v, uint32_t *sh) {
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t nb = 0;
+  if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
+  const bool small = push && nb <= kSmallBuckets;
+  const uint64_t sm = ballot(small);
+  if (lane == 0) sh[w] = uint32_t(__popcll(sm));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < NW; ++k) {
+      const uint32_t c = sh[k];
+      sh[k] = tot;
+      tot += c;
+    }
+    sh[NW] = tot ? atomicAdd(&S.cnt->small[slot], tot) : 0u;
+  }
+  __syncthreads();
+  if (small) S.q_small[side][sh[NW] + sh[w] + lanes_below(sm)] = v;
+  if (push && !small) {
+    const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
+    S.q_large[side][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+  }
+  const uint64_t pm = ballot(push);
+  if (lane == 0) add_stat(S.cnt, kStatPushed, uint32_t(__popcll(pm)));
 }
 
 MK_DEV bool inv_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> (v & 31u)) & 1u; }
@@ -91,8 +160,8 @@ MK_DEV bool inv_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> 
 // Per-slab work of the three passes.  `v` is this lane's word of one slab of
 // vertex u (distance du).  Wave-collective; returns live keys seen.
 template <bool MAP, bool WEIGHTS, int MODE>
-MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
-                          uint32_t epoch, uint32_t u, uint32_t du, uint32_t v) {
+MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
+                          uint32_t slot, uint32_t epoch, uint32_t u, uint32_t du, uint32_t v) {
   const uint32_t lane = lane_id();
   const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
   const bool live = key_lane && v < kTomb;
@@ -109,7 +178,8 @@ MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, uint32_t side, 
     }
   } else if (MODE == 1) {  // invalidate tree children of u
     if (live) {
-      const uint64_t cur = coherent_load64(S.dist + v);
+      // values only move to INF during this pass, so a plain load suffices
+      const uint64_t cur = S.dist[v];
       if (cur != kInfInf && parent_of(cur) == u) {
         if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                       (unsigned long long)kInfInf) == cur) {
@@ -128,18 +198,18 @@ MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, uint32_t side, 
       }
     }
   }
-  enqueue(G, S, side, slot, push, v);
+  enqueue(G, S, q, side, slot, push, v);
   return uint32_t(__popcll(ballot(live)));
 }
 
 // Work of a whole chain given its first slab's word `v` (already loaded).
 template <bool MAP, bool WEIGHTS, int MODE>
-MK_DEV void chain_work(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
-                       uint32_t epoch, uint32_t u, uint32_t du, uint32_t v, uint32_t &slabs,
-                       uint32_t &edges) {
+MK_DEV void chain_work(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
+                       uint32_t slot, uint32_t epoch, uint32_t u, uint32_t du, uint32_t v,
+                       uint32_t &slabs, uint32_t &edges) {
   for (;;) {
     ++slabs;
-    edges += slab_work<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, u, du, v);
+    edges += slab_work<MAP, WEIGHTS, MODE>(G, S, q, side, slot, epoch, u, du, v);
     const uint32_t nx = read_lane(v, kNextLane);
     if (nx == kEmpty) break;
     v = slab_ptr(G.pool, nx)[lane_id()];
@@ -149,9 +219,9 @@ MK_DEV void chain_work(const GraphView &G, const TravView &S, uint32_t side, uin
 // Process the buckets named by lanes of `m`: lane j owns bucket bk (its
 // vertex uj with distance dj).  Slabs are loaded kBatch at a time.
 template <bool MAP, bool WEIGHTS, int MODE>
-MK_DEV void batched_buckets(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
-                            uint32_t epoch, uint64_t m, uint32_t bk, uint32_t uj, uint32_t dj,
-                            uint32_t &slabs, uint32_t &edges) {
+MK_DEV void batched_buckets(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
+                            uint32_t slot, uint32_t epoch, uint64_t m, uint32_t bk, uint32_t uj,
+                            uint32_t dj, uint32_t &slabs, uint32_t &edges) {
   const uint32_t lane = lane_id();
   while (m) {
     uint32_t js[kBatch];
@@ -171,7 +241,7 @@ MK_DEV void batched_buckets(const GraphView &G, const TravView &S, uint32_t side
 #pragma unroll
     for (uint32_t k = 0; k < kBatch; ++k)
       if (k < c)
-        chain_work<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, read_lane(uj, js[k]),
+        chain_work<MAP, WEIGHTS, MODE>(G, S, q, side, slot, epoch, read_lane(uj, js[k]),
                                        read_lane(dj, js[k]), v[k], slabs, edges);
   }
 }
@@ -196,6 +266,8 @@ __global__ void __launch_bounds__(kBlock)
     if (n_small + n_large) atomicAdd(&S.cnt->steps, 1u);
   }
   if (n_small + n_large == 0) return;
+  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
+  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
   uint32_t slabs = 0, edges = 0, visited = 0;
   // hub vertices: bucket work items spread over all waves
   const uint2 *ql = S.q_large[side];
@@ -217,11 +289,11 @@ __global__ void __launch_bounds__(kBlock)
       const uint2 ent = ql[e];
       u = ent.x;
       b = G.bucket_off[u] + (t - ent.y);
-      du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
+      du = MODE == 1 ? 0u : dist_of(S.dist[u]);
       if (MODE == 0 && du == kInf) act = false;
     }
-    batched_buckets<MAP, WEIGHTS, MODE>(G, S, nside, nslot, epoch, ballot(act), b, u, du, slabs,
-                                        edges);
+    batched_buckets<MAP, WEIGHTS, MODE>(G, S, q, nside, nslot, epoch, ballot(act), b, u, du,
+                                        slabs, edges);
   }
   // small vertices: 64 per wave step
   const uint32_t *qs = S.q_small[side];
@@ -233,21 +305,23 @@ __global__ void __launch_bounds__(kBlock)
       u = qs[i];
       b0 = G.bucket_off[u];
       nb = G.bucket_off[u + 1] - b0;
-      du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
+      du = MODE == 1 ? 0u : dist_of(S.dist[u]);
       if (MODE == 0 && du == kInf) act = false;
     }
     visited += uint32_t(__popcll(ballot(act)));
-    batched_buckets<MAP, WEIGHTS, MODE>(G, S, nside, nslot, epoch, ballot(act), b0, u, du, slabs,
-                                        edges);
+    batched_buckets<MAP, WEIGHTS, MODE>(G, S, q, nside, nslot, epoch, ballot(act), b0, u, du,
+                                        slabs, edges);
     // further buckets (at most kSmallBuckets - 1 for queued vertices)
     for (uint32_t k = 1; ballot(act && nb > k); ++k)
-      batched_buckets<MAP, WEIGHTS, MODE>(G, S, nside, nslot, epoch, ballot(act && nb > k),
+      batched_buckets<MAP, WEIGHTS, MODE>(G, S, q, nside, nslot, epoch, ballot(act && nb > k),
                                           b0 + k, u, du, slabs, edges);
   }
+  wq_flush(q, S.q_small[nside], &S.cnt->small[nslot]);
   if (lane == 0) {
-    if (slabs) atomicAdd(&S.cnt->slabs, (unsigned long long)slabs);
-    if (edges) atomicAdd(&S.cnt->edges, (unsigned long long)edges);
-    if (visited) atomicAdd(&S.cnt->visited, (unsigned long long)visited);
+    add_stat(S.cnt, kStatSlabs, slabs);
+    add_stat(S.cnt, kStatEdges, edges);
+    add_stat(S.cnt, kStatVisited, visited);
+    add_stat(S.cnt, kStatPushed, q.pushed);
   }
 }
 
@@ -260,6 +334,8 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
+  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
   uint32_t slabs = 0, edges = 0;
   for (uint64_t b0 = uint64_t(gw) * 64; b0 < G.total_buckets; b0 += uint64_t(W) * 64) {
     const uint64_t b = b0 + lane;
@@ -273,12 +349,175 @@ __global__ void __launch_bounds__(kBlock)
         act = du != kInf && !inv_bit(S.invbits, u);
       }
     }
-    batched_buckets<MAP, WEIGHTS, 2>(G, S, 0, 0, epoch, ballot(act), uint32_t(b), u, du, slabs,
-                                     edges);
+    batched_buckets<MAP, WEIGHTS, 2>(G, S, q, 0, 0, epoch, ballot(act), uint32_t(b), u, du,
+                                     slabs, edges);
   }
+  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
   if (lane == 0) {
-    if (slabs) atomicAdd(&S.cnt->slabs, (unsigned long long)slabs);
-    if (edges) atomicAdd(&S.cnt->edges, (unsigned long long)edges);
+    add_stat(S.cnt, kStatSlabs, slabs);
+    add_stat(S.cnt, kStatEdges, edges);
+    add_stat(S.cnt, kStatPushed, q.pushed);
+  }
+}
+
+// Minimum of a 64-bit value over the wave (all lanes active).
+MK_DEV uint64_t wave_min_u64(uint64_t x) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t lo = __shfl_xor(uint32_t(x), off, 64);
+    const uint32_t hi = __shfl_xor(uint32_t(x >> 32), off, 64);
+    const uint64_t o = (uint64_t(hi) << 32) | lo;
+    x = o < x ? o : x;
+  }
+  return x;
+}
+
+// Pull one in-chain of invalidated vertex tv (first slab word xv loaded):
+// the wave's minimum of dist[p] + w over valid, reached in-neighbours p is
+// applied to tv with one atomicMin.  Wave-collective.
+template <bool MAP, bool WEIGHTS>
+MK_DEV void pull_chain(const GraphView &G, const GraphView &R, const TravView &S, WaveQ &q,
+                       uint32_t epoch, uint32_t tv, uint32_t xv, uint32_t &slabs,
+                       uint32_t &edges) {
+  const uint32_t lane = lane_id();
+  const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
+  for (;;) {
+    ++slabs;
+    const bool live = key_lane && xv < kTomb;
+    edges += uint32_t(__popcll(ballot(live)));
+    const uint32_t wv = MAP ? __shfl_down(xv, 1, 64) : 1u;
+    uint64_t cand = kInfInf;
+    if (live && !inv_bit(S.invbits, xv)) {
+      const uint32_t dp = dist_of(S.dist[xv]);
+      const uint32_t D = dp + (WEIGHTS ? wv : 1u);
+      if (dp != kInf && D >= dp) cand = pack(D, xv);
+    }
+    cand = wave_min_u64(cand);
+    bool push = false;
+    if (lane == 0 && cand != kInfInf) {
+      const uint64_t old = atomicMin((unsigned long long *)(S.dist + tv), (unsigned long long)cand);
+      if (dist_of(cand) < dist_of(old)) push = atomicExch(S.stamp + tv, epoch) != epoch;
+    }
+    enqueue(G, S, q, 0, 0, push, tv);
+    const uint32_t nx = read_lane(xv, kNextLane);
+    if (nx == kEmpty) break;
+    xv = slab_ptr(R.pool, nx)[lane];
+  }
+}
+
+// Pull the in-buckets named by lanes of `m` (lane j: vertex tj, in-bucket
+// bk), kBatch slab loads in flight.
+template <bool MAP, bool WEIGHTS>
+MK_DEV void pull_buckets(const GraphView &G, const GraphView &R, const TravView &S, WaveQ &q,
+                         uint32_t epoch, uint64_t m, uint32_t bk, uint32_t tj, uint32_t &slabs,
+                         uint32_t &edges) {
+  const uint32_t lane = lane_id();
+  while (m) {
+    uint32_t js[kBatch];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t t = 0; t < kBatch; ++t) {
+      js[t] = m ? ffs64(m) : 0u;
+      if (m) {
+        m &= m - 1;
+        c = t + 1;
+      }
+    }
+    uint32_t x[kBatch];
+#pragma unroll
+    for (uint32_t t = 0; t < kBatch; ++t)
+      x[t] = t < c ? R.heads[size_t(read_lane(bk, js[t])) * kSlabWords + lane] : kEmpty;
+#pragma unroll
+    for (uint32_t t = 0; t < kBatch; ++t)
+      if (t < c)
+        pull_chain<MAP, WEIGHTS>(G, R, S, q, epoch, read_lane(tj, js[t]), x[t], slabs, edges);
+  }
+}
+
+// Pull re-seed with the in-edge mirror R: every invalidated vertex v (a set
+// bit of invbits) takes the minimum of dist[p] + w over its valid, reached
+// in-neighbours p.  Waves take 64 bitmap words at a time; each set bit
+// becomes one lane's vertex.  Vertices with more than kSmallBuckets
+// in-buckets are listed (queue side 1, slot 1, as bucket work items) for
+// k_pull_hubs, which spreads their buckets over all waves.
+template <bool MAP, bool WEIGHTS>
+__global__ void __launch_bounds__(kBlock)
+    k_reseed_pull(GraphView G, GraphView R, TravView S, uint32_t epoch) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
+  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  const uint32_t nwords = (G.n + 31) / 32;
+  uint32_t slabs = 0, edges = 0, visited = 0;
+  for (uint32_t w0 = gw * 64; w0 < nwords; w0 += W * 64) {
+    uint32_t word = w0 + lane < nwords ? S.invbits[w0 + lane] : 0u;
+    while (ballot(word != 0)) {
+      bool act = word != 0;
+      uint32_t v = 0, rb0 = 0, rnb = 0;
+      if (act) {
+        v = (w0 + lane) * 32u + uint32_t(__ffs(word) - 1);
+        word &= word - 1;
+        rb0 = R.bucket_off[v];
+        rnb = R.bucket_off[v + 1] - rb0;
+        if (rnb > kSmallBuckets) {
+          const unsigned long long r = atomicAdd(&S.cnt->large[1], (1ull << 32) | rnb);
+          S.q_large[1][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+          act = false;
+        }
+      }
+      visited += uint32_t(__popcll(ballot(act)));
+      for (uint32_t k = 0; ballot(act && rnb > k); ++k)
+        pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act && rnb > k), rb0 + k, v, slabs,
+                                   edges);
+    }
+  }
+  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
+  if (lane == 0) {
+    add_stat(S.cnt, kStatSlabs, slabs);
+    add_stat(S.cnt, kStatEdges, edges);
+    add_stat(S.cnt, kStatVisited, visited);
+    add_stat(S.cnt, kStatPushed, q.pushed);
+  }
+}
+
+// Pull re-seed of the hub vertices listed by k_reseed_pull: their in-buckets
+// are work items spread over all waves (same layout as the expand hub list).
+template <bool MAP, bool WEIGHTS>
+__global__ void __launch_bounds__(kBlock)
+    k_pull_hubs(GraphView G, GraphView R, TravView S, uint32_t epoch) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  const unsigned long long lg = S.cnt->large[1];
+  const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
+  if (n_large == 0) return;
+  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
+  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  uint32_t slabs = 0, edges = 0;
+  const uint2 *ql = S.q_large[1];
+  for (uint32_t t0 = gw * kLargeChunk; t0 < items; t0 += W * kLargeChunk) {
+    uint32_t lo = 0, hi = n_large;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ql[mid].y <= t0) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t t = t0 + lane;
+    const bool act = lane < kLargeChunk && t < items;
+    uint32_t e = lo, v = 0, b = 0;
+    if (act) {
+      while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
+      const uint2 ent = ql[e];
+      v = ent.x;
+      b = R.bucket_off[v] + (t - ent.y);
+    }
+    pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges);
+  }
+  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
+  if (lane == 0) {
+    add_stat(S.cnt, kStatSlabs, slabs);
+    add_stat(S.cnt, kStatEdges, edges);
+    add_stat(S.cnt, kStatPushed, q.pushed);
   }
 }
 
@@ -287,10 +526,13 @@ __global__ void __launch_bounds__(kBlock)
 // graph's last insert and the edge was applied (`status`, may be NULL), and
 // is searched in the graph otherwise (the graph keeps an existing edge's
 // weight), one wave-serial search per such edge.
+constexpr uint32_t kItemBlock = 1024;  // one item per thread kernels
+
 template <bool MAP, bool WEIGHTS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kItemBlock)
     k_seed_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
                  const uint32_t *wts, const uint8_t *status, uint32_t n, uint32_t epoch) {
+  __shared__ uint32_t sh[kItemBlock / 64 + 1];
   const uint32_t lane = lane_id();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t u = 0, v = 0, wgt = 1;
@@ -327,14 +569,15 @@ __global__ void __launch_bounds__(kBlock)
       if (D < dist_of(old)) push = atomicExch(S.stamp + v, epoch) != epoch;
     }
   }
-  enqueue(G, S, 0, 0, push, v);
+  block_enqueue<kItemBlock / 64>(G, S, 0, 0, push, v, sh);
 }
 
 // Decremental seed: batch edge (u, v) was deleted; if it was v's tree edge,
 // v becomes invalid (Invalidate, BFSImpl.cuh:3-15).
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kItemBlock)
     k_invalidate_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
                        uint32_t n, uint32_t source) {
+  __shared__ uint32_t sh[kItemBlock / 64 + 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool push = false;
   uint32_t v = 0;
@@ -342,7 +585,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t u = src[i];
     v = dst[i];
     if (u != v && u < G.n && v < G.n && v != source) {
-      const uint64_t cur = coherent_load64(S.dist + v);
+      const uint64_t cur = S.dist[v];
       if (cur != kInfInf && parent_of(cur) == u) {
         if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                       (unsigned long long)kInfInf) == cur) {
@@ -352,7 +595,7 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
   }
-  enqueue(G, S, 0, 0, push, v);
+  block_enqueue<kItemBlock / 64>(G, S, 0, 0, push, v, sh);
 }
 
 // Start a static run: the source at distance 0 is the frontier (side 0,
@@ -474,10 +717,22 @@ struct Traversal {
     fill_u64(reinterpret_cast<uint64_t *>(cnt->large), 0ull, 3, stream);
   }
 
-  int read_counts(Counts &h) {
+  int read_counts(Totals &h) {
     MK_HIP(hipMemcpyAsync(host.ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
     MK_HIP(hipStreamSynchronize(stream));
-    h = *host.as<Counts>();
+    const Counts &c = *host.as<Counts>();
+    for (int i = 0; i < 3; ++i) {
+      h.small[i] = c.small[i];
+      h.large[i] = c.large[i];
+    }
+    h.steps = c.steps;
+    h.slabs = h.edges = h.visited = h.pushed = 0;
+    for (uint32_t k = 0; k < kStatSlots; ++k) {
+      h.slabs += c.stats[k][kStatSlabs];
+      h.edges += c.stats[k][kStatEdges];
+      h.visited += c.stats[k][kStatVisited];
+      h.pushed += c.stats[k][kStatPushed];
+    }
     return MK_OK;
   }
 
@@ -501,12 +756,12 @@ struct Traversal {
   // read after each group.  Algorithmic bytes (profiling) per group: 256 B
   // per slab, 8 B per key (neighbour distance), 24 B per expanded vertex
   // (entry, bucket range, distance), 16 B per push (atomic, stamp, entry).
-  template <int MODE> int iterate(Counts &h) {
+  template <int MODE> int iterate(Totals &h) {
     uint32_t side = 0, slot = 0;
     uint32_t group = 2;
     uint64_t launched = 0;
     const uint64_t limit = uint64_t(g->n) + 8;
-    Counts prev;
+    Totals prev;
     if (int r = read_counts(prev)) return r;
     h = prev;
     while (h.small[slot] != 0 || (h.large[slot] >> 32) != 0) {
@@ -545,7 +800,7 @@ struct Traversal {
     float t = 0.f;
     MK_HIP(hipEventElapsedTime(&t, ev0, ev1));
     if (ms) *ms = t;
-    Counts h;
+    Totals h;
     if (int r = read_counts(h)) return r;
     last_slabs = h.slabs;
     last_edges = h.edges;
@@ -557,7 +812,7 @@ struct Traversal {
     if (int r = begin()) return r;
     fill_u64(dist, kInfInf, g->n, stream);
     k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
-    Counts h;
+    Totals h;
     if (int r = iterate<0>(h)) return r;
     return end(ms);
   }
@@ -575,18 +830,19 @@ struct Traversal {
       const uint8_t *st = same ? g->status.as<uint8_t>() : nullptr;
       {
         ProfScope prof(stream, "k_seed_batch", 24.0 * n);
+        const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
         if (weights)
-          k_seed_batch<true, true><<<blocks_for_threads(n), kBlock, 0, stream>>>(G, view(), src,
-                                                                                dst, w, st, n, ep);
+          k_seed_batch<true, true><<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst, w, st, n,
+                                                                   ep);
         else if (g->weighted)
-          k_seed_batch<true, false><<<blocks_for_threads(n), kBlock, 0, stream>>>(
-              G, view(), src, dst, nullptr, nullptr, n, ep);
+          k_seed_batch<true, false><<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst, nullptr,
+                                                                    nullptr, n, ep);
         else
-          k_seed_batch<false, false><<<blocks_for_threads(n), kBlock, 0, stream>>>(
-              G, view(), src, dst, nullptr, nullptr, n, ep);
+          k_seed_batch<false, false><<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst,
+                                                                     nullptr, nullptr, n, ep);
       }
       MK_HIP(hipGetLastError());
-      Counts h;
+      Totals h;
       if (int r = iterate<0>(h)) return r;
     }
     return end(ms);
@@ -599,22 +855,54 @@ struct Traversal {
       const GraphView G = g->view();
       {
         ProfScope prof(stream, "k_invalidate_batch", 24.0 * n);
-        k_invalidate_batch<<<blocks_for_threads(n), kBlock, 0, stream>>>(G, view(), src, dst, n,
-                                                                        source);
+        const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
+        k_invalidate_batch<<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst, n, source);
       }
       MK_HIP(hipGetLastError());
-      // propagate invalidation down the tree
-      Counts h;
+      // propagate invalidation down the tree; any step ran = something was
+      // invalidated
+      Totals h;
       if (int r = iterate<1>(h)) return r;
-      if (h.pushed != 0) {
+      if (h.steps != 0) {
         // re-seed from valid in-neighbours; the re-seeded vertices start a
         // fresh frontier at side 0 / slot 0
         const uint32_t ep = next_epoch();
         clear_slots();
         const uint64_t slabs0 = h.slabs, edges0 = h.edges, pushed0 = h.pushed;
-        const unsigned grid = blocks_for_waves(std::min<uint64_t>(
-            (uint64_t(g->total_buckets) + 63) / 64, uint64_t(max_waves) * 2));
-        {
+        if (g->rev) {
+          // pull from in-neighbours of the invalidated vertices
+          const GraphView R = g->rev->view();
+          const unsigned grid = blocks_for_waves(std::min<uint64_t>(
+              (uint64_t(g->n) / 32 + 64) / 64, uint64_t(max_waves)));
+          const unsigned hgrid = blocks_for_waves(uint64_t(max_waves));
+          {
+            ProfScope prof(stream, "k_reseed_pull", 0.0);
+            if (g->weighted) {
+              if (weights)
+                k_reseed_pull<true, true><<<grid, kBlock, 0, stream>>>(G, R, view(), ep);
+              else
+                k_reseed_pull<true, false><<<grid, kBlock, 0, stream>>>(G, R, view(), ep);
+            } else {
+              k_reseed_pull<false, false><<<grid, kBlock, 0, stream>>>(G, R, view(), ep);
+            }
+          }
+          {
+            ProfScope prof(stream, "k_reseed_pull", 0.0);  // hub half of the pull
+            if (g->weighted) {
+              if (weights)
+                k_pull_hubs<true, true><<<hgrid, kBlock, 0, stream>>>(G, R, view(), ep);
+              else
+                k_pull_hubs<true, false><<<hgrid, kBlock, 0, stream>>>(G, R, view(), ep);
+            } else {
+              k_pull_hubs<false, false><<<hgrid, kBlock, 0, stream>>>(G, R, view(), ep);
+            }
+          }
+          // the hub list used slot 1, which the first relax step appends to
+          fill_u32(cnt->small + 1, 0u, 1, stream);
+          fill_u64(reinterpret_cast<uint64_t *>(cnt->large + 1), 0ull, 1, stream);
+        } else {
+          const unsigned grid = blocks_for_waves(std::min<uint64_t>(
+              (uint64_t(g->total_buckets) + 63) / 64, uint64_t(max_waves) * 2));
           ProfScope prof(stream, "k_reseed", 0.0);
           if (g->weighted) {
             if (weights)
@@ -626,11 +914,22 @@ struct Traversal {
           }
         }
         MK_HIP(hipGetLastError());
-        if (profiling_on()) {
+        if (profiling_on() && g->rev) {
+          // 4 B per bitmap word, 8 B per invalidated vertex (in-bucket
+          // range), 256 B per in-slab, 4 + 8 B per in-neighbour (invalid bit,
+          // distance), 16 B per re-seeded vertex
+          Totals a;
+          if (int r = read_counts(a)) return r;
+          prof_add_bytes("k_reseed_pull", 4.0 * double((g->n + 31) / 32) +
+                                              8.0 * double(a.visited - h.visited) +
+                                              256.0 * double(a.slabs - slabs0) +
+                                              12.0 * double(a.edges - edges0) +
+                                              16.0 * double(a.pushed - pushed0));
+        } else if (profiling_on()) {
           // 20 B per bucket (owner, degree, distance, invalid bit of the
           // owner), 256 B per slab read, 4 B per key (invalid bit), 16 B per
           // re-seeded vertex
-          Counts a;
+          Totals a;
           if (int r = read_counts(a)) return r;
           prof_add_bytes("k_reseed", 20.0 * double(g->total_buckets) +
                                          256.0 * double(a.slabs - slabs0) +

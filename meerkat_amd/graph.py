@@ -78,20 +78,45 @@ class DynamicGraph:
     """Per-vertex slab-hash adjacency: a set (unweighted) or a map from
     destination to uint32 weight (weighted)."""
 
-    def __init__(self, n, pool, load_factor=0.7, degree_hints=None, weighted=False, device=0):
+    WEIGHTED = 1
+    REVERSE = 2
+
+    def __init__(self, n, pool, load_factor=0.7, degree_hints=None, weighted=False, device=0,
+                 reverse=False, in_degree_hints=None):
+        """reverse=True also keeps an in-edge mirror, updated by every
+        InsertEdges/DeleteEdges, which decremental BFS/SSSP pull from."""
         self.n = int(n)
         self.pool = pool
         self.weighted = bool(weighted)
+        self.reverse = bool(reverse)
         self.device = device
         self._h = _vp()
-        hints = None
-        if degree_hints is not None:
-            hints = np.ascontiguousarray(degree_hints, dtype=np.uint32)
-            if len(hints) != self.n:
-                raise ValueError("degree_hints must have n entries")
-        self._hints = hints
-        _lib.call("mk_graph_create", pool.handle, self.n, int(self.weighted), float(load_factor),
-                  None if hints is None else hints.ctypes.data_as(_vp), ctypes.byref(self._h))
+        self._owned = True
+
+        def _hint(h):
+            if h is None:
+                return None
+            h = np.ascontiguousarray(h, dtype=np.uint32)
+            if len(h) != self.n:
+                raise ValueError("degree hints must have n entries")
+            return h
+
+        hints, ihints = _hint(degree_hints), _hint(in_degree_hints)
+        self._hints = (hints, ihints)
+        flags = (self.WEIGHTED if self.weighted else 0) | (self.REVERSE if self.reverse else 0)
+        _lib.call("mk_graph_create_ex", pool.handle, self.n, flags, float(load_factor),
+                  None if hints is None else hints.ctypes.data_as(_vp),
+                  None if ihints is None else ihints.ctypes.data_as(_vp), ctypes.byref(self._h))
+
+    def mirror(self):
+        """Read-only view of the in-edge mirror (inspection calls only)."""
+        h = _vp()
+        _lib.call("mk_graph_reverse", self._h, ctypes.byref(h))
+        m = DynamicGraph.__new__(DynamicGraph)
+        m.n, m.pool, m.weighted, m.reverse, m.device = self.n, self.pool, self.weighted, False, \
+            self.device
+        m._h, m._owned, m._hints, m._parent = h, False, (None, None), self
+        return m
 
     @property
     def handle(self):
@@ -183,13 +208,13 @@ class DynamicGraph:
         return {"head_slabs": h.value, "overflow_slabs": o.value, "max_chain": mx.value}
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             _lib.call("mk_graph_destroy", self._h)
-            self._h = _vp()
+        self._h = _vp()
 
     def __del__(self):
         try:
-            if self._h:
+            if self._h and self._owned:
                 _lib.lib().mk_graph_destroy(self._h)
         except Exception:
             pass

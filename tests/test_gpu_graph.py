@@ -179,3 +179,43 @@ def test_empty_batches_and_handle_order(gpu):
     assert ei.value.code == _lib.MK_ERR_STATE
     g.close()
     pool.close()
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_in_edge_mirror_tracks_every_mutation(gpu, weighted):
+    n, s, d, w = gen.rmat(12, 16, seed=21, weighted=True)
+    w = w if weighted else None
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=weighted, reverse=True,
+                        in_degree_hints=gen.degree_hints(n, d))
+    g.InsertEdges(s, d, w)
+    rng = np.random.default_rng(5)
+    mark = rng.random(len(s)) < 0.3
+    g.DeleteEdges(s[mark], d[mark])
+    bs, bd, bw = gen.insert_batch(gen.keys_of(s, d), n, 3000, seed=6, weighted=True)
+    g.InsertEdges(bs, bd, bw if weighted else None)
+    S = np.r_[s[~mark], bs]
+    D = np.r_[d[~mark], bd]
+    Wt = None if not weighted else np.r_[w[~mark], bw]
+    fwd = oracle.csr(n, S, D, Wt)
+    rev = oracle.csr(n, D, S, Wt)  # transpose, with the same weights
+    assert_same_adjacency(g, fwd)
+    assert_same_adjacency(g.mirror(), rev)
+    assert np.array_equal(g.mirror().degrees(), rev.degrees())
+    with pytest.raises(mk.MeerkatError) as ei:  # the mirror cannot be mutated directly
+        _lib.call("mk_graph_insert", g.mirror().handle, None, None, None, 0, None)
+    assert ei.value.code == _lib.MK_ERR_STATE
+
+
+def test_mirror_rolls_back_with_the_graph(gpu):
+    n = 200000
+    pool = mk.SlabPool(2048 * 256)  # one block: the hub chain needs 2381 slabs
+    g = mk.DynamicGraph(n, pool, 0.7, None, reverse=True)
+    g.InsertEdges(np.zeros(500, np.uint32), np.arange(1, 501, dtype=np.uint32))
+    before_f, before_r = g.export_csr(), g.mirror().export_csr()
+    with pytest.raises(mk.MeerkatError) as ei:
+        g.InsertEdges(np.zeros(150000, np.uint32), np.arange(1, 150001, dtype=np.uint32))
+    assert ei.value.code == _lib.MK_ERR_POOL_FULL
+    after_f, after_r = g.export_csr(), g.mirror().export_csr()
+    assert np.array_equal(before_f[1], after_f[1]) and np.array_equal(before_r[1], after_r[1])
+    assert np.array_equal(before_r[0], after_r[0])

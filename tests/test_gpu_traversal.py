@@ -97,10 +97,12 @@ def test_bfs_incremental_config1_shape(gpu):
         check_bfs(b, es, n, 0)
 
 
-def test_bfs_decremental_and_mixed(gpu):
+@pytest.mark.parametrize("reverse", [False, True])
+def test_bfs_decremental_and_mixed(gpu, reverse):
     n, s, d, _ = gen.rmat(14, 8, seed=2)
     pool = mk.SlabPool(1 << 28)
-    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s))
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), reverse=reverse,
+                        in_degree_hints=gen.degree_hints(n, d))
     g.InsertEdges(s, d)
     es = EdgeSet(s, d)
     b = mk.BFS(g, 0)
@@ -118,14 +120,15 @@ def test_bfs_decremental_and_mixed(gpu):
         check_bfs(b, es, n, 0)
 
 
-def test_bfs_decremental_cuts_off_subtrees(gpu):
+@pytest.mark.parametrize("reverse", [False, True])
+def test_bfs_decremental_cuts_off_subtrees(gpu, reverse):
     # a path 0-1-2-...-99 plus a shortcut 0->50: deleting 0->50 pushes the
-    # tail back to path levels; deleting 10->11 disconnects 11..49
+    # tail back to path levels; deleting 10->11 disconnects 11..99
     n = 100
     s = np.r_[np.arange(99), 0].astype(np.uint32)
     d = np.r_[np.arange(1, 100), 50].astype(np.uint32)
     pool = mk.SlabPool(1 << 24)
-    g = mk.DynamicGraph(n, pool, 0.7, None)
+    g = mk.DynamicGraph(n, pool, 0.7, None, reverse=reverse)
     g.InsertEdges(s, d)
     es = EdgeSet(s, d)
     b = mk.BFS(g, 0)
@@ -158,12 +161,14 @@ def test_incremental_empty_and_selfloop_batches(gpu):
     check_bfs(b, EdgeSet(s, d), n, 0)
 
 
-def test_sssp_static_incremental_decremental_config2_shape(gpu):
+@pytest.mark.parametrize("reverse", [False, True])
+def test_sssp_static_incremental_decremental_config2_shape(gpu, reverse):
     # config 2 shape at scale 15: mixed batches applied as delete ->
     # Decremental, then insert -> Incremental
     n, s, d, w = gen.rmat(15, 16, seed=3, weighted=True)
     pool = mk.SlabPool(1 << 29)
-    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=reverse,
+                        in_degree_hints=gen.degree_hints(n, d))
     g.InsertEdges(s, d, w)
     es = EdgeSet(s, d, w)
     sp = mk.SSSP(g, 0)
