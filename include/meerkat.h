@@ -178,6 +178,32 @@ int mk_sssp_read(mk_sssp *s, uint32_t *dist_host, uint32_t *parent_host);
 int mk_sssp_last_stats(mk_sssp *s, uint32_t *iterations, uint64_t *slabs_read,
                        uint64_t *edges_read);
 
+/* ------------------------------------------------------ A12 PageRank
+ * PageRank(G, DampingFactor, ErrorMargin, N, VertexOutDegree, MaxIter)
+ * (include/Algorithms/PageRank.cuh:32-67; impl/PageRankImpl.cuh:146-251).
+ * g is either a graph of reversed edges, as the reference's RevUnweighted
+ * policy graph (the caller inserts (dst, src)), with host_outdeg giving the
+ * out-degrees (NULL: counted from g); or a graph created with
+ * MK_GRAPH_REVERSE, whose in-edge mirror is used and whose live out-degrees
+ * are exact (host_outdeg ignored).  Ranks are FP32, reductions FP64; every
+ * compute call continues from the current ranks (initially 1/N) and stops
+ * when the L1 delta <= error_margin or after max_iter iterations, as the
+ * reference loop (:173).  Incremental / Decremental take the batch just
+ * inserted / deleted (original direction) and fix the out-degrees first
+ * (CheckPresent, :64-87). */
+int mk_pagerank_create(mk_graph *g, float damping, float error_margin, uint32_t max_iter,
+                       const uint32_t *host_outdeg, mk_pagerank **out);
+int mk_pagerank_destroy(mk_pagerank *p);
+int mk_pagerank_static(mk_pagerank *p, float *ms);
+int mk_pagerank_incremental(mk_pagerank *p, const uint32_t *src_dev, const uint32_t *dst_dev,
+                            uint32_t n, float *ms);
+int mk_pagerank_decremental(mk_pagerank *p, const uint32_t *src_dev, const uint32_t *dst_dev,
+                            uint32_t n, float *ms);
+/* Values() (PageRank.cuh:50-55): n FP32 ranks. */
+int mk_pagerank_read(mk_pagerank *p, float *values_host);
+/* Iterations run and final L1 delta of the last compute call. */
+int mk_pagerank_last_stats(mk_pagerank *p, uint32_t *iterations, double *last_delta);
+
 /* ------------------------------------------------------ kernel profiling
  * No reference counterpart (the reference times whole calls with
  * cudaEvent, BFSImpl.cuh:644-661).  When enabled, every instrumented kernel

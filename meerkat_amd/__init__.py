@@ -6,7 +6,7 @@ tests.  Importing it does not load the library; the first call does, and
 fails loudly if it is not built.
 """
 from ._lib import MeerkatError, header_symbols, lib
-from .algorithms import BFS, INF, SSSP
+from .algorithms import BFS, INF, SSSP, PageRank
 from .graph import DynamicGraph, SlabPool, device_u32
 from . import gen
 
@@ -15,6 +15,7 @@ __all__ = [
     "DynamicGraph",
     "INF",
     "MeerkatError",
+    "PageRank",
     "SSSP",
     "SlabPool",
     "device_u32",

@@ -89,6 +89,13 @@ _PROTOS = {
     "mk_sssp_decremental": [vp, vp, vp, vp, u32, P(f32)],
     "mk_sssp_read": [vp, vp, vp],
     "mk_sssp_last_stats": [vp, P(u32), P(u64), P(u64)],
+    "mk_pagerank_create": [vp, f32, f32, u32, vp, P(vp)],
+    "mk_pagerank_destroy": [vp],
+    "mk_pagerank_static": [vp, P(f32)],
+    "mk_pagerank_incremental": [vp, vp, vp, u32, P(f32)],
+    "mk_pagerank_decremental": [vp, vp, vp, u32, P(f32)],
+    "mk_pagerank_read": [vp, vp],
+    "mk_pagerank_last_stats": [vp, P(u32), P(f64)],
     "mk_profile_enable": [i32],
     "mk_profile_reset": [],
     "mk_profile_count": [P(i32)],

@@ -101,3 +101,62 @@ class SSSP(_Traversal):
         _lib.call("mk_sssp_decremental", self._h, dptr(s), dptr(d), dptr(w), len(s),
                   ctypes.byref(ms))
         return ms.value
+
+
+class PageRank:
+    """FP32 PageRank with FP64 reductions (include/Algorithms/PageRank.cuh:9-68).
+
+    ``graph`` holds reversed edges (reference style; pass ``out_degree``) or
+    was created with ``reverse=True`` (its in-edge mirror is used and the
+    out-degrees are exact)."""
+
+    def __init__(self, graph, damping=0.85, error_margin=1e-7, out_degree=None, max_iter=100):
+        self.G = graph
+        self._h = _vp()
+        od = None
+        if out_degree is not None:
+            od = np.ascontiguousarray(out_degree, dtype=np.uint32)
+            if len(od) != graph.n:
+                raise ValueError("out_degree must have n entries")
+        _lib.call("mk_pagerank_create", graph.handle, float(damping), float(error_margin),
+                  int(max_iter), None if od is None else od.ctypes.data_as(_vp),
+                  ctypes.byref(self._h))
+
+    def Static(self):
+        ms = ctypes.c_float()
+        _lib.call("mk_pagerank_static", self._h, ctypes.byref(ms))
+        return ms.value
+
+    def Incremental(self, src, dst):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_pagerank_incremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+    def Decremental(self, src, dst):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_pagerank_decremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+    def Values(self):
+        out = np.empty(self.G.n, np.float32)
+        _lib.call("mk_pagerank_read", self._h, out.ctypes.data_as(_vp))
+        return out
+
+    def last_stats(self):
+        it, delta = ctypes.c_uint32(), ctypes.c_double()
+        _lib.call("mk_pagerank_last_stats", self._h, ctypes.byref(it), ctypes.byref(delta))
+        return {"iterations": it.value, "delta": delta.value}
+
+    def close(self):
+        if self._h:
+            _lib.call("mk_pagerank_destroy", self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().mk_pagerank_destroy(self._h)
+        except Exception:
+            pass
