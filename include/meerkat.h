@@ -204,6 +204,22 @@ int mk_pagerank_read(mk_pagerank *p, float *values_host);
 /* Iterations run and final L1 delta of the last compute call. */
 int mk_pagerank_last_stats(mk_pagerank *p, uint32_t *iterations, double *last_delta);
 
+/* ------------------------------------------------------ A14 CC
+ * Connected components (edges taken as undirected) by concurrent
+ * union-find.  The reference has only primitives (include/Util/
+ * UnionFind.cuh:24-253; UnionAsync :122-149 links the larger root under the
+ * smaller), no caller and no driver.  Labels are canonical: the smallest
+ * vertex id of each component.  Incremental unions the inserted batch;
+ * Decremental recomputes from the graph (union-find cannot split). */
+int mk_cc_create(mk_graph *g, mk_cc **out);
+int mk_cc_destroy(mk_cc *c);
+int mk_cc_static(mk_cc *c, float *ms);
+int mk_cc_incremental(mk_cc *c, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                      float *ms);
+int mk_cc_decremental(mk_cc *c, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                      float *ms);
+int mk_cc_labels(mk_cc *c, uint32_t *labels_host);
+
 /* ------------------------------------------------------ kernel profiling
  * No reference counterpart (the reference times whole calls with
  * cudaEvent, BFSImpl.cuh:644-661).  When enabled, every instrumented kernel

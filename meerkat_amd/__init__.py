@@ -6,12 +6,13 @@ tests.  Importing it does not load the library; the first call does, and
 fails loudly if it is not built.
 """
 from ._lib import MeerkatError, header_symbols, lib
-from .algorithms import BFS, INF, SSSP, PageRank
+from .algorithms import BFS, CC, INF, SSSP, PageRank
 from .graph import DynamicGraph, SlabPool, device_u32
 from . import gen
 
 __all__ = [
     "BFS",
+    "CC",
     "DynamicGraph",
     "INF",
     "MeerkatError",

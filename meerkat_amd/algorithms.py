@@ -160,3 +160,47 @@ class PageRank:
                 _lib.lib().mk_pagerank_destroy(self._h)
         except Exception:
             pass
+
+
+class CC:
+    """Connected components with min-vertex labels by union-find (the
+    reference's include/Util/UnionFind.cuh primitives; no reference driver)."""
+
+    def __init__(self, graph):
+        self.G = graph
+        self._h = _vp()
+        _lib.call("mk_cc_create", graph.handle, ctypes.byref(self._h))
+
+    def Static(self):
+        ms = ctypes.c_float()
+        _lib.call("mk_cc_static", self._h, ctypes.byref(ms))
+        return ms.value
+
+    def Incremental(self, src, dst):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_cc_incremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+    def Decremental(self, src, dst):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_cc_decremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+    def Labels(self):
+        out = np.empty(self.G.n, np.uint32)
+        _lib.call("mk_cc_labels", self._h, out.ctypes.data_as(_vp))
+        return out
+
+    def close(self):
+        if self._h:
+            _lib.call("mk_cc_destroy", self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().mk_cc_destroy(self._h)
+        except Exception:
+            pass
