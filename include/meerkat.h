@@ -204,6 +204,27 @@ int mk_pagerank_read(mk_pagerank *p, float *values_host);
 /* Iterations run and final L1 delta of the last compute call. */
 int mk_pagerank_last_stats(mk_pagerank *p, uint32_t *iterations, double *last_delta);
 
+/* ------------------------------------------------------ A13 TC
+ * TC(G, N) (include/Algorithms/TC.cuh:28-40; impl/TCImpl.cuh) on an
+ * undirected graph stored in both directions.  Static counts every triangle
+ * once (the reference's /6, :368); Incremental / Decremental take the batch
+ * just inserted / deleted, in both directions, and update the count exactly
+ * (the reference combines three counts in closed form, :488-489, 556-557).
+ * The batch's changed edges are those the graph's last mutation applied
+ * when it is the same batch (same device pointers and size); otherwise
+ * every non-loop entry is taken as changed, as the reference assumes.  The
+ * reference's Allocator argument (used there for a temporary batch graph,
+ * :457) has no counterpart: no temporary graph is built. */
+int mk_tc_create(mk_graph *g, mk_tc **out);
+int mk_tc_destroy(mk_tc *t);
+int mk_tc_static(mk_tc *t, float *ms);
+int mk_tc_incremental(mk_tc *t, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                      float *ms);
+int mk_tc_decremental(mk_tc *t, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
+                      float *ms);
+/* GetCount() (TC.cuh:30). */
+int mk_tc_count(mk_tc *t, uint64_t *count);
+
 /* ------------------------------------------------------ A14 CC
  * Connected components (edges taken as undirected) by concurrent
  * union-find.  The reference has only primitives (include/Util/

@@ -6,7 +6,7 @@ tests.  Importing it does not load the library; the first call does, and
 fails loudly if it is not built.
 """
 from ._lib import MeerkatError, header_symbols, lib
-from .algorithms import BFS, CC, INF, SSSP, PageRank
+from .algorithms import BFS, CC, INF, SSSP, TC, PageRank
 from .graph import DynamicGraph, SlabPool, device_u32
 from . import gen
 
@@ -19,6 +19,7 @@ __all__ = [
     "PageRank",
     "SSSP",
     "SlabPool",
+    "TC",
     "device_u32",
     "gen",
     "header_symbols",

@@ -96,6 +96,12 @@ _PROTOS = {
     "mk_pagerank_decremental": [vp, vp, vp, u32, P(f32)],
     "mk_pagerank_read": [vp, vp],
     "mk_pagerank_last_stats": [vp, P(u32), P(f64)],
+    "mk_tc_create": [vp, P(vp)],
+    "mk_tc_destroy": [vp],
+    "mk_tc_static": [vp, P(f32)],
+    "mk_tc_incremental": [vp, vp, vp, u32, P(f32)],
+    "mk_tc_decremental": [vp, vp, vp, u32, P(f32)],
+    "mk_tc_count": [vp, P(u64)],
     "mk_cc_create": [vp, P(vp)],
     "mk_cc_destroy": [vp],
     "mk_cc_static": [vp, P(f32)],

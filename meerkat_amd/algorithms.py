@@ -204,3 +204,47 @@ class CC:
                 _lib.lib().mk_cc_destroy(self._h)
         except Exception:
             pass
+
+
+class TC:
+    """Triangle counting on an undirected graph stored in both directions
+    (include/Algorithms/TC.cuh:9-41).  Batches must hold both directions."""
+
+    def __init__(self, graph):
+        self.G = graph
+        self._h = _vp()
+        _lib.call("mk_tc_create", graph.handle, ctypes.byref(self._h))
+
+    def Static(self, edges_n=0):
+        ms = ctypes.c_float()
+        _lib.call("mk_tc_static", self._h, ctypes.byref(ms))
+        return ms.value
+
+    def Incremental(self, src, dst, alloc=None):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_tc_incremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+    def Decremental(self, src, dst, alloc=None):
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_tc_decremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        return ms.value
+
+    def GetCount(self):
+        c = ctypes.c_uint64()
+        _lib.call("mk_tc_count", self._h, ctypes.byref(c))
+        return c.value
+
+    def close(self):
+        if self._h:
+            _lib.call("mk_tc_destroy", self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().mk_tc_destroy(self._h)
+        except Exception:
+            pass
