@@ -241,6 +241,20 @@ int mk_cc_decremental(mk_cc *c, const uint32_t *src_dev, const uint32_t *dst_dev
                       float *ms);
 int mk_cc_labels(mk_cc *c, uint32_t *labels_host);
 
+/* ------------------------------------------------------ A17 multi-source
+ * Up to 64 BFS (sssp = 0) or SSSP (sssp != 0, weighted graph) queries
+ * answered by one traversal, for the serving loop's batching.  No reference
+ * counterpart (the reference answers one source per BFS/SSSP object,
+ * include/Algorithms/BFS.cuh:34-53).  Source slot s is bit s of per-vertex
+ * masks; sources may repeat.  run blocks until all k traversals are done;
+ * read returns slot s's distances (UINT32_MAX = unreachable), identical to
+ * a single-source static BFS / SSSP from that source. */
+int mk_msq_create(mk_graph *g, int sssp, mk_msq **out);
+int mk_msq_destroy(mk_msq *q);
+int mk_msq_run(mk_msq *q, const uint32_t *sources_host, uint32_t k, float *ms);
+int mk_msq_read(mk_msq *q, uint32_t slot, uint32_t *dist_host);
+int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
+
 /* ------------------------------------------------------ kernel profiling
  * No reference counterpart (the reference times whole calls with
  * cudaEvent, BFSImpl.cuh:644-661).  When enabled, every instrumented kernel

@@ -6,7 +6,7 @@ tests.  Importing it does not load the library; the first call does, and
 fails loudly if it is not built.
 """
 from ._lib import MeerkatError, header_symbols, lib
-from .algorithms import BFS, CC, INF, SSSP, TC, PageRank
+from .algorithms import BFS, CC, INF, SSSP, TC, MultiSource, PageRank
 from .graph import DynamicGraph, SlabPool, device_u32
 from . import gen
 
@@ -16,6 +16,7 @@ __all__ = [
     "DynamicGraph",
     "INF",
     "MeerkatError",
+    "MultiSource",
     "PageRank",
     "SSSP",
     "SlabPool",

@@ -108,6 +108,11 @@ _PROTOS = {
     "mk_cc_incremental": [vp, vp, vp, u32, P(f32)],
     "mk_cc_decremental": [vp, vp, vp, u32, P(f32)],
     "mk_cc_labels": [vp, vp],
+    "mk_msq_create": [vp, i32, P(vp)],
+    "mk_msq_destroy": [vp],
+    "mk_msq_run": [vp, vp, u32, P(f32)],
+    "mk_msq_read": [vp, u32, vp],
+    "mk_msq_last_stats": [vp, P(u32), P(u64)],
     "mk_profile_enable": [i32],
     "mk_profile_reset": [],
     "mk_profile_count": [P(i32)],

@@ -248,3 +248,43 @@ class TC:
                 _lib.lib().mk_tc_destroy(self._h)
         except Exception:
             pass
+
+
+class MultiSource:
+    """Up to 64 BFS (sssp=False) or SSSP queries answered by one traversal."""
+
+    MAX_SOURCES = 64
+
+    def __init__(self, graph, sssp=False):
+        self.G = graph
+        self.sssp = bool(sssp)
+        self._h = _vp()
+        _lib.call("mk_msq_create", graph.handle, int(self.sssp), ctypes.byref(self._h))
+
+    def run(self, sources):
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        ms = ctypes.c_float()
+        _lib.call("mk_msq_run", self._h, src.ctypes.data_as(_vp), len(src), ctypes.byref(ms))
+        return ms.value
+
+    def distances(self, slot):
+        out = np.empty(self.G.n, np.uint32)
+        _lib.call("mk_msq_read", self._h, int(slot), out.ctypes.data_as(_vp))
+        return out
+
+    def last_stats(self):
+        st, e = ctypes.c_uint32(), ctypes.c_uint64()
+        _lib.call("mk_msq_last_stats", self._h, ctypes.byref(st), ctypes.byref(e))
+        return {"steps": st.value, "edges_read": e.value}
+
+    def close(self):
+        if self._h:
+            _lib.call("mk_msq_destroy", self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().mk_msq_destroy(self._h)
+        except Exception:
+            pass

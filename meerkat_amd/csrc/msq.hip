@@ -1,0 +1,680 @@
+// Multi-source BFS / SSSP: up to 64 queries answered by one traversal.
+//
+// No reference counterpart (the reference runs one source per BFS/SSSP
+// object, include/Algorithms/BFS.cuh:34).  Source slot s (0..63) is bit s of
+// 64-bit per-vertex masks:
+//  - BFS: seen[v] holds the sources that reached v; F[L][v] the sources for
+//    which v is at level L.  A level step expands every frontier vertex u
+//    once for all its sources: per edge (u, v) new = F[L][u] & ~seen[v] is
+//    claimed with one 64-bit atomicOr, so 64 BFS advance together.  Every
+//    frontier entry (v, F[L][v]) stays in an append-only per-level log, from
+//    which the distances of one source are materialised on demand (each
+//    (v, s) bit is set at exactly one level).
+//  - SSSP: dist is stored source-minor (dist[v * 64 + s]) so the 64 lanes of
+//    a wave relax one edge for all sources with one coalesced 256-byte
+//    access; F[i][v] holds the sources whose distance to v improved in
+//    step i (label-correcting, as the single-source SSSP).
+// Frontier bookkeeping follows the single-source traversal: counters in
+// three rotating slots read on the device, hub vertices (more than
+// kMsSmall buckets) split into bucket work items over all waves, and the
+// host reading counters only every few steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "profile.hpp"
+#include "slabhash.hpp"
+
+namespace mk {
+
+constexpr uint32_t kMsSmall = 2;
+constexpr uint32_t kMsChunk = 8;
+
+// BFS rotates three slots (its queue is the append-only log); SSSP rotates
+// four, because a step clears the masks of the previous queue (slot i-1)
+// and the counters of the one before (slot i-2).
+constexpr uint32_t kMsSlots = 4;
+
+struct MsCounts {
+  uint32_t cnt[kMsSlots];          // small entries per slot
+  uint32_t steps;                  // non-empty steps
+  uint32_t pad0;
+  unsigned long long large[kMsSlots];  // (entries << 32) | items per slot
+  uint32_t overflow;               // log capacity exceeded
+  uint32_t pad;
+  unsigned long long edges;        // adjacency entries read
+};
+
+struct MsView {
+  uint64_t *seen;      // BFS
+  uint64_t *F[kMsSlots];  // per-step masks
+  uint32_t *dist;      // SSSP, n * 64
+  uint32_t *logv;      // BFS log / SSSP queues (kMsSlots * n)
+  uint64_t *logm;      // BFS log masks
+  uint32_t *seg;       // BFS: first log index of each level
+  uint32_t log_cap;
+  uint2 *ql[kMsSlots];  // hub lists
+  MsCounts *c;
+  uint32_t n;
+};
+
+MK_DEV uint64_t read_lane64(uint64_t x, uint32_t l) {
+  return (uint64_t(read_lane(uint32_t(x >> 32), l)) << 32) | read_lane(uint32_t(x), l);
+}
+
+// Append v (lanes with `push`) to the list `list` at base + atomic count
+// `cnt`, and hub vertices also to the hub list `ql` / `large`.  Entries past
+// `cap` set the overflow flag instead of being written.
+MK_DEV void ms_push(const GraphView &G, const MsView &S, bool push, uint32_t v, uint32_t *list,
+                    uint32_t base, uint32_t *cnt, uint2 *ql, unsigned long long *large,
+                    uint32_t cap, bool log_all) {
+  const uint64_t pm = ballot(push);
+  if (pm == 0) return;
+  uint32_t nb = 0;
+  if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
+  const bool hub = push && nb > kMsSmall;
+  const bool to_list = push && (log_all || !hub);
+  const uint64_t lm = ballot(to_list);
+  if (lm) {
+    uint32_t pos = 0;
+    const uint32_t leader = ffs64(lm);
+    if (lane_id() == leader) pos = atomicAdd(cnt, uint32_t(__popcll(lm)));
+    pos = read_lane(pos, leader) + lanes_below(lm);
+    if (to_list) {
+      if (base + pos < cap) list[base + pos] = v;
+      else atomicOr(&S.c->overflow, 1u);
+    }
+  }
+  if (hub) {
+    const unsigned long long r = atomicAdd(large, (1ull << 32) | nb);
+    ql[uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+  }
+}
+
+// ------------------------------------------------------------------ BFS
+// BFS work on one slab of u (mask mu): claim new source bits of each
+// neighbour; a vertex entering level L+1 is pushed once (first bits).
+MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, uint64_t key_mask_all, uint32_t x,
+                       uint64_t mu, uint64_t *Fn, uint32_t nbase, uint32_t nslot,
+                       uint32_t &edges) {
+  const uint32_t lane = lane_id();
+  const bool live = ((key_mask_all >> lane) & 1ull) && x < kTomb;
+  edges += uint32_t(__popcll(ballot(live)));
+  bool push = false;
+  if (live) {
+    const uint64_t nw = mu & ~S.seen[x];
+    if (nw) {
+      const uint64_t old = atomicOr((unsigned long long *)(S.seen + x), (unsigned long long)nw);
+      const uint64_t mine = nw & ~old;
+      if (mine) push = atomicOr((unsigned long long *)(Fn + x), (unsigned long long)mine) == 0;
+    }
+  }
+  ms_push(G, S, push, x, S.logv, nbase, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
+          S.log_cap, true);
+}
+
+template <bool MAP>
+MK_DEV void msbfs_chain(const GraphView &G, const MsView &S, uint32_t x, uint64_t mu,
+                        uint64_t *Fn, uint32_t nbase, uint32_t nslot, uint32_t &edges) {
+  for (;;) {
+    msbfs_slab(G, S, key_mask<MAP>(), x, mu, Fn, nbase, nslot, edges);
+    const uint32_t nx = read_lane(x, kNextLane);
+    if (nx == kEmpty) break;
+    x = slab_ptr(G.pool, nx)[lane_id()];
+  }
+}
+
+// Level L: log segment [seg[L], seg[L] + cnt[L%3]) plus the hub list of
+// slot L%3.  Appends to the log after it and to the hub list of slot
+// (L+1)%3; clears slot (L+2)%3 and the masks of level L-1.
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, uint32_t L) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t slot = L % 3, nslot = (L + 1) % 3, pslot = (L + 2) % 3;
+  const uint32_t start = S.seg[L];
+  const uint32_t count = S.c->cnt[slot];
+  const uint32_t nbase = start + count;
+  const unsigned long long lg = S.c->large[slot];
+  const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.seg[L + 1] = nbase;
+    S.c->cnt[pslot] = 0;
+    S.c->large[pslot] = 0ull;
+    if (count) atomicAdd(&S.c->steps, 1u);
+  }
+  if (count == 0) return;
+  uint64_t *Fc = S.F[slot], *Fn = S.F[nslot], *Fp = S.F[pslot];
+  // after a log overflow (the run is repeated) only stored entries are read
+  const uint32_t end = min(nbase, S.log_cap);
+  // masks of level L-1 are no longer read: clear them for level L+2
+  if (L > 0) {
+    const uint32_t ps = S.seg[L - 1];
+    for (uint32_t i = ps + gw * 64 + lane; i < min(start, S.log_cap); i += W * 64)
+      Fp[S.logv[i]] = 0ull;
+  }
+  uint32_t edges = 0;
+  // hubs: bucket work items
+  const uint2 *ql = S.ql[slot];
+  for (uint32_t t0 = gw * kMsChunk; t0 < items; t0 += W * kMsChunk) {
+    uint32_t lo = 0, hi = n_large;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ql[mid].y <= t0) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t t = t0 + lane;
+    const bool act = lane < kMsChunk && t < items;
+    uint32_t e = lo, b = 0;
+    uint64_t mu = 0;
+    if (act) {
+      while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
+      const uint2 ent = ql[e];
+      b = G.bucket_off[ent.x] + (t - ent.y);
+      mu = Fc[ent.x];
+    }
+    uint64_t m = ballot(act);
+    while (m) {
+      const uint32_t j = ffs64(m);
+      m &= m - 1;
+      const uint32_t x = G.heads[size_t(read_lane(b, j)) * kSlabWords + lane];
+      msbfs_chain<MAP>(G, S, x, read_lane64(mu, j), Fn, nbase, nslot, edges);
+    }
+  }
+  // log entries: record the mask; expand non-hub vertices
+  for (uint32_t i0 = start + gw * 64; i0 < end; i0 += W * 64) {
+    const uint32_t i = i0 + lane;
+    bool act = i < end;
+    uint32_t u = 0, b0 = 0, nb = 0;
+    uint64_t mu = 0;
+    if (act) {
+      u = S.logv[i];
+      mu = Fc[u];
+      S.logm[i] = mu;
+      b0 = G.bucket_off[u];
+      nb = G.bucket_off[u + 1] - b0;
+      if (nb > kMsSmall) act = false;
+    }
+    for (uint32_t k = 0; ballot(act && nb > k); ++k) {
+      uint64_t m = ballot(act && nb > k);
+      while (m) {
+        uint32_t js[4];
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          js[q] = m ? ffs64(m) : 0u;
+          if (m) {
+            m &= m - 1;
+            c = q + 1;
+          }
+        }
+        uint32_t x[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q)
+          x[q] = q < c ? G.heads[size_t(read_lane(b0, js[q]) + k) * kSlabWords + lane] : kEmpty;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q)
+          if (q < c) msbfs_chain<MAP>(G, S, x[q], read_lane64(mu, js[q]), Fn, nbase, nslot, edges);
+      }
+    }
+  }
+  if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
+}
+
+__global__ void k_msbfs_init(MsView S, const uint32_t *sources, uint32_t k) {
+  // one thread: sources may repeat (same vertex in two slots)
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint32_t cntv = 0;
+  for (uint32_t s = 0; s < k; ++s) {
+    const uint32_t v = sources[s];
+    const uint64_t bit = 1ull << s;
+    if (S.F[0][v] == 0) S.logv[cntv++] = v;
+    S.F[0][v] |= bit;
+    S.seen[v] |= bit;
+  }
+  S.seg[0] = 0;
+  S.c->cnt[0] = cntv;
+}
+
+// Hub vertices among the sources go to the hub list of slot 0.
+__global__ void k_ms_source_hubs(GraphView G, MsView S) {
+  const uint32_t i = threadIdx.x;
+  if (i >= S.c->cnt[0]) return;
+  const uint32_t v = S.logv[i];
+  const uint32_t nb = G.bucket_off[v + 1] - G.bucket_off[v];
+  if (nb > kMsSmall) {
+    const unsigned long long r = atomicAdd(&S.c->large[0], (1ull << 32) | nb);
+    S.ql[0][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+  }
+}
+
+// Distances of slot s from the log: the level whose mask holds bit s.
+__global__ void k_msbfs_read(MsView S, uint32_t s, uint32_t total, uint32_t levels,
+                             uint32_t *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  if (!((S.logm[i] >> s) & 1ull)) return;
+  uint32_t lo = 0, hi = levels;  // last level with seg <= i
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (S.seg[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  out[S.logv[i]] = lo;
+}
+
+// ------------------------------------------------------------------ SSSP
+// Relax one slab of u for all sources in mu: per live key (neighbour v,
+// weight w), lanes are sources; four neighbours' distance rows are loaded at
+// a time.
+template <bool MAP>
+MK_DEV void mssp_slab(const GraphView &G, const MsView &S, uint32_t x, uint32_t du_lane,
+                      uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
+  const uint32_t lane = lane_id();
+  const uint64_t km = key_mask<MAP>();
+  uint64_t live = ballot(x < kTomb) & km;
+  edges += uint32_t(__popcll(live));
+  const bool mine = (mu >> lane) & 1ull;
+  while (live) {
+    uint32_t ks[4];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      ks[q] = live ? ffs64(live) : 0u;
+      if (live) {
+        live &= live - 1;
+        c = q + 1;
+      }
+    }
+    uint32_t v[4], w[4], cur[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      v[q] = read_lane(x, ks[q]);
+      w[q] = MAP ? read_lane(x, ks[q] + 1) : 1u;
+      cur[q] = q < c ? S.dist[size_t(v[q]) * 64 + lane] : 0u;
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      if (q >= c) continue;
+      const uint32_t cand = du_lane + w[q];
+      bool imp = mine && du_lane != kInf && cand >= du_lane && cand < cur[q];
+      if (imp) {
+        const uint32_t old = atomicMin(S.dist + size_t(v[q]) * 64 + lane, cand);
+        imp = cand < old;
+      }
+      const uint64_t im = ballot(imp);
+      bool push = false;
+      if (im && lane == 0)
+        push = atomicOr((unsigned long long *)(Fn + v[q]), (unsigned long long)im) == 0;
+      ms_push(G, S, push, v[q], qn, 0, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
+              S.n, false);
+    }
+  }
+}
+
+template <bool MAP>
+MK_DEV void mssp_chain(const GraphView &G, const MsView &S, uint32_t x, uint32_t u, uint64_t mu,
+                       uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
+  const uint32_t du = S.dist[size_t(u) * 64 + lane_id()];
+  for (;;) {
+    mssp_slab<MAP>(G, S, x, du, mu, Fn, qn, nslot, edges);
+    const uint32_t nx = read_lane(x, kNextLane);
+    if (nx == kEmpty) break;
+    x = slab_ptr(G.pool, nx)[lane_id()];
+  }
+}
+
+// SSSP step i: queue slot i%4 (vertices, masks F[i%4]); appends to slot
+// (i+1)%4; clears the masks of queue i-1 (slot (i+3)%4) and the counters of
+// slot (i+2)%4 (queue i-2, whose masks step i-1 cleared).
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uint32_t i) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t slot = i % kMsSlots, nslot = (i + 1) % kMsSlots;
+  const uint32_t zslot = (i + 2) % kMsSlots, pslot = (i + 3) % kMsSlots;
+  const uint32_t count = S.c->cnt[slot];
+  const unsigned long long lg = S.c->large[slot];
+  const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
+  uint32_t *qc = S.logv + size_t(slot) * S.n, *qn = S.logv + size_t(nslot) * S.n;
+  const uint32_t *qp = S.logv + size_t(pslot) * S.n;
+  uint64_t *Fc = S.F[slot], *Fn = S.F[nslot], *Fp = S.F[pslot];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.c->cnt[zslot] = 0;
+    S.c->large[zslot] = 0ull;
+  }
+  // queue i-1 is done: clear its masks before its slot is reused
+  const uint32_t pcount = i > 0 ? S.c->cnt[pslot] : 0u;
+  const uint32_t plarge = i > 0 ? uint32_t(S.c->large[pslot] >> 32) : 0u;
+  for (uint32_t k = gw * 64 + lane; k < pcount; k += W * 64) Fp[qp[k]] = 0ull;
+  for (uint32_t k = gw * 64 + lane; k < plarge; k += W * 64) Fp[S.ql[pslot][k].x] = 0ull;
+  if (count + n_large == 0) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&S.c->steps, 1u);
+  uint32_t edges = 0;
+  const uint2 *ql = S.ql[slot];
+  for (uint32_t t0 = gw * kMsChunk; t0 < items; t0 += W * kMsChunk) {
+    uint32_t lo = 0, hi = n_large;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ql[mid].y <= t0) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t t = t0 + lane;
+    const bool act = lane < kMsChunk && t < items;
+    uint32_t e = lo, b = 0, u = 0;
+    uint64_t mu = 0;
+    if (act) {
+      while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
+      const uint2 ent = ql[e];
+      u = ent.x;
+      b = G.bucket_off[u] + (t - ent.y);
+      mu = Fc[u];
+    }
+    uint64_t m = ballot(act);
+    while (m) {
+      const uint32_t j = ffs64(m);
+      m &= m - 1;
+      const uint32_t x = G.heads[size_t(read_lane(b, j)) * kSlabWords + lane];
+      mssp_chain<MAP>(G, S, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
+    }
+  }
+  for (uint32_t k0 = gw * 64; k0 < count; k0 += W * 64) {
+    const uint32_t k = k0 + lane;
+    bool act = k < count;
+    uint32_t u = 0, b0 = 0, nb = 0;
+    uint64_t mu = 0;
+    if (act) {
+      u = qc[k];
+      mu = Fc[u];
+      b0 = G.bucket_off[u];
+      nb = G.bucket_off[u + 1] - b0;
+    }
+    for (uint32_t kk = 0; ballot(act && nb > kk); ++kk) {
+      uint64_t m = ballot(act && nb > kk);
+      while (m) {
+        const uint32_t j = ffs64(m);
+        m &= m - 1;
+        const uint32_t x = G.heads[size_t(read_lane(b0, j) + kk) * kSlabWords + lane];
+        mssp_chain<MAP>(G, S, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
+      }
+    }
+  }
+  if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
+}
+
+__global__ void k_mssp_init(MsView S, const uint32_t *sources, uint32_t k) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint32_t cntv = 0;
+  for (uint32_t s = 0; s < k; ++s) {
+    const uint32_t v = sources[s];
+    if (S.F[0][v] == 0) S.logv[cntv++] = v;
+    S.F[0][v] |= 1ull << s;
+    S.dist[size_t(v) * 64 + s] = 0;
+  }
+  S.c->cnt[0] = cntv;
+}
+
+__global__ void k_mssp_read(const uint32_t *dist, uint32_t n, uint32_t s, uint32_t *out) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) out[v] = dist[size_t(v) * 64 + s];
+}
+
+// ------------------------------------------------------------------ host
+struct MultiSource {
+  mk_graph *g = nullptr;
+  bool sssp = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint64_t *seen = nullptr;
+  uint64_t *F[kMsSlots] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *dist = nullptr;
+  uint32_t *logv = nullptr;
+  uint64_t *logm = nullptr;
+  uint32_t *seg = nullptr;
+  uint32_t log_cap = 0;
+  uint2 *ql[kMsSlots] = {nullptr, nullptr, nullptr, nullptr};
+  MsCounts *c = nullptr;
+  uint32_t *src_dev = nullptr;
+  HostBuf host;
+  uint32_t k = 0;
+  uint32_t last_steps = 0;
+  uint64_t last_edges = 0;
+  uint32_t log_total = 0;
+  int max_waves = 0;
+
+  MsView view() const {
+    MsView v;
+    v.seen = seen;
+    for (uint32_t i = 0; i < kMsSlots; ++i) {
+      v.F[i] = F[i];
+      v.ql[i] = ql[i];
+    }
+    v.dist = dist;
+    v.logv = logv;
+    v.logm = logm;
+    v.seg = seg;
+    v.log_cap = log_cap;
+    v.c = c;
+    v.n = g->n;
+    return v;
+  }
+
+  int alloc_log(uint64_t entries) {
+    (void)hipFree(logv);
+    (void)hipFree(logm);
+    logv = nullptr;
+    logm = nullptr;
+    entries = std::min<uint64_t>(entries, 0xFFFFFFF0ull);
+    MK_HIP(hipMalloc(&logv, entries * 4));
+    MK_HIP(hipMalloc(&logm, entries * 8));
+    log_cap = uint32_t(entries);
+    return MK_OK;
+  }
+
+  int init(mk_graph *graph, bool is_sssp) {
+    g = graph;
+    sssp = is_sssp;
+    const size_t n = g->n;
+    MK_HIP(hipSetDevice(g->device));
+    MK_HIP(hipStreamCreate(&stream));
+    MK_HIP(hipEventCreate(&ev0));
+    MK_HIP(hipEventCreate(&ev1));
+    for (uint32_t i = 0; i < kMsSlots; ++i) {
+      MK_HIP(hipMalloc(&F[i], n * 8));
+      MK_HIP(hipMalloc(&ql[i], n * 8));
+      fill_u64(F[i], 0ull, n, stream);
+    }
+    MK_HIP(hipMalloc(&c, sizeof(MsCounts)));
+    MK_HIP(hipMalloc(&src_dev, 64 * 4));
+    if (sssp) {
+      MK_HIP(hipMalloc(&dist, n * 64 * 4));
+      MK_HIP(hipMalloc(&logv, kMsSlots * n * 4));  // one queue per slot
+      log_cap = uint32_t(std::min<uint64_t>(kMsSlots * n, 0xFFFFFFF0ull));
+    } else {
+      MK_HIP(hipMalloc(&seen, n * 8));
+      MK_HIP(hipMalloc(&seg, (n + 2) * 4));
+      if (int r = alloc_log(8 * uint64_t(n) + 64)) return r;
+    }
+    if (int r = host.ensure(sizeof(MsCounts))) return r;
+    MK_HIP(hipStreamSynchronize(stream));
+    max_waves = device_cus(g->device) * 16;
+    return MK_OK;
+  }
+
+  void destroy() {
+    (void)hipSetDevice(g->device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(seen);
+    for (uint32_t i = 0; i < kMsSlots; ++i) {
+      (void)hipFree(F[i]);
+      (void)hipFree(ql[i]);
+    }
+    (void)hipFree(dist);
+    (void)hipFree(logv);
+    (void)hipFree(logm);
+    (void)hipFree(seg);
+    (void)hipFree(c);
+    (void)hipFree(src_dev);
+    host.release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int read_counts(MsCounts &h) {
+    MK_HIP(hipMemcpyAsync(host.ptr, c, sizeof(MsCounts), hipMemcpyDeviceToHost, stream));
+    MK_HIP(hipStreamSynchronize(stream));
+    h = *host.as<MsCounts>();
+    return MK_OK;
+  }
+
+  void reset_state() {
+    const size_t n = g->n;
+    for (uint32_t i = 0; i < kMsSlots; ++i) fill_u64(F[i], 0ull, n, stream);
+    fill_u32(reinterpret_cast<uint32_t *>(c), 0u, sizeof(MsCounts) / 4, stream);
+    if (sssp) fill_u32(dist, kInf, n * 64, stream);
+    else fill_u64(seen, 0ull, n, stream);
+  }
+
+  int run(const uint32_t *sources, uint32_t nsrc, float *ms) {
+    MK_HIP(hipSetDevice(g->device));
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      MK_HIP(hipEventRecord(ev0, stream));
+      reset_state();
+      MK_HIP(hipMemcpyAsync(src_dev, sources, nsrc * 4, hipMemcpyHostToDevice, stream));
+      const GraphView G = g->view();
+      const MsView S = view();
+      if (sssp) k_mssp_init<<<1, 64, 0, stream>>>(S, src_dev, nsrc);
+      else k_msbfs_init<<<1, 64, 0, stream>>>(S, src_dev, nsrc);
+      k_ms_source_hubs<<<1, 64, 0, stream>>>(G, S);
+      MK_HIP(hipGetLastError());
+      const unsigned grid = blocks_for_waves(uint64_t(max_waves));
+      uint32_t step = 0, group = 2;
+      const uint64_t limit = uint64_t(g->n) * (sssp ? 1 : 64) + 8;
+      MsCounts h{};
+      for (;;) {
+        for (uint32_t q = 0; q < group; ++q, ++step) {
+          ProfScope prof(stream, sssp ? "k_mssp_step" : "k_msbfs_level", 0.0);
+          if (sssp) {
+            if (g->weighted) k_mssp_step<true><<<grid, kBlock, 0, stream>>>(G, S, step);
+            else k_mssp_step<false><<<grid, kBlock, 0, stream>>>(G, S, step);
+          } else {
+            if (step + 1 >= g->n + 2) break;  // seg has n + 2 entries
+            if (g->weighted) k_msbfs_level<true><<<grid, kBlock, 0, stream>>>(G, S, step);
+            else k_msbfs_level<false><<<grid, kBlock, 0, stream>>>(G, S, step);
+          }
+        }
+        MK_HIP(hipGetLastError());
+        if (int r = read_counts(h)) return r;
+        const uint32_t slot = step % (sssp ? kMsSlots : 3);
+        if (h.overflow || (h.cnt[slot] == 0 && (h.large[slot] >> 32) == 0)) break;
+        if (step > limit || (!sssp && step + 1 >= g->n + 2))
+          return fail(MK_ERR_STATE, "multi-source traversal did not drain");
+        group = std::min<uint32_t>(group * 2, 8);
+      }
+      if (h.overflow) {
+        // the BFS log ran out of room: enlarge and run again
+        if (int r = alloc_log(uint64_t(log_cap) * 4)) return r;
+        continue;
+      }
+      MK_HIP(hipEventRecord(ev1, stream));
+      MK_HIP(hipEventSynchronize(ev1));
+      float t = 0.f;
+      MK_HIP(hipEventElapsedTime(&t, ev0, ev1));
+      if (ms) *ms = t;
+      k = nsrc;
+      last_steps = h.steps;
+      last_edges = h.edges;
+      if (!sssp) {
+        // levels 0..steps-1 are non-empty; seg[steps] is the log length
+        uint32_t segs[1];
+        MK_HIP(hipMemcpy(segs, seg + h.steps, 4, hipMemcpyDeviceToHost));
+        log_total = segs[0];
+      }
+      return MK_OK;
+    }
+    return fail(MK_ERR_STATE, "multi-source BFS log kept overflowing");
+  }
+
+  int read(uint32_t slot, uint32_t *out) {
+    if (slot >= k) return fail(MK_ERR_RANGE, "mk_msq_read: slot not in the last run");
+    MK_HIP(hipSetDevice(g->device));
+    DevBuf b;
+    if (int r = b.ensure(size_t(g->n) * 4)) return r;
+    if (sssp) {
+      k_mssp_read<<<blocks_for_threads(g->n), kBlock, 0, stream>>>(dist, g->n, slot,
+                                                                   b.as<uint32_t>());
+    } else {
+      fill_u32(b.as<uint32_t>(), kInf, g->n, stream);
+      if (log_total)
+        k_msbfs_read<<<blocks_for_threads(log_total), kBlock, 0, stream>>>(
+            view(), slot, log_total, last_steps + 1, b.as<uint32_t>());
+    }
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(out, b.ptr, size_t(g->n) * 4, hipMemcpyDeviceToHost);
+    b.release();
+    if (e != hipSuccess) return hip_fail(e, "mk_msq_read");
+    return MK_OK;
+  }
+};
+
+}  // namespace mk
+
+struct mk_msq {
+  mk::MultiSource m;
+};
+
+using namespace mk;
+
+extern "C" {
+
+int mk_msq_create(mk_graph *g, int sssp, mk_msq **out) {
+  if (!out) return fail(MK_ERR_BAD_ARG, "mk_msq_create: out is NULL");
+  *out = nullptr;
+  if (!g) return fail(MK_ERR_BAD_ARG, "mk_msq_create: NULL graph");
+  if (sssp && !g->weighted) return fail(MK_ERR_BAD_ARG, "multi-source SSSP needs a weighted graph");
+  mk_msq *h = new mk_msq();
+  int r = h->m.init(g, sssp != 0);
+  if (r) {
+    h->m.destroy();
+    delete h;
+    return r;
+  }
+  g->users++;
+  *out = h;
+  return MK_OK;
+}
+
+int mk_msq_destroy(mk_msq *q) {
+  if (!q) return fail(MK_ERR_BAD_ARG, "mk_msq_destroy: NULL handle");
+  q->m.g->users--;
+  q->m.destroy();
+  delete q;
+  return MK_OK;
+}
+
+int mk_msq_run(mk_msq *q, const uint32_t *sources_host, uint32_t k, float *ms) {
+  if (!q || (k && !sources_host)) return fail(MK_ERR_BAD_ARG, "mk_msq_run: NULL argument");
+  if (k == 0 || k > 64) return fail(MK_ERR_BAD_ARG, "mk_msq_run: 1..64 sources");
+  for (uint32_t i = 0; i < k; ++i)
+    if (sources_host[i] >= q->m.g->n) return fail(MK_ERR_RANGE, "mk_msq_run: source >= n");
+  return q->m.run(sources_host, k, ms);
+}
+
+int mk_msq_read(mk_msq *q, uint32_t slot, uint32_t *dist_host) {
+  if (!q || !dist_host) return fail(MK_ERR_BAD_ARG, "mk_msq_read: NULL argument");
+  return q->m.read(slot, dist_host);
+}
+
+int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges) {
+  if (!q) return fail(MK_ERR_BAD_ARG, "mk_msq_last_stats: NULL handle");
+  if (steps) *steps = q->m.last_steps;
+  if (edges) *edges = q->m.last_edges;
+  return MK_OK;
+}
+
+}  // extern "C"
