@@ -106,8 +106,9 @@ class ServingLoop:
     def _run_queries(self, engine, reqs, now):
         if not reqs:
             return
+        t_start = now()
         for r in reqs:
-            r.start = now()
+            r.start = t_start  # one launch for the whole batch
             r.version = self.version
         engine.run(np.array([r.src for r in reqs], np.uint32))
         t = now()
