@@ -156,6 +156,11 @@ int mk_bfs_incremental(mk_bfs *b, const uint32_t *src_dev, const uint32_t *dst_d
 int mk_bfs_decremental(mk_bfs *b, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
                        float *ms);
 int mk_bfs_read(mk_bfs *b, uint32_t *dist_host, uint32_t *parent_host);
+/* Rebind the source vertex (no reference counterpart: the reference fixes
+ * it at construction).  Distances are those of the old source until the
+ * next mk_bfs_static; lets one handle answer many single-source queries
+ * without reallocating its n-sized buffers. */
+int mk_bfs_set_source(mk_bfs *b, uint32_t source);
 /* Frontier iterations, adjacency slabs and live keys (edges) read by the
  * last compute call; edges/ms gives its traversed-edges rate. */
 int mk_bfs_last_stats(mk_bfs *b, uint32_t *iterations, uint64_t *slabs_read,
@@ -175,6 +180,8 @@ int mk_sssp_incremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst
 int mk_sssp_decremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst_dev,
                         const uint32_t *w_dev, uint32_t n, float *ms);
 int mk_sssp_read(mk_sssp *s, uint32_t *dist_host, uint32_t *parent_host);
+/* As mk_bfs_set_source. */
+int mk_sssp_set_source(mk_sssp *s, uint32_t source);
 int mk_sssp_last_stats(mk_sssp *s, uint32_t *iterations, uint64_t *slabs_read,
                        uint64_t *edges_read);
 

@@ -81,6 +81,7 @@ _PROTOS = {
     "mk_bfs_incremental": [vp, vp, vp, u32, P(f32)],
     "mk_bfs_decremental": [vp, vp, vp, u32, P(f32)],
     "mk_bfs_read": [vp, vp, vp],
+    "mk_bfs_set_source": [vp, u32],
     "mk_bfs_last_stats": [vp, P(u32), P(u64), P(u64)],
     "mk_sssp_create": [vp, u32, P(vp)],
     "mk_sssp_destroy": [vp],
@@ -88,6 +89,7 @@ _PROTOS = {
     "mk_sssp_incremental": [vp, vp, vp, vp, u32, P(f32)],
     "mk_sssp_decremental": [vp, vp, vp, vp, u32, P(f32)],
     "mk_sssp_read": [vp, vp, vp],
+    "mk_sssp_set_source": [vp, u32],
     "mk_sssp_last_stats": [vp, P(u32), P(u64), P(u64)],
     "mk_pagerank_create": [vp, f32, f32, u32, vp, P(vp)],
     "mk_pagerank_destroy": [vp],

@@ -31,6 +31,11 @@ class _Traversal:
         _lib.call(f"{self._prefix}_static", self._h, ctypes.byref(ms))
         return ms.value
 
+    def set_source(self, source):
+        """Rebind the source; call Static() before reading distances."""
+        _lib.call(f"{self._prefix}_set_source", self._h, int(source))
+        self.source = int(source)
+
     def read(self):
         """(distances, parents) as uint32 numpy arrays; INF = 0xFFFFFFFF."""
         d = np.empty(self.G.n, np.uint32)

@@ -64,12 +64,36 @@ MK_DEV uint64_t read_lane64(uint64_t x, uint32_t l) {
   return (uint64_t(read_lane(uint32_t(x >> 32), l)) << 32) | read_lane(uint32_t(x), l);
 }
 
-// Append v (lanes with `push`) to the list `list` at base + atomic count
-// `cnt`, and hub vertices also to the hub list `ql` / `large`.  Entries past
-// `cap` set the overflow flag instead of being written.
-MK_DEV void ms_push(const GraphView &G, const MsView &S, bool push, uint32_t v, uint32_t *list,
-                    uint32_t base, uint32_t *cnt, uint2 *ql, unsigned long long *large,
-                    uint32_t cap, bool log_all) {
+constexpr uint32_t kMsWaveQ = 512;  // LDS staging entries per wave
+
+// Per-wave LDS staging of list appends: one global atomicAdd per flush.
+struct MsQ {
+  uint32_t *buf;
+  uint32_t n;  // wave-uniform
+};
+
+MK_DEV void ms_flush(const MsView &S, MsQ &q, uint32_t *list, uint32_t base, uint32_t *cnt,
+                     uint32_t cap) {
+  if (q.n == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  uint32_t pos = 0;
+  if (lane_id() == 0) pos = atomicAdd(cnt, q.n);
+  pos = read_lane(pos, 0);
+  for (uint32_t i = lane_id(); i < q.n; i += 64) {
+    if (base + pos + i < cap) list[base + pos + i] = q.buf[i];
+    else atomicOr(&S.c->overflow, 1u);
+  }
+  __builtin_amdgcn_wave_barrier();
+  q.n = 0;
+}
+
+// Append v (lanes with `push`) to the list (through the wave's staging) and
+// hub vertices also to the hub list `ql` / `large`; `log_all` keeps hubs in
+// the list too.  Entries past `cap` set the overflow flag instead of being
+// written.  Wave-collective.
+MK_DEV void ms_push(const GraphView &G, const MsView &S, MsQ &q, bool push, uint32_t v,
+                    uint32_t *list, uint32_t base, uint32_t *cnt, uint2 *ql,
+                    unsigned long long *large, uint32_t cap, bool log_all) {
   const uint64_t pm = ballot(push);
   if (pm == 0) return;
   uint32_t nb = 0;
@@ -77,16 +101,10 @@ MK_DEV void ms_push(const GraphView &G, const MsView &S, bool push, uint32_t v, 
   const bool hub = push && nb > kMsSmall;
   const bool to_list = push && (log_all || !hub);
   const uint64_t lm = ballot(to_list);
-  if (lm) {
-    uint32_t pos = 0;
-    const uint32_t leader = ffs64(lm);
-    if (lane_id() == leader) pos = atomicAdd(cnt, uint32_t(__popcll(lm)));
-    pos = read_lane(pos, leader) + lanes_below(lm);
-    if (to_list) {
-      if (base + pos < cap) list[base + pos] = v;
-      else atomicOr(&S.c->overflow, 1u);
-    }
-  }
+  const uint32_t c = uint32_t(__popcll(lm));
+  if (q.n + c > kMsWaveQ) ms_flush(S, q, list, base, cnt, cap);
+  if (to_list) q.buf[q.n + lanes_below(lm)] = v;
+  q.n += c;
   if (hub) {
     const unsigned long long r = atomicAdd(large, (1ull << 32) | nb);
     ql[uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
@@ -96,8 +114,8 @@ MK_DEV void ms_push(const GraphView &G, const MsView &S, bool push, uint32_t v, 
 // ------------------------------------------------------------------ BFS
 // BFS work on one slab of u (mask mu): claim new source bits of each
 // neighbour; a vertex entering level L+1 is pushed once (first bits).
-MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, uint64_t key_mask_all, uint32_t x,
-                       uint64_t mu, uint64_t *Fn, uint32_t nbase, uint32_t nslot,
+MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, MsQ &q, uint64_t key_mask_all,
+                       uint32_t x, uint64_t mu, uint64_t *Fn, uint32_t nbase, uint32_t nslot,
                        uint32_t &edges) {
   const uint32_t lane = lane_id();
   const bool live = ((key_mask_all >> lane) & 1ull) && x < kTomb;
@@ -111,15 +129,15 @@ MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, uint64_t key_mask_al
       if (mine) push = atomicOr((unsigned long long *)(Fn + x), (unsigned long long)mine) == 0;
     }
   }
-  ms_push(G, S, push, x, S.logv, nbase, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
+  ms_push(G, S, q, push, x, S.logv, nbase, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
           S.log_cap, true);
 }
 
 template <bool MAP>
-MK_DEV void msbfs_chain(const GraphView &G, const MsView &S, uint32_t x, uint64_t mu,
+MK_DEV void msbfs_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint64_t mu,
                         uint64_t *Fn, uint32_t nbase, uint32_t nslot, uint32_t &edges) {
   for (;;) {
-    msbfs_slab(G, S, key_mask<MAP>(), x, mu, Fn, nbase, nslot, edges);
+    msbfs_slab(G, S, q, key_mask<MAP>(), x, mu, Fn, nbase, nslot, edges);
     const uint32_t nx = read_lane(x, kNextLane);
     if (nx == kEmpty) break;
     x = slab_ptr(G.pool, nx)[lane_id()];
@@ -156,6 +174,8 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
     for (uint32_t i = ps + gw * 64 + lane; i < min(start, S.log_cap); i += W * 64)
       Fp[S.logv[i]] = 0ull;
   }
+  __shared__ uint32_t lqbuf[kBlock / 64][kMsWaveQ];
+  MsQ q{lqbuf[threadIdx.x >> 6], 0u};
   uint32_t edges = 0;
   // hubs: bucket work items
   const uint2 *ql = S.ql[slot];
@@ -178,10 +198,23 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
     }
     uint64_t m = ballot(act);
     while (m) {
-      const uint32_t j = ffs64(m);
-      m &= m - 1;
-      const uint32_t x = G.heads[size_t(read_lane(b, j)) * kSlabWords + lane];
-      msbfs_chain<MAP>(G, S, x, read_lane64(mu, j), Fn, nbase, nslot, edges);
+      uint32_t js[4];
+      uint32_t c = 0;
+#pragma unroll
+      for (uint32_t qq = 0; qq < 4; ++qq) {
+        js[qq] = m ? ffs64(m) : 0u;
+        if (m) {
+          m &= m - 1;
+          c = qq + 1;
+        }
+      }
+      uint32_t x[4];
+#pragma unroll
+      for (uint32_t qq = 0; qq < 4; ++qq)
+        x[qq] = qq < c ? G.heads[size_t(read_lane(b, js[qq])) * kSlabWords + lane] : kEmpty;
+#pragma unroll
+      for (uint32_t qq = 0; qq < 4; ++qq)
+        if (qq < c) msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges);
     }
   }
   // log entries: record the mask; expand non-hub vertices
@@ -204,23 +237,25 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
         uint32_t js[4];
         uint32_t c = 0;
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-          js[q] = m ? ffs64(m) : 0u;
+        for (uint32_t qq = 0; qq < 4; ++qq) {
+          js[qq] = m ? ffs64(m) : 0u;
           if (m) {
             m &= m - 1;
-            c = q + 1;
+            c = qq + 1;
           }
         }
         uint32_t x[4];
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q)
-          x[q] = q < c ? G.heads[size_t(read_lane(b0, js[q]) + k) * kSlabWords + lane] : kEmpty;
+        for (uint32_t qq = 0; qq < 4; ++qq)
+          x[qq] = qq < c ? G.heads[size_t(read_lane(b0, js[qq]) + k) * kSlabWords + lane] : kEmpty;
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q)
-          if (q < c) msbfs_chain<MAP>(G, S, x[q], read_lane64(mu, js[q]), Fn, nbase, nslot, edges);
+        for (uint32_t qq = 0; qq < 4; ++qq)
+          if (qq < c)
+            msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges);
       }
     }
   }
+  ms_flush(S, q, S.logv, nbase, &S.c->cnt[nslot], S.log_cap);
   if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
 }
 
@@ -271,7 +306,7 @@ __global__ void k_msbfs_read(MsView S, uint32_t s, uint32_t total, uint32_t leve
 // weight w), lanes are sources; four neighbours' distance rows are loaded at
 // a time.
 template <bool MAP>
-MK_DEV void mssp_slab(const GraphView &G, const MsView &S, uint32_t x, uint32_t du_lane,
+MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t du_lane,
                       uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
   const uint32_t lane = lane_id();
   const uint64_t km = key_mask<MAP>();
@@ -282,45 +317,45 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, uint32_t x, uint32_t 
     uint32_t ks[4];
     uint32_t c = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-      ks[q] = live ? ffs64(live) : 0u;
+    for (uint32_t qq = 0; qq < 4; ++qq) {
+      ks[qq] = live ? ffs64(live) : 0u;
       if (live) {
         live &= live - 1;
-        c = q + 1;
+        c = qq + 1;
       }
     }
     uint32_t v[4], w[4], cur[4];
 #pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-      v[q] = read_lane(x, ks[q]);
-      w[q] = MAP ? read_lane(x, ks[q] + 1) : 1u;
-      cur[q] = q < c ? S.dist[size_t(v[q]) * 64 + lane] : 0u;
+    for (uint32_t qq = 0; qq < 4; ++qq) {
+      v[qq] = read_lane(x, ks[qq]);
+      w[qq] = MAP ? read_lane(x, ks[qq] + 1) : 1u;
+      cur[qq] = qq < c ? S.dist[size_t(v[qq]) * 64 + lane] : 0u;
     }
 #pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-      if (q >= c) continue;
-      const uint32_t cand = du_lane + w[q];
-      bool imp = mine && du_lane != kInf && cand >= du_lane && cand < cur[q];
+    for (uint32_t qq = 0; qq < 4; ++qq) {
+      if (qq >= c) continue;
+      const uint32_t cand = du_lane + w[qq];
+      bool imp = mine && du_lane != kInf && cand >= du_lane && cand < cur[qq];
       if (imp) {
-        const uint32_t old = atomicMin(S.dist + size_t(v[q]) * 64 + lane, cand);
+        const uint32_t old = atomicMin(S.dist + size_t(v[qq]) * 64 + lane, cand);
         imp = cand < old;
       }
       const uint64_t im = ballot(imp);
       bool push = false;
       if (im && lane == 0)
-        push = atomicOr((unsigned long long *)(Fn + v[q]), (unsigned long long)im) == 0;
-      ms_push(G, S, push, v[q], qn, 0, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
+        push = atomicOr((unsigned long long *)(Fn + v[qq]), (unsigned long long)im) == 0;
+      ms_push(G, S, q, push, v[qq], qn, 0, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
               S.n, false);
     }
   }
 }
 
 template <bool MAP>
-MK_DEV void mssp_chain(const GraphView &G, const MsView &S, uint32_t x, uint32_t u, uint64_t mu,
-                       uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
+MK_DEV void mssp_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t u,
+                       uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
   const uint32_t du = S.dist[size_t(u) * 64 + lane_id()];
   for (;;) {
-    mssp_slab<MAP>(G, S, x, du, mu, Fn, qn, nslot, edges);
+    mssp_slab<MAP>(G, S, q, x, du, mu, Fn, qn, nslot, edges);
     const uint32_t nx = read_lane(x, kNextLane);
     if (nx == kEmpty) break;
     x = slab_ptr(G.pool, nx)[lane_id()];
@@ -354,6 +389,8 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
   for (uint32_t k = gw * 64 + lane; k < plarge; k += W * 64) Fp[S.ql[pslot][k].x] = 0ull;
   if (count + n_large == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&S.c->steps, 1u);
+  __shared__ uint32_t lqbuf[kBlock / 64][kMsWaveQ];
+  MsQ q{lqbuf[threadIdx.x >> 6], 0u};
   uint32_t edges = 0;
   const uint2 *ql = S.ql[slot];
   for (uint32_t t0 = gw * kMsChunk; t0 < items; t0 += W * kMsChunk) {
@@ -376,10 +413,25 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
     }
     uint64_t m = ballot(act);
     while (m) {
-      const uint32_t j = ffs64(m);
-      m &= m - 1;
-      const uint32_t x = G.heads[size_t(read_lane(b, j)) * kSlabWords + lane];
-      mssp_chain<MAP>(G, S, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
+      uint32_t js[4];
+      uint32_t c = 0;
+#pragma unroll
+      for (uint32_t qq = 0; qq < 4; ++qq) {
+        js[qq] = m ? ffs64(m) : 0u;
+        if (m) {
+          m &= m - 1;
+          c = qq + 1;
+        }
+      }
+      uint32_t x[4];
+#pragma unroll
+      for (uint32_t qq = 0; qq < 4; ++qq)
+        x[qq] = qq < c ? G.heads[size_t(read_lane(b, js[qq])) * kSlabWords + lane] : kEmpty;
+#pragma unroll
+      for (uint32_t qq = 0; qq < 4; ++qq)
+        if (qq < c)
+          mssp_chain<MAP>(G, S, q, x[qq], read_lane(u, js[qq]), read_lane64(mu, js[qq]), Fn, qn,
+                          nslot, edges);
     }
   }
   for (uint32_t k0 = gw * 64; k0 < count; k0 += W * 64) {
@@ -392,6 +444,7 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
       mu = Fc[u];
       b0 = G.bucket_off[u];
       nb = G.bucket_off[u + 1] - b0;
+      if (nb > kMsSmall) act = false;  // hubs are expanded from the hub list
     }
     for (uint32_t kk = 0; ballot(act && nb > kk); ++kk) {
       uint64_t m = ballot(act && nb > kk);
@@ -399,10 +452,11 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
         const uint32_t j = ffs64(m);
         m &= m - 1;
         const uint32_t x = G.heads[size_t(read_lane(b0, j) + kk) * kSlabWords + lane];
-        mssp_chain<MAP>(G, S, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
+        mssp_chain<MAP>(G, S, q, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
       }
     }
   }
+  ms_flush(S, q, qn, 0, &S.c->cnt[nslot], S.n);
   if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
 }
 

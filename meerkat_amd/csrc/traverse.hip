@@ -1016,6 +1016,18 @@ int mk_bfs_decremental(mk_bfs *b, const uint32_t *src, const uint32_t *dst, uint
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_decremental: NULL handle");
   return b->t.run_decremental(src, dst, n, ms);
 }
+int mk_bfs_set_source(mk_bfs *b, uint32_t source) {
+  if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_set_source: NULL handle");
+  if (source >= b->t.g->n) return fail(MK_ERR_RANGE, "mk_bfs_set_source: source >= n");
+  b->t.source = source;
+  return MK_OK;
+}
+int mk_sssp_set_source(mk_sssp *s, uint32_t source) {
+  if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_set_source: NULL handle");
+  if (source >= s->t.g->n) return fail(MK_ERR_RANGE, "mk_sssp_set_source: source >= n");
+  s->t.source = source;
+  return MK_OK;
+}
 int mk_bfs_read(mk_bfs *b, uint32_t *d, uint32_t *p) {
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_read: NULL handle");
   return b->t.read(d, p);

@@ -10,8 +10,8 @@ Requests arrive on a timeline (seconds from the start): update batches
   at the same time (each ``MultiSource.run`` releases the GIL, so both are in
   flight together).  Requests that arrive while a round runs join the next
   round.
-- ``sequential``: one request at a time in arrival order, queries as
-  single-source traversals.
+- ``sequential``: one request at a time in arrival order, queries by the
+  single-source static BFS / SSSP (the one-at-a-time incremental path).
 
 Each query records the graph version (updates applied before it ran) and can
 keep its distances, so tests can check that a query's answer in the mix
@@ -24,7 +24,7 @@ import time
 
 import numpy as np
 
-from .algorithms import MultiSource
+from .algorithms import BFS, SSSP, MultiSource
 from .graph import device_u32
 
 
@@ -75,6 +75,9 @@ class ServingLoop:
         self.slab_margin = slab_margin
         self.bfs = MultiSource(graph, sssp=False)
         self.sssp = MultiSource(graph, sssp=True) if graph.weighted else None
+        # one-at-a-time mode answers with the single-source engines
+        self.bfs1 = BFS(graph, 0)
+        self.sssp1 = SSSP(graph, 0) if graph.weighted else None
         self.version = 0
         self.rejected_rounds = 0
 
@@ -117,6 +120,16 @@ class ServingLoop:
                 r.result = engine.distances(slot)
             r.done = t
 
+    def _run_single(self, r, now):
+        eng = self.bfs1 if r.kind == "bfs" else self.sssp1
+        r.start = now()
+        r.version = self.version
+        eng.set_source(r.src)
+        eng.Static()
+        if self.keep_results:
+            r.result = eng.Distances()
+        r.done = now()
+
     # ------------------------------------------------------------ modes
     def run(self, requests, mode="batched"):
         """Serve `requests` (sorted by arrival) and return them, completed."""
@@ -136,7 +149,7 @@ class ServingLoop:
                     r.admitted = now()
                     self._apply(r, now)
                 else:
-                    self._run_queries(self.bfs if r.kind == "bfs" else self.sssp, [r], now)
+                    self._run_single(r, now)
             return reqs
 
         pending_u = collections.deque()

@@ -94,3 +94,19 @@ def test_update_admission_against_pool(gpu):
     q = serving.Request("bfs", 0.0, src=1)
     out = loop.run([small, q], mode="batched")
     assert out[0].version == 1 and out[1].done is not None
+
+
+def test_sequential_mode_answers_match_oracle(gpu):
+    n, s, d, w, pool, g = build(11, 43)
+    keys = gen.keys_of(s, d)
+    reqs = serving.make_stream(n, keys, 40, rate=1.0, seed=8, median=200, max_size=1000,
+                               all_at_once=True)
+    loop = serving.ServingLoop(g, keep_results=True)
+    done = loop.run(reqs, mode="sequential")
+    versions = replay_versions(n, s, d, w, done)
+    for r in done:
+        if r.kind == "update":
+            continue
+        ref = versions[r.version]
+        exp = oracle.bfs(ref, r.src) if r.kind == "bfs" else oracle.dijkstra(ref, r.src)
+        assert np.array_equal(r.result, exp), (r.kind, r.src, r.version)
