@@ -13,8 +13,10 @@ value = edge updates per second over the timed steps, shortest paths
 maintained (B * K / seconds).  Also reported: update-only throughput,
 incremental/decremental SSSP and incremental BFS traversed-edge rates,
 per-step latency percentiles, the roofline of the dominant kernel (HIP-event
-timing inside the library over the timed region) and a CPU baseline (the
-oracle's static Dijkstra recompute on the same graph, one run, 1 core).
+timing inside the library over the timed region), a CPU baseline (the
+oracle's static Dijkstra recompute on the same graph, one run, 1 core) and a
+serving-loop phase (continuous batching vs one request at a time on the same
+request mix; query latency percentiles under a Poisson load).
 
   python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--scale 24]
                   [--batch 1000000] [--no-cpu-baseline]
@@ -38,6 +40,66 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def run_serving(args, n, g, s, d, del_idx, used, log):
+    """Serving-loop phase on the resident graph: the same request mix served
+    one request at a time (single-source static BFS / SSSP) and with
+    continuous batching (64-source traversals, BFS and SSSP rounds
+    overlapped), all requests arriving at once (saturation throughput); then
+    a Poisson stream at a fixed offered load for latency percentiles."""
+    import time
+
+    import numpy as np
+
+    from meerkat_amd import gen, serving
+
+    # deletions for the serving streams come from initial edges the update
+    # steps did not delete
+    keep = np.ones(len(s), bool)
+    keep[del_idx[:used]] = False
+    avail = np.nonzero(keep)[0]
+    rng = np.random.default_rng(args.seed + 7)
+    rng.shuffle(avail)
+    thirds = np.array_split(avail[: min(len(avail), 30_000_000)], 3)
+    R = args.serve_requests
+
+    def stream(i, rate, all_at_once):
+        keys = gen.keys_of(s[thirds[i]], d[thirds[i]])
+        return serving.make_stream(n, keys, R, rate=rate, seed=args.seed + 100 + i,
+                                   all_at_once=all_at_once)
+
+    loop = serving.ServingLoop(g)
+    res = {}
+    for i, mode in enumerate(("sequential", "batched")):
+        reqs = stream(i, 1.0, True)
+        t0 = time.perf_counter()
+        done = loop.run(reqs, mode=mode)
+        el = time.perf_counter() - t0
+        res[mode] = serving.summarize(done, el)
+        log(f"serving {mode}: {res[mode]}")
+    cap = res["batched"]["requests_per_s"]
+    rate = 0.5 * cap
+    reqs = stream(2, rate, False)
+    t0 = time.perf_counter()
+    done = loop.run(reqs, mode="batched")
+    el = time.perf_counter() - t0
+    lat = serving.summarize(done, el)
+    log(f"serving at {rate:.0f} req/s offered: {lat}")
+    return {
+        "workload": f"RMAT-{args.scale} resident; {R} requests per run: 30% update batches "
+                    "(lognormal sizes, median 10,000, max 1,000,000 edges, 50/50 insert/delete), "
+                    "70% BFS/SSSP queries (50/50) from uniform random sources",
+        "sequential_requests_per_s": round(res["sequential"]["requests_per_s"], 1),
+        "batched_requests_per_s": round(cap, 1),
+        "speedup": round(cap / res["sequential"]["requests_per_s"], 2),
+        "saturation_query_p99_ms": round(res["batched"].get("query_p99_ms", 0.0), 2),
+        "offered_load_requests_per_s": round(rate, 1),
+        "query_p50_ms": round(lat.get("query_p50_ms", 0.0), 2),
+        "query_p99_ms": round(lat.get("query_p99_ms", 0.0), 2),
+        "update_p99_ms": round(lat.get("update_p99_ms", 0.0), 2),
+        "sequential_query_p50_ms": round(res["sequential"].get("query_p50_ms", 0.0), 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +112,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mirror", action="store_true",
                     help="no in-edge mirror: decremental re-seeds by scanning every edge")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop phase")
+    ap.add_argument("--serve-requests", type=int, default=400)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,6 +220,10 @@ def main():
         if ms > 0:
             bfs_rates.append(er / (ms * 1e-3) / 1e9)
 
+    serving_out = None
+    if not args.no_serving:
+        serving_out = run_serving(args, n, g, s, d, del_idx, steps_total * half, log)
+
     K = args.steps
     ms_per_step = elapsed / K * 1e3
     value = args.batch * K / elapsed
@@ -227,6 +295,7 @@ def main():
                         "GBps": round(v["achieved_GBps"], 1)} for k, v in prof.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving_out,
     }
     print(json.dumps(out), flush=True)
 

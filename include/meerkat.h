@@ -262,6 +262,16 @@ int mk_msq_run(mk_msq *q, const uint32_t *sources_host, uint32_t k, float *ms);
 int mk_msq_read(mk_msq *q, uint32_t slot, uint32_t *dist_host);
 int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
 
+/* ------------------------------------------------------ tuning knobs
+ * Launch-shape knobs, read when a handle is created (no reference
+ * counterpart; the reference hard-codes its shapes, e.g. BFSImpl.cuh:
+ * 620-625).  "waves_per_cu" (1..64, default 16): waves the grid-stride
+ * kernels launch per compute unit.  "ms_unroll" (4 or 8, default 8):
+ * neighbour distance rows in flight per wave in multi-source SSSP.  The
+ * Python autotuner (meerkat_amd/autotune.py) sets them from measurements. */
+int mk_tune_set(const char *name, int64_t value);
+int mk_tune_get(const char *name, int64_t *value);
+
 /* ------------------------------------------------------ kernel profiling
  * No reference counterpart (the reference times whole calls with
  * cudaEvent, BFSImpl.cuh:644-661).  When enabled, every instrumented kernel

@@ -8,7 +8,7 @@ fails loudly if it is not built.
 from ._lib import MeerkatError, header_symbols, lib
 from .algorithms import BFS, CC, INF, SSSP, TC, MultiSource, PageRank
 from .graph import DynamicGraph, SlabPool, device_u32
-from . import gen, serving
+from . import autotune, gen, serving
 
 __all__ = [
     "BFS",

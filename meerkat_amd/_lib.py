@@ -115,6 +115,8 @@ _PROTOS = {
     "mk_msq_run": [vp, vp, u32, P(f32)],
     "mk_msq_read": [vp, u32, vp],
     "mk_msq_last_stats": [vp, P(u32), P(u64)],
+    "mk_tune_set": [ctypes.c_char_p, ctypes.c_int64],
+    "mk_tune_get": [ctypes.c_char_p, P(ctypes.c_int64)],
     "mk_profile_enable": [i32],
     "mk_profile_reset": [],
     "mk_profile_count": [P(i32)],

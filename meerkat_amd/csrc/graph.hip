@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "profile.hpp"
@@ -41,6 +43,20 @@ int device_cus(int device) {
     cached[device] = v;
   }
   return cached[device];
+}
+
+// ---------------------------------------------------------------- tuning
+// Launch-shape knobs read when a handle is created (see mk_tune_set).
+static std::mutex g_tune_mu;
+static std::map<std::string, int64_t> g_tune = {
+    {"waves_per_cu", 16},   // resident waves per CU the grid-stride kernels ask for
+    {"ms_unroll", 8},       // neighbour rows in flight per wave in multi-source SSSP
+};
+
+int64_t tune_get(const char *name, int64_t dflt) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(name);
+  return it == g_tune.end() ? dflt : it->second;
 }
 
 int DevBuf::ensure(size_t need) {
@@ -335,6 +351,29 @@ extern "C" {
 
 const char *mk_last_error(void) { return g_err.c_str(); }
 int mk_version(void) { return 100; }
+
+int mk_tune_set(const char *name, int64_t value) {
+  if (!name) return fail(MK_ERR_BAD_ARG, "mk_tune_set: NULL name");
+  const std::string k(name);
+  if (k == "waves_per_cu" && (value < 1 || value > 64))
+    return fail(MK_ERR_BAD_ARG, "waves_per_cu must be in [1, 64]");
+  if (k == "ms_unroll" && value != 4 && value != 8)
+    return fail(MK_ERR_BAD_ARG, "ms_unroll must be 4 or 8");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) return fail(MK_ERR_BAD_ARG, "mk_tune_set: unknown knob " + k);
+  it->second = value;
+  return MK_OK;
+}
+
+int mk_tune_get(const char *name, int64_t *value) {
+  if (!name || !value) return fail(MK_ERR_BAD_ARG, "mk_tune_get: NULL argument");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(name);
+  if (it == g_tune.end()) return fail(MK_ERR_BAD_ARG, std::string("unknown knob ") + name);
+  *value = it->second;
+  return MK_OK;
+}
 
 int mk_device_info(int device, int *compute_units, int *wave_size) {
   int cu = 0, ws = 0;

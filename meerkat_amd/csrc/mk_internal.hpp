@@ -156,6 +156,9 @@ void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 int device_cus(int device);
+int64_t tune_get(const char *name, int64_t dflt);
+// Grid-stride kernels: the number of waves to launch on `device`.
+inline int grid_waves(int device) { return device_cus(device) * int(tune_get("waves_per_cu", 16)); }
 
 #define MK_HIP(call)                                                      \
   do {                                                                    \

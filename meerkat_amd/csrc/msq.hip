@@ -305,7 +305,7 @@ __global__ void k_msbfs_read(MsView S, uint32_t s, uint32_t total, uint32_t leve
 // Relax one slab of u for all sources in mu: per live key (neighbour v,
 // weight w), lanes are sources; four neighbours' distance rows are loaded at
 // a time.
-template <bool MAP>
+template <bool MAP, uint32_t U>
 MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t du_lane,
                       uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
   const uint32_t lane = lane_id();
@@ -314,25 +314,25 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
   edges += uint32_t(__popcll(live));
   const bool mine = (mu >> lane) & 1ull;
   while (live) {
-    uint32_t ks[4];
+    uint32_t ks[U];
     uint32_t c = 0;
 #pragma unroll
-    for (uint32_t qq = 0; qq < 4; ++qq) {
+    for (uint32_t qq = 0; qq < U; ++qq) {
       ks[qq] = live ? ffs64(live) : 0u;
       if (live) {
         live &= live - 1;
         c = qq + 1;
       }
     }
-    uint32_t v[4], w[4], cur[4];
+    uint32_t v[U], w[U], cur[U];
 #pragma unroll
-    for (uint32_t qq = 0; qq < 4; ++qq) {
+    for (uint32_t qq = 0; qq < U; ++qq) {
       v[qq] = read_lane(x, ks[qq]);
       w[qq] = MAP ? read_lane(x, ks[qq] + 1) : 1u;
       cur[qq] = qq < c ? S.dist[size_t(v[qq]) * 64 + lane] : 0u;
     }
 #pragma unroll
-    for (uint32_t qq = 0; qq < 4; ++qq) {
+    for (uint32_t qq = 0; qq < U; ++qq) {
       if (qq >= c) continue;
       const uint32_t cand = du_lane + w[qq];
       bool imp = mine && du_lane != kInf && cand >= du_lane && cand < cur[qq];
@@ -350,12 +350,12 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
   }
 }
 
-template <bool MAP>
+template <bool MAP, uint32_t U>
 MK_DEV void mssp_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t u,
                        uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
   const uint32_t du = S.dist[size_t(u) * 64 + lane_id()];
   for (;;) {
-    mssp_slab<MAP>(G, S, q, x, du, mu, Fn, qn, nslot, edges);
+    mssp_slab<MAP, U>(G, S, q, x, du, mu, Fn, qn, nslot, edges);
     const uint32_t nx = read_lane(x, kNextLane);
     if (nx == kEmpty) break;
     x = slab_ptr(G.pool, nx)[lane_id()];
@@ -365,7 +365,7 @@ MK_DEV void mssp_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, 
 // SSSP step i: queue slot i%4 (vertices, masks F[i%4]); appends to slot
 // (i+1)%4; clears the masks of queue i-1 (slot (i+3)%4) and the counters of
 // slot (i+2)%4 (queue i-2, whose masks step i-1 cleared).
-template <bool MAP>
+template <bool MAP, uint32_t U>
 __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uint32_t i) {
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -430,8 +430,8 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
 #pragma unroll
       for (uint32_t qq = 0; qq < 4; ++qq)
         if (qq < c)
-          mssp_chain<MAP>(G, S, q, x[qq], read_lane(u, js[qq]), read_lane64(mu, js[qq]), Fn, qn,
-                          nslot, edges);
+          mssp_chain<MAP, U>(G, S, q, x[qq], read_lane(u, js[qq]), read_lane64(mu, js[qq]), Fn,
+                             qn, nslot, edges);
     }
   }
   for (uint32_t k0 = gw * 64; k0 < count; k0 += W * 64) {
@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
         const uint32_t j = ffs64(m);
         m &= m - 1;
         const uint32_t x = G.heads[size_t(read_lane(b0, j) + kk) * kSlabWords + lane];
-        mssp_chain<MAP>(G, S, q, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
+        mssp_chain<MAP, U>(G, S, q, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
       }
     }
   }
@@ -499,6 +499,7 @@ struct MultiSource {
   uint64_t last_edges = 0;
   uint32_t log_total = 0;
   int max_waves = 0;
+  int unroll = 4;
 
   MsView view() const {
     MsView v;
@@ -555,7 +556,8 @@ struct MultiSource {
     }
     if (int r = host.ensure(sizeof(MsCounts))) return r;
     MK_HIP(hipStreamSynchronize(stream));
-    max_waves = device_cus(g->device) * 16;
+    max_waves = grid_waves(g->device);
+    unroll = int(tune_get("ms_unroll", 8));
     return MK_OK;
   }
 
@@ -614,8 +616,13 @@ struct MultiSource {
         for (uint32_t q = 0; q < group; ++q, ++step) {
           ProfScope prof(stream, sssp ? "k_mssp_step" : "k_msbfs_level", 0.0);
           if (sssp) {
-            if (g->weighted) k_mssp_step<true><<<grid, kBlock, 0, stream>>>(G, S, step);
-            else k_mssp_step<false><<<grid, kBlock, 0, stream>>>(G, S, step);
+            if (unroll == 8) {
+              if (g->weighted) k_mssp_step<true, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
+              else k_mssp_step<false, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
+            } else {
+              if (g->weighted) k_mssp_step<true, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
+              else k_mssp_step<false, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
+            }
           } else {
             if (step + 1 >= g->n + 2) break;  // seg has n + 2 entries
             if (g->weighted) k_msbfs_level<true><<<grid, kBlock, 0, stream>>>(G, S, step);

@@ -386,7 +386,7 @@ struct PageRank {
       if (int r = recount()) return r;
     }
     MK_HIP(hipStreamSynchronize(stream));
-    max_waves = device_cus(g->device) * 16;
+    max_waves = grid_waves(g->device);
     cur = 0;
     return MK_OK;
   }

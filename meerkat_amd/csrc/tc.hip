@@ -402,7 +402,7 @@ struct TC {
     MK_HIP(hipMalloc(&slots, (kTcSlots + 1) * 8));
     MK_HIP(hipMalloc(&n_owned, 4));
     if (int r = host.ensure((kTcSlots + 2) * 8)) return r;
-    max_waves = device_cus(g->device) * 16;
+    max_waves = grid_waves(g->device);
     return MK_OK;
   }
 

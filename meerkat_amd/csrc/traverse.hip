@@ -678,7 +678,7 @@ struct Traversal {
     clear_counts();
     k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
     MK_HIP(hipStreamSynchronize(stream));
-    max_waves = device_cus(g->device) * 16;
+    max_waves = grid_waves(g->device);
     return MK_OK;
   }
 

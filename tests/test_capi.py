@@ -89,3 +89,22 @@ def test_update_batches():
     assert len(np.unique(dk)) == 1500 and np.isin(dk, keys).all()
     with pytest.raises(mk.MeerkatError):
         gen.delete_batch(keys, len(keys) + 1, seed=1)
+
+
+def test_tune_knobs_validate_without_gpu():
+    from meerkat_amd import autotune
+
+    before = autotune.knobs()
+    assert before == {"waves_per_cu": 16, "ms_unroll": 8}
+    try:
+        autotune.set_knob("waves_per_cu", 32)
+        assert autotune.get_knob("waves_per_cu") == 32
+        for name, bad in [("waves_per_cu", 0), ("waves_per_cu", 65), ("ms_unroll", 6),
+                          ("no_such_knob", 1)]:
+            with pytest.raises(_lib.MeerkatError) as e:
+                autotune.set_knob(name, bad)
+            assert e.value.code == _lib.MK_ERR_BAD_ARG
+        assert autotune.knobs()["waves_per_cu"] == 32
+    finally:
+        for k, v in before.items():
+            autotune.set_knob(k, v)

@@ -29,8 +29,19 @@ def test_msbfs_equals_single_source(gpu, k):
         assert np.array_equal(q.distances(slot), oracle.bfs(ref, int(v))), slot
 
 
+@pytest.fixture
+def unroll(request):
+    from meerkat_amd import autotune
+
+    before = autotune.get_knob("ms_unroll")
+    autotune.set_knob("ms_unroll", request.param)
+    yield request.param
+    autotune.set_knob("ms_unroll", before)
+
+
+@pytest.mark.parametrize("unroll", [4, 8], indirect=True)
 @pytest.mark.parametrize("k", [1, 64])
-def test_mssp_equals_dijkstra(gpu, k):
+def test_mssp_equals_dijkstra(gpu, k, unroll):
     n, s, d, w = gen.rmat(13, 16, seed=32, weighted=True)
     pool = mk.SlabPool(1 << 27)
     g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
@@ -64,3 +75,32 @@ def test_msbfs_reuse_across_runs_and_updates(gpu):
     assert np.array_equal(q.distances(0), oracle.bfs(ref, 0))
     with pytest.raises(mk.MeerkatError):
         q.distances(1)  # slot not in the last run
+
+
+def test_autotune_picks_a_setting_and_keeps_parity(gpu):
+    from meerkat_amd import autotune
+
+    before = autotune.knobs()
+    n, s, d, w = gen.rmat(12, 16, seed=33, weighted=True)
+    pool = mk.SlabPool(1 << 26)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
+    g.InsertEdges(s, d, w)
+    try:
+        r = autotune.tune(g, waves=(8, 16), unrolls=(4, 8), repeats=1)
+        assert r["waves_per_cu"] in (8, 16) and r["ms_unroll"] in (4, 8)
+        assert set(r["timings"]["waves_per_cu"]) == {8, 16}
+        assert autotune.knobs() == {"waves_per_cu": r["waves_per_cu"], "ms_unroll": r["ms_unroll"]}
+        srcs = np.array([0, 5, 77], np.uint32)
+        q = mk.MultiSource(g, sssp=True)
+        q.run(srcs)
+        ref = oracle.csr(n, s, d, w)
+        for slot, v in enumerate(srcs):
+            assert np.array_equal(q.distances(slot), oracle.dijkstra(ref, int(v))), slot
+        q.close()
+        # apply=False restores what was set before the call
+        autotune.set_knob("waves_per_cu", 12)
+        autotune.tune(g, waves=(8,), unrolls=(4,), repeats=1, apply=False)
+        assert autotune.get_knob("waves_per_cu") == 12
+    finally:
+        for k_, v in before.items():
+            autotune.set_knob(k_, v)
