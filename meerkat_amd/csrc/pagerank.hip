@@ -97,17 +97,25 @@ MK_DEV float chain_sum(const GraphView &R, const float *contrib, uint32_t x, uin
   return wave_sum_f32(acc);
 }
 
+// Clears every reduction slot and the iteration counters of a compute call
+// (one block of kSlots threads, launched before k_pr_prepare adds into
+// dang[0]).
+__global__ void k_pr_reset(PrState *st) {
+  const uint32_t t = threadIdx.x;
+  for (uint32_t k = 0; k < 3; ++k) {
+    st->delta[k][t] = 0.0;
+    st->dang[k][t] = 0.0;
+  }
+  if (t == 0) {
+    st->iters = 0;
+    st->stopped = 0;
+  }
+}
+
 // Initial contributions and dangling mass of the current ranks.
 __global__ void __launch_bounds__(kBlock) k_pr_prepare(PrView P, uint32_t cur) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   double dang = 0.0;
-  if (blockIdx.x == 0 && threadIdx.x < kSlots) {
-    P.st->delta[0][threadIdx.x] = 0.0;
-    P.st->delta[1][threadIdx.x] = 0.0;
-    P.st->delta[2][threadIdx.x] = 0.0;
-    P.st->dang[1][threadIdx.x] = 0.0;
-    P.st->dang[2][threadIdx.x] = 0.0;
-  }
   if (i < P.n) {
     const float r = P.pr[cur][i];
     const uint32_t od = P.outdeg[i];
@@ -424,7 +432,7 @@ struct PageRank {
   int compute(float *ms) {
     MK_HIP(hipSetDevice(g->device));
     MK_HIP(hipEventRecord(ev0, stream));
-    fill_u32(reinterpret_cast<uint32_t *>(&st->iters), 0u, 2, stream);
+    k_pr_reset<<<1, kSlots, 0, stream>>>(st);
     const PrView P = view();
     const GraphView R = in->view();
     k_pr_prepare<<<blocks_for_threads(g->n), kBlock, 0, stream>>>(P, cur);

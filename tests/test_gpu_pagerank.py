@@ -68,10 +68,34 @@ def test_reverse_graph_mode_recount_path(gpu):
     pr = mk.PageRank(rev, D, MARGIN, out_degree=gen.degree_hints(n, s))
     pr.Static()
     bs, bd, _ = gen.insert_batch(gen.keys_of(s, d), n, 5000, seed=8)
-    rev.InsertEdges(bd, bs)
+    # keep the insert's device arrays alive so the copies get other addresses
+    held = rev.InsertEdges(bd, bs)
     pr.Incremental(bs.copy(), bd.copy())
+    del held
     s, d = np.r_[s, bs], np.r_[d, bd]
     assert l1(pr.Values(), fp64_reference(n, s, d)) <= TOL
+
+
+def test_reduction_slots_cleared_over_dirty_device_memory(gpu):
+    # device memory released with NaN bit patterns in it, then a new handle:
+    # every reduction slot a compute call adds into must be cleared first
+    import torch
+
+    junk = torch.full((1 << 24,), float("nan"), device=gpu)
+    torch.cuda.synchronize()
+    del junk
+    torch.cuda.empty_cache()
+    n, s, d, _ = gen.erdos_renyi(5000, 30000, seed=11)
+    pool = mk.SlabPool(1 << 24)
+    rev = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, d))
+    rev.InsertEdges(d, s)
+    for _ in range(2):
+        pr = mk.PageRank(rev, D, MARGIN, out_degree=gen.degree_hints(n, s))
+        pr.Static()
+        v = pr.Values()
+        assert np.isfinite(v).all()
+        assert l1(v, fp64_reference(n, s, d)) <= TOL
+        pr.close()
 
 
 @pytest.mark.parametrize("weighted", [False, True])
