@@ -5,7 +5,7 @@
  * The reference (Meerkat) is a header-only CUDA C++ template library; its
  * "boundary" is the set of class methods its drivers and tests call.  Each
  * entry point below replaces one of those methods and cites it (paths are
- * relative to the reference tree).  The C++ classes in include/meerkat/ (one .hpp per class)
+ * relative to the reference tree).  The C++ classes in include/meerkat/ (SlabGraph.hpp, Algorithms.hpp)
  * restore the reference's class and method names on top of this ABI.
  *
  * Conventions

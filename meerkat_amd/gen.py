@@ -81,3 +81,23 @@ def delete_batch(sorted_keys, count, seed):
 def degree_hints(n, src):
     """Out-degree of every vertex in an edge list (the DynamicGraph hints)."""
     return np.bincount(src, minlength=n).astype(np.uint32)
+
+
+def write_edge_file(path, n, src, dst):
+    """Write a base graph in the text format the drivers read: "V E" then
+    one "u v" line per edge (the reference's src/BFS.cu:27-36)."""
+    src = np.asarray(src, np.uint32)
+    dst = np.asarray(dst, np.uint32)
+    with open(path, "w") as f:
+        f.write(f"{int(n)} {len(src)}\n")
+        np.savetxt(f, np.stack([src, dst], 1), fmt="%d")
+
+
+def write_batch_file(prefix, i, src, dst):
+    """Write update batch `i` as "<prefix>.batch.<i>": "n" then one "u v"
+    line per edge (the reference's src/BFS.cu:76-91)."""
+    src = np.asarray(src, np.uint32)
+    dst = np.asarray(dst, np.uint32)
+    with open(f"{prefix}.batch.{i}", "w") as f:
+        f.write(f"{len(src)}\n")
+        np.savetxt(f, np.stack([src, dst], 1), fmt="%d")
