@@ -40,6 +40,42 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def calibration_copy(dev, nbytes=1 << 30):
+    """A plain copy kernel of known size (reads and writes `nbytes`) in its
+    own profiler region: the scale for rocprofv3's FETCH_SIZE / WRITE_SIZE
+    counters when they are collected over this program."""
+    import torch
+
+    from meerkat_amd import profiling
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    with profiling.region():
+        torch.add(src, 0.0, out=dst)
+        torch.cuda.synchronize()
+    del src, dst
+
+
+def measured_traffic(kernel, scale):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 FETCH_SIZE +
+    WRITE_SIZE passes over this bench (scaled by the calibration copy), as
+    recorded in profiles/hbm_traffic_rmat<scale>.txt ("<kernel> <bytes>"
+    lines); None when not measured."""
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"hbm_traffic_rmat{scale}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 2 and parts[0] == kernel:
+                return int(parts[1])
+    return None
+
+
 def run_serving(args, n, g, s, d, del_idx, used, log):
     """Serving-loop phase on the resident graph: the same request mix served
     one request at a time (single-source static BFS / SSSP) and with
@@ -198,13 +234,16 @@ def main():
     profiling.reset()
     profiling.enable(True)
     torch.cuda.synchronize()
-    ts = time.perf_counter()
-    for i in range(args.warmup, steps_total):
-        step(i, rec)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - ts
+    # the timed region is also the region rocprofv3 --selected-regions collects
+    with profiling.region():
+        ts = time.perf_counter()
+        for i in range(args.warmup, steps_total):
+            step(i, rec)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - ts
     profiling.enable(False)
     prof = profiling.totals()
+    calibration_copy(dev)
 
     # incremental BFS rate on the updated graph (not part of the step)
     bfs = mk.BFS(g, 0)
@@ -243,7 +282,8 @@ def main():
         achieved = t["achieved_GBps"]
         roofline = {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": measured_traffic(k, args.scale),
                     "avg_launch_ms": round(t["avg_ms"], 4),
                     "algo_bytes_per_launch": round(per_launch_bytes)}
 

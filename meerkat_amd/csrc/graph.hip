@@ -134,14 +134,17 @@ __global__ void k_validate(const uint32_t *src, const uint32_t *dst, uint32_t n,
   if (m && lane_id() == ffs64(m)) atomicAdd(counters, uint32_t(__popcll(m)));
 }
 
-// Each wave takes `epw` (<= 64) consecutive edges, lane i holding edge i, and
-// applies them one after another with the whole wave (the reference's
-// per-warp work queue, include/Graph/warp/InsertEdge.cuh:66-103).
+// Each wave takes `epw` (<= 64) consecutive edges, lane i holding edge i.
+// The four 16-lane groups of the wave apply their own lanes' edges one after
+// another, each group walking and claiming in its own slab chain (the
+// reference runs one edge at a time per warp, include/Graph/warp/
+// InsertEdge.cuh:66-103).
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_insert(GraphView G, PoolView P, const uint32_t *src, const uint32_t *dst, const uint32_t *wt,
              uint32_t n, uint32_t epw, uint8_t *status, uint32_t *counters) {
   const uint32_t lane = lane_id();
+  const uint32_t gl = group_lane();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
@@ -156,25 +159,30 @@ __global__ void __launch_bounds__(kBlock)
     act = s != d;
     if (!act) st = MK_EDGE_SELF_LOOP;
   }
-  uint64_t pending = ballot(act);
+  uint32_t pending = group_bits(ballot(act));
   uint32_t applied = 0, pool_fail = 0;
   while (pending) {
-    const uint32_t j = ffs64(pending);
+    const uint32_t j = ffs32(pending);
     pending &= pending - 1;
-    const uint32_t sj = read_lane(s, j), dj = read_lane(d, j), wj = read_lane(w, j);
-    const int r = slab_insert<MAP>(G, P, sj, dj, wj, wave * 64u + j);
-    if (lane == j) {
+    const uint32_t sj = group_read(s, j), dj = group_read(d, j), wj = group_read(w, j);
+    const int r = slab_insert_group<MAP>(G, P, sj, dj, wj, wave * 64u + group_first_lane() + j);
+    if (gl == j) {
       if (r == 1) st = MK_EDGE_APPLIED;
       else if (r < 0) st = MK_EDGE_POOL_FULL;  // rolled back by the host
     }
     if (r == 1) {
       ++applied;
-      if (lane == 0) atomicAdd(G.vdeg + sj, 1u);
+      if (gl == 0) atomicAdd(G.vdeg + sj, 1u);
     } else if (r < 0) {
       ++pool_fail;
     }
   }
   if (lane < cnt) status[base + lane] = st;
+  // group totals (held by every lane of a group) summed over the 4 groups
+  applied = gl == 0 ? applied : 0u;
+  pool_fail = gl == 0 ? pool_fail : 0u;
+  applied = wave_sum_u32(applied);
+  pool_fail = wave_sum_u32(pool_fail);
   if (lane == 0) {
     if (applied) atomicAdd(counters + 2, applied);
     if (pool_fail) atomicAdd(counters + 1, pool_fail);
@@ -182,12 +190,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Delete the batch edges whose status equals `only_status` (all edges when
-// only_status == 0xFF); records MK_EDGE_APPLIED / MK_EDGE_UNCHANGED.
+// only_status == 0xFF); records MK_EDGE_APPLIED / MK_EDGE_UNCHANGED.  Edges
+// are applied per 16-lane group as in k_insert.
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_delete(GraphView G, const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t epw,
              uint8_t *status, uint32_t only_status, uint32_t *counters) {
   const uint32_t lane = lane_id();
+  const uint32_t gl = group_lane();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
@@ -205,17 +215,17 @@ __global__ void __launch_bounds__(kBlock)
       if (!act) st = MK_EDGE_SELF_LOOP;
     }
   }
-  uint64_t pending = ballot(act);
+  uint32_t pending = group_bits(ballot(act));
   uint32_t applied = 0;
   while (pending) {
-    const uint32_t j = ffs64(pending);
+    const uint32_t j = ffs32(pending);
     pending &= pending - 1;
-    const uint32_t sj = read_lane(s, j), dj = read_lane(d, j);
-    const int r = slab_delete<MAP>(G, sj, dj);
-    if (lane == j && r == 1) st = MK_EDGE_APPLIED;
+    const uint32_t sj = group_read(s, j), dj = group_read(d, j);
+    const int r = slab_delete_group<MAP>(G, sj, dj);
+    if (gl == j && r == 1) st = MK_EDGE_APPLIED;
     if (r == 1) {
       ++applied;
-      if (lane == 0) atomicSub(G.vdeg + sj, 1u);
+      if (gl == 0) atomicSub(G.vdeg + sj, 1u);
     }
   }
   if (only_status != 0xFFu) {
@@ -223,6 +233,7 @@ __global__ void __launch_bounds__(kBlock)
   } else if (lane < cnt) {
     status[base + lane] = st;
   }
+  applied = wave_sum_u32(gl == 0 ? applied : 0u);
   if (lane == 0 && applied) atomicAdd(counters + 2, applied);
 }
 
@@ -244,13 +255,14 @@ __global__ void __launch_bounds__(kBlock)
   }
   uint8_t f = 0;
   uint32_t wv = 0;
-  uint64_t pending = ballot(act);
+  const uint32_t gl = group_lane();
+  uint32_t pending = group_bits(ballot(act));
   while (pending) {
-    const uint32_t j = ffs64(pending);
+    const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     uint32_t wj = 0;
-    const int r = slab_search<MAP>(G, read_lane(s, j), read_lane(d, j), &wj);
-    if (lane == j) {
+    const int r = slab_search_group<MAP>(G, group_read(s, j), group_read(d, j), &wj);
+    if (gl == j) {
       f = uint8_t(r);
       wv = wj;
     }
@@ -336,8 +348,9 @@ __global__ void __launch_bounds__(kBlock) k_release(GraphView G, PoolView P) {
 }
 
 static uint32_t edges_per_wave(uint32_t n, int device) {
-  // enough waves to fill every CU ~16 deep, at most 64 edges (one per lane)
-  const uint64_t target_waves = uint64_t(device_cus(device)) * 16;
+  // enough waves to fill every CU ~32 deep (4 slab operations in flight per
+  // wave), at most 64 edges (one per lane)
+  const uint64_t target_waves = uint64_t(device_cus(device)) * 32;
   uint64_t epw = (n + target_waves - 1) / target_waves;
   return uint32_t(std::min<uint64_t>(std::max<uint64_t>(epw, 1), 64));
 }
