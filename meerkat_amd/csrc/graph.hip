@@ -142,13 +142,17 @@ __global__ void k_validate(const uint32_t *src, const uint32_t *dst, uint32_t n,
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_insert(GraphView G, PoolView P, const uint32_t *src, const uint32_t *dst, const uint32_t *wt,
-             uint32_t n, uint32_t epw, uint8_t *status, uint32_t *counters) {
+             uint32_t n, uint32_t epw, uint8_t *status, uint32_t *counters, const uint32_t *bad) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  if (*bad) {  // the batch failed the range check: leave the graph as it is
+    if (lane < cnt) status[base + lane] = MK_EDGE_UNCHANGED;
+    return;
+  }
   uint32_t s = 0, d = 0, w = 0;
   bool act = false;
   uint8_t st = MK_EDGE_UNCHANGED;
@@ -195,13 +199,17 @@ __global__ void __launch_bounds__(kBlock)
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_delete(GraphView G, const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t epw,
-             uint8_t *status, uint32_t only_status, uint32_t *counters) {
+             uint8_t *status, uint32_t only_status, uint32_t *counters, const uint32_t *bad) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  if (*bad) {  // the batch failed the range check: leave the graph as it is
+    if (lane < cnt && only_status == 0xFFu) status[base + lane] = MK_EDGE_UNCHANGED;
+    return;
+  }
   uint32_t s = 0, d = 0;
   bool act = false;
   uint8_t st = MK_EDGE_UNCHANGED;
@@ -579,23 +587,27 @@ static int batch_prologue(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   g->last_src = g->last_dst = g->last_w = nullptr;
   fill_u32(g->counters, 0u, 4, s);
   if (n == 0) return MK_OK;
+  // the range check's count (counters[0]) is read by the mutation kernels,
+  // which leave the graph unchanged when it is non-zero; the host reads it
+  // with the mutation's results (range_error)
   k_validate<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters);
-  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
-  MK_HIP(hipStreamSynchronize(s));
-  const uint32_t bad = g->host_scratch.as<uint32_t>()[0];
-  if (bad) {
-    fill_u8(g->status.as<uint8_t>(), MK_EDGE_UNCHANGED, n, s);
-    MK_HIP(hipStreamSynchronize(s));
-    return fail(MK_ERR_RANGE, std::string(who) + ": " + std::to_string(bad) +
-                                  " edges have a vertex id >= n; graph unchanged");
-  }
   return MK_OK;
+}
+
+// After a mutation's counters were copied to host_scratch and the stream
+// synchronised: MK_ERR_RANGE when the batch failed the range check.
+static int range_error(mk_graph *g, const char *who) {
+  const uint32_t bad = g->host_scratch.as<uint32_t>()[0];
+  if (!bad) return MK_OK;
+  return fail(MK_ERR_RANGE, std::string(who) + ": " + std::to_string(bad) +
+                                " edges have a vertex id >= n; graph unchanged");
 }
 
 // Launch the insert kernel on graph g (forward or mirror) with its status
 // buffer and counters.
 static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
-                          const uint32_t *w, uint32_t n, hipStream_t s, const char *name) {
+                          const uint32_t *w, uint32_t n, hipStream_t s, const char *name,
+                          const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
   uint8_t *st = g->status.as<uint8_t>();
@@ -604,17 +616,18 @@ static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   ProfScope prof(s, name, double(n) * (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1));
   if (g->weighted)
     k_insert<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src, dst,
-                                                              w, n, epw, st, g->counters);
+                                                              w, n, epw, st, g->counters, bad);
   else
     k_insert<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
                                                                dst, nullptr, n, epw, st,
-                                                               g->counters);
+                                                               g->counters, bad);
 }
 
 // Launch the delete kernel on graph g; only_status != 0xFF restricts it to
 // edges whose status equals it (roll-back of an insert).
 static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
-                          hipStream_t s, uint32_t only_status, const char *name) {
+                          hipStream_t s, uint32_t only_status, const char *name,
+                          const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
   uint8_t *st = g->status.as<uint8_t>();
@@ -623,10 +636,10 @@ static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   ProfScope prof(s, name, double(n) * (8 + 256 + 4 + 8 + 1));
   if (g->weighted)
     k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                              only_status, g->counters);
+                                                              only_status, g->counters, bad);
   else
     k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                               only_status, g->counters);
+                                                               only_status, g->counters, bad);
 }
 
 // Status buffer and counters of the in-edge mirror for a batch of n edges.
@@ -646,20 +659,23 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n, s)) return r;
-  launch_insert(g, src, dst, w, n, s, "k_insert");
-  if (g->rev) launch_insert(g->rev, dst, src, w, n, s, "k_insert_mirror");
+  launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
+  if (g->rev) launch_insert(g->rev, dst, src, w, n, s, "k_insert_mirror", g->counters);
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
   if (g->rev)
     MK_HIP(hipMemcpyAsync(g->rev->host_scratch.ptr, g->rev->counters, 16, hipMemcpyDeviceToHost,
                           s));
   MK_HIP(hipStreamSynchronize(s));
+  if (int r = range_error(g, "mk_graph_insert")) return r;
   uint32_t pool_fail = g->host_scratch.as<uint32_t>()[1];
   if (g->rev) pool_fail += g->rev->host_scratch.as<uint32_t>()[1];
   if (pool_fail) {
     // roll back: delete what this batch inserted, restoring the edge set
-    launch_delete(g, src, dst, n, s, MK_EDGE_APPLIED, "k_insert_rollback");
-    if (g->rev) launch_delete(g->rev, dst, src, n, s, MK_EDGE_APPLIED, "k_insert_rollback");
+    // (counters[0] is zero here, so the kernels run)
+    launch_delete(g, src, dst, n, s, MK_EDGE_APPLIED, "k_insert_rollback", g->counters);
+    if (g->rev)
+      launch_delete(g->rev, dst, src, n, s, MK_EDGE_APPLIED, "k_insert_rollback", g->counters);
     MK_HIP(hipStreamSynchronize(s));
     return fail(MK_ERR_POOL_FULL, "mk_graph_insert: slab pool exhausted (" +
                                       std::to_string(pool_fail) +
@@ -679,10 +695,12 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n, s)) return r;
-  launch_delete(g, src, dst, n, s, 0xFFu, "k_delete");
-  if (g->rev) launch_delete(g->rev, dst, src, n, s, 0xFFu, "k_delete_mirror");
+  launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);
+  if (g->rev) launch_delete(g->rev, dst, src, n, s, 0xFFu, "k_delete_mirror", g->counters);
   MK_HIP(hipGetLastError());
+  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
   MK_HIP(hipStreamSynchronize(s));
+  if (int r = range_error(g, "mk_graph_delete")) return r;
   g->last_op = 2;
   g->last_src = src;
   g->last_dst = dst;

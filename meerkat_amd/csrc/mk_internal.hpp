@@ -105,6 +105,14 @@ MK_DEV uint32_t lanes_below(uint64_t mask) {
 
 MK_DEV uint32_t ffs64(uint64_t m) { return uint32_t(__ffsll((unsigned long long)m) - 1); }
 
+// Items each of W waves takes per step of a grid-stride loop over `count`
+// items: spread small counts over every wave (short dependent-load chains
+// per wave) and give a full 64 (one per lane) once the grid is covered.
+MK_DEV uint32_t per_wave(uint32_t count, uint32_t W) {
+  const uint32_t p = (count + W - 1) / W;
+  return p < 1u ? 1u : (p > 64u ? 64u : p);
+}
+
 // Loads of words that other waves may be writing in the same kernel: agent-
 // scope relaxed atomics, so a retry never re-reads a stale non-coherent copy.
 MK_DEV uint32_t coherent_load(const uint32_t *p) {

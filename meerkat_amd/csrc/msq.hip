@@ -218,9 +218,10 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
     }
   }
   // log entries: record the mask; expand non-hub vertices
-  for (uint32_t i0 = start + gw * 64; i0 < end; i0 += W * 64) {
+  const uint32_t per = per_wave(end - start, W);
+  for (uint32_t i0 = start + gw * per; i0 < end; i0 += W * per) {
     const uint32_t i = i0 + lane;
-    bool act = i < end;
+    bool act = lane < per && i < end;
     uint32_t u = 0, b0 = 0, nb = 0;
     uint64_t mu = 0;
     if (act) {
@@ -434,9 +435,10 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
                              qn, nslot, edges);
     }
   }
-  for (uint32_t k0 = gw * 64; k0 < count; k0 += W * 64) {
+  const uint32_t per = per_wave(count, W);
+  for (uint32_t k0 = gw * per; k0 < count; k0 += W * per) {
     const uint32_t k = k0 + lane;
-    bool act = k < count;
+    bool act = lane < per && k < count;
     uint32_t u = 0, b0 = 0, nb = 0;
     uint64_t mu = 0;
     if (act) {

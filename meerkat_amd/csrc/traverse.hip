@@ -295,11 +295,12 @@ __global__ void __launch_bounds__(kBlock)
     batched_buckets<MAP, WEIGHTS, MODE>(G, S, q, nside, nslot, epoch, ballot(act), b, u, du,
                                         slabs, edges);
   }
-  // small vertices: 64 per wave step
+  // small vertices: up to 64 per wave step, spread over all waves when few
   const uint32_t *qs = S.q_small[side];
-  for (uint32_t i0 = gw * 64; i0 < n_small; i0 += W * 64) {
+  const uint32_t per = per_wave(n_small, W);
+  for (uint32_t i0 = gw * per; i0 < n_small; i0 += W * per) {
     const uint32_t i = i0 + lane;
-    bool act = i < n_small;
+    bool act = lane < per && i < n_small;
     uint32_t u = 0, b0 = 0, nb = 0, du = 0;
     if (act) {
       u = qs[i];
