@@ -762,10 +762,15 @@ struct Traversal {
     uint32_t group = 2;
     uint64_t launched = 0;
     const uint64_t limit = uint64_t(g->n) + 8;
+    // the first group is launched without reading the counters (steps on an
+    // empty frontier return at once); later groups only while the frontier
+    // the last group left is non-empty
     Totals prev;
-    if (int r = read_counts(prev)) return r;
-    h = prev;
-    while (h.small[slot] != 0 || (h.large[slot] >> 32) != 0) {
+    if (profiling_on()) {
+      if (int r = read_counts(prev)) return r;
+    }
+    bool more = true;
+    while (more) {
       if (launched > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
       for (uint32_t k = 0; k < group; ++k) {
         launch_expand<MODE>(side, slot, MODE == 0 ? next_epoch() : 0u);
@@ -775,6 +780,7 @@ struct Traversal {
       }
       MK_HIP(hipGetLastError());
       if (int r = read_counts(h)) return r;
+      more = h.small[slot] != 0 || (h.large[slot] >> 32) != 0;
       if (profiling_on()) {
         const double bytes = 256.0 * double(h.slabs - prev.slabs) +
                              8.0 * double(h.edges - prev.edges) +

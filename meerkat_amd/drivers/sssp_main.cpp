@@ -1,7 +1,8 @@
 // mk_sssp - command-line SSSP driver (the counterpart of the reference's
 // src/SSSP.cu).  Weights are drawn as the reference draws them: srand(seed),
 // then 1 + rand() % 100 per base edge in file order, then per batch edge
-// (src/SSSP.cu:17-19,50-57,110-116).  For a file named *orkut.txt* the
+// (src/SSSP.cu:17-19,50-57,110-116); all are drawn before the device is
+// initialised (the batch files are read up front).  For a file named *orkut.txt* the
 // degree hints count the base edges (src/SSSP.cu:45-49), else they are 1.
 //
 //   mk_sssp <graph> <op 0|1> <update-prefix> <num-batches> <seed> [--dump=<path>]
@@ -31,11 +32,21 @@ int main(int argc, char **argv) {
     const bool orkut = std::strstr(file.c_str(), "orkut.txt") != nullptr;
     std::cout << file << std::flush;
 
+    // every file is read and every weight drawn before the first device
+    // call, so nothing the runtime does can consume rand() values in between
     mkdrv::EdgeList base = mkdrv::read_graph(file);
     std::vector<uint32_t> hints(base.n, 1), w(base.src.size());
     for (size_t i = 0; i < base.src.size(); ++i) {
       w[i] = 1 + uint32_t(std::rand() % 100);
       if (orkut) ++hints[base.src[i]];
+    }
+    std::vector<mkdrv::EdgeList> updates;
+    std::vector<std::vector<uint32_t>> update_w;
+    for (int i = 0; i < batches; ++i) {
+      updates.push_back(mkdrv::read_batch(a.pos[2], i));
+      std::vector<uint32_t> bw(updates.back().src.size());
+      for (auto &x : bw) x = 1 + uint32_t(std::rand() % 100);
+      update_w.push_back(std::move(bw));
     }
 
     using AllocPolicy = meerkat::FullAllocatorPolicy<9, 48, 1>;
@@ -51,10 +62,8 @@ int main(int argc, char **argv) {
     std::cout << "," << sssp.Static(uint32_t(base.src.size())) << std::flush;
 
     for (int i = 0; i < batches; ++i) {
-      mkdrv::EdgeList b = mkdrv::read_batch(a.pos[2], i);
-      std::vector<uint32_t> bw(b.src.size());
-      for (auto &x : bw) x = 1 + uint32_t(std::rand() % 100);
-      mkdrv::DeviceU32 s(b.src), d(b.dst), wd(bw);
+      const mkdrv::EdgeList &b = updates[size_t(i)];
+      mkdrv::DeviceU32 s(b.src), d(b.dst), wd(update_w[size_t(i)]);
       const uint32_t n = uint32_t(b.src.size());
       float ms;
       if (insertion) {

@@ -169,4 +169,4 @@ def test_tc_driver_counts_and_components(gpu, tmp_path, op):
     raw = np.fromfile(dump, np.uint32)
     count = int(raw[:2].view(np.uint64)[0])
     assert count == oracle.tc(g)
-    assert np.array_equal(raw[2:], oracle.cc(g))
+    assert np.array_equal(raw[2:], oracle.cc(g)[0])
