@@ -41,8 +41,8 @@ def log(msg):
 
 
 def calibration_copy(dev, nbytes=1 << 30):
-    """A plain copy kernel of known size (reads and writes `nbytes`) in its
-    own profiler region: the scale for rocprofv3's FETCH_SIZE / WRITE_SIZE
+    """A plain copy kernel of known size (reads and writes `nbytes`) between
+    marker pair 1: the scale for rocprofv3's FETCH_SIZE / WRITE_SIZE
     counters when they are collected over this program."""
     import torch
 
@@ -50,10 +50,9 @@ def calibration_copy(dev, nbytes=1 << 30):
 
     src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    with profiling.region():
-        torch.add(src, 0.0, out=dst)
-        torch.cuda.synchronize()
+    profiling.marker(dev)
+    torch.add(src, 0.0, out=dst)
+    profiling.marker(dev)
     del src, dst
 
 
@@ -234,13 +233,14 @@ def main():
     profiling.reset()
     profiling.enable(True)
     torch.cuda.synchronize()
-    # the timed region is also the region rocprofv3 --selected-regions collects
-    with profiling.region():
-        ts = time.perf_counter()
-        for i in range(args.warmup, steps_total):
-            step(i, rec)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - ts
+    # marker pair 0 delimits the timed region in a rocprofv3 trace
+    profiling.marker(dev)
+    ts = time.perf_counter()
+    for i in range(args.warmup, steps_total):
+        step(i, rec)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - ts
+    profiling.marker(dev)
     profiling.enable(False)
     prof = profiling.totals()
     calibration_copy(dev)

@@ -146,6 +146,14 @@ MK_DEV uint32_t wave_sum_u32(uint32_t v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+MK_DEV uint64_t wave_sum_u64(uint64_t v) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t lo = __shfl_xor(uint32_t(v), off, 64);
+    const uint32_t hi = __shfl_xor(uint32_t(v >> 32), off, 64);
+    v += (uint64_t(hi) << 32) | lo;
+  }
+  return v;
+}
 MK_DEV double wave_sum_f64(double v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;

@@ -8,14 +8,13 @@
 // graph in closed form.
 //
 // Method here (same counts):
-//  - Static: vertices are ranked by (degree, id); each triangle a < b < c
-//    (by rank) is counted once at a, for the pair (b, c) of a's
-//    higher-ranked neighbours, by testing c in adj[b].  A wave stages a's
-//    higher-ranked neighbours (with degrees) in LDS and tests 64 candidates c
-//    at a time: lanes that target the same bucket of b share one walk of its
-//    slab chain, every slab key being compared against every lane's c.
-//    Vertices with more than kLdsCap higher-ranked neighbours re-read a's
-//    slabs instead of LDS.
+//  - Static: vertices are ranked by (degree, id) and N+(x) holds the
+//    neighbours of x ranked above x (|N+(x)| <= sqrt(2m)).  Each triangle
+//    a < b < c is counted once, at a, as an element of N+(a) and N+(b) for b
+//    in N+(a).  The N+ lists are built compactly in HBM (count pass,
+//    exclusive scan, fill pass over the slabs); then one wave per vertex a
+//    puts N+(a) (1024 keys at a time) into an LDS hash table and streams
+//    every N+(b), b in N+(a), through it with coalesced loads.
 //  - Incremental / Decremental: the batch's changed undirected edges (the
 //    entries the last insert / delete applied) go into a device hash set.
 //    For each changed edge {u, v} the wave enumerates the neighbours w of the
@@ -35,7 +34,6 @@
 
 namespace mk {
 
-constexpr uint32_t kLdsCap = 512;     // higher-ranked neighbours staged per wave
 constexpr uint64_t kNoKey = ~0ull;
 constexpr uint32_t kTcSlots = 64;
 
@@ -59,15 +57,27 @@ MK_DEV bool probe_member(const GraphView &G, uint32_t y, bool act, uint32_t c, u
     const uint32_t bb = read_lane(bk, ffs64(pending));
     const bool mine = act && bk == bb;
     pending &= ~ballot(mine);
+    const uint64_t mm = ballot(mine);
     const uint32_t *slab = G.heads + size_t(bb) * kSlabWords;
     for (;;) {
       const uint32_t x = slab[lane];
       ++slabs;
       uint64_t live = ballot(x < kTomb) & km;
-      while (live) {
-        const uint32_t k = ffs64(live);
-        live &= live - 1;
-        hit |= mine && c == read_lane(x, k);
+      if (__popcll(mm) < __popcll(live)) {
+        // few candidates: broadcast each and compare with the whole slab
+        uint64_t t = mm;
+        while (t) {
+          const uint32_t j = ffs64(t);
+          t &= t - 1;
+          const uint32_t cj = read_lane(c, j);
+          if (ballot(x == cj) & live) hit |= lane == j;
+        }
+      } else {
+        while (live) {
+          const uint32_t k = ffs64(live);
+          live &= live - 1;
+          hit |= mine && c == read_lane(x, k);
+        }
       }
       const uint32_t nx = read_lane(x, kNextLane);
       if (nx == kEmpty) break;
@@ -108,110 +118,135 @@ MK_DEV void add_count(unsigned long long *slots, uint64_t v) {
   if (v) atomicAdd(&slots[blockIdx.x % kTcSlots], (unsigned long long)v);
 }
 
-// Static count.  One wave per vertex a (grid-stride).
-template <bool MAP>
-__global__ void __launch_bounds__(kBlock)
-    k_tc_static(GraphView G, unsigned long long *slots, unsigned long long *slab_count) {
+// Static count over higher-ranked neighbour lists.  With vertices ranked by
+// (degree, id) and N+(x) the neighbours of x ranked above x, every triangle
+// a < b < c is counted once, at a, as an element of N+(a) \cap N+(b) for
+// b in N+(a).  Pass 1 counts |N+(a)|, an exclusive scan places the lists,
+// pass 2 writes them (compact, in HBM); the intersection kernel puts N+(a)
+// (in chunks of kUpChunk) in an LDS hash table and streams each N+(b)
+// through it with coalesced loads.
+
+// Wave-collective walk of all slabs of vertex a (buckets loaded four at a
+// time); for every live key x ranked above a calls f(up-ballot, x).
+template <bool MAP, typename F>
+MK_DEV void walk_up(const GraphView &G, uint32_t a, uint32_t da, uint32_t &slabs, F f) {
   const uint32_t lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
-  const uint64_t km = key_mask<MAP>();
-  const bool key_lane = (km >> lane) & 1ull;
-  __shared__ uint32_t lk[kBlock / 64][kLdsCap];
-  __shared__ uint32_t ld[kBlock / 64][kLdsCap];
-  uint32_t *nk = lk[threadIdx.x >> 6], *nd = ld[threadIdx.x >> 6];
-  uint64_t count = 0;
-  uint32_t slabs = 0;
-  for (uint32_t a = gw; a < G.n; a += W) {
-    const uint32_t da = G.vdeg[a];
-    if (da < 2) continue;
-    const uint32_t b0 = G.bucket_off[a], b1 = G.bucket_off[a + 1];
-    // stage a's higher-ranked neighbours
-    uint32_t K = 0;
-    bool overflow = false;
-    for (uint32_t bk = b0; bk < b1; ++bk) {
-      const uint32_t *slab = G.heads + size_t(bk) * kSlabWords;
+  const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
+  const uint32_t b0 = G.bucket_off[a], b1 = G.bucket_off[a + 1];
+  for (uint32_t bk = b0; bk < b1; bk += 4) {
+    uint32_t x[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      x[k] = bk + k < b1 ? G.heads[size_t(bk + k) * kSlabWords + lane] : kEmpty;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      if (bk + k >= b1) break;
+      uint32_t v = x[k];
       for (;;) {
-        const uint32_t x = slab[lane];
         ++slabs;
-        const bool live = key_lane && x < kTomb;
-        const uint32_t dx = live ? G.vdeg[x] : 0u;
-        const bool up = live && rank_lt(da, a, dx, x);
-        const uint64_t m = ballot(up);
-        const uint32_t c = uint32_t(__popcll(m));
-        if (K + c <= kLdsCap) {
-          if (up) {
-            nk[K + lanes_below(m)] = x;
-            nd[K + lanes_below(m)] = dx;
-          }
-        } else {
-          overflow = true;
-        }
-        K += c;
-        const uint32_t nx = read_lane(x, kNextLane);
+        const bool live = key_lane && v < kTomb;
+        const uint32_t dv = live ? G.vdeg[v] : 0u;
+        const bool up = live && rank_lt(da, a, dv, v);
+        f(ballot(up), up, v);
+        const uint32_t nx = read_lane(v, kNextLane);
         if (nx == kEmpty) break;
-        slab = slab_ptr(G.pool, nx);
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (K < 2) continue;
-    if (!overflow) {
-      // pairs (b, c) of the staged list with rank(b) < rank(c)
-      for (uint32_t i = 0; i < K; ++i) {
-        const uint32_t b = nk[i], db = nd[i];
-        for (uint32_t j0 = 0; j0 < K; j0 += 64) {
-          const uint32_t j = j0 + lane;
-          uint32_t c = 0;
-          bool act = false;
-          if (j < K) {
-            c = nk[j];
-            act = rank_lt(db, b, nd[j], c);
-          }
-          if (ballot(act) == 0) continue;
-          const bool hit = probe_member<MAP>(G, b, act, c, slabs);
-          count += uint64_t(__popcll(ballot(hit)));
-        }
-      }
-    } else {
-      // too many to stage: take b and c from a's slabs directly
-      for (uint32_t bk = b0; bk < b1; ++bk) {
-        const uint32_t *slab = G.heads + size_t(bk) * kSlabWords;
-        for (;;) {
-          const uint32_t x = slab[lane];
-          const bool live = key_lane && x < kTomb;
-          const uint32_t dx = live ? G.vdeg[x] : 0u;
-          uint64_t mb = ballot(live && rank_lt(da, a, dx, x));
-          while (mb) {
-            const uint32_t jl = ffs64(mb);
-            mb &= mb - 1;
-            const uint32_t b = read_lane(x, jl), db = read_lane(dx, jl);
-            for (uint32_t ck = b0; ck < b1; ++ck) {
-              const uint32_t *cs = G.heads + size_t(ck) * kSlabWords;
-              for (;;) {
-                const uint32_t y = cs[lane];
-                const bool ly = key_lane && y < kTomb;
-                const uint32_t dy = ly ? G.vdeg[y] : 0u;
-                const bool act = ly && rank_lt(db, b, dy, y);
-                if (ballot(act)) {
-                  const bool hit = probe_member<MAP>(G, b, act, y, slabs);
-                  count += uint64_t(__popcll(ballot(hit)));
-                }
-                const uint32_t ny = read_lane(y, kNextLane);
-                if (ny == kEmpty) break;
-                cs = slab_ptr(G.pool, ny);
-              }
-            }
-          }
-          const uint32_t nx = read_lane(x, kNextLane);
-          if (nx == kEmpty) break;
-          slab = slab_ptr(G.pool, nx);
-        }
+        v = slab_ptr(G.pool, nx)[lane];
       }
     }
   }
+}
+
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_tc_count_up(GraphView G, unsigned long long *K, unsigned long long *slab_count) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  uint32_t slabs = 0;
+  for (uint32_t a = gw; a < G.n; a += W) {
+    const uint32_t da = G.vdeg[a];
+    uint64_t k = 0;
+    if (da) walk_up<MAP>(G, a, da, slabs, [&](uint64_t m, bool, uint32_t) { k += __popcll(m); });
+    if (lane == 0) K[a] = k;
+  }
+  if (lane == 0 && slabs) atomicAdd(slab_count, (unsigned long long)slabs);
+}
+
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_tc_fill_up(GraphView G, const unsigned long long *off, uint32_t *up,
+                 unsigned long long *slab_count) {
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  uint32_t slabs = 0;
+  for (uint32_t a = gw; a < G.n; a += W) {
+    const uint32_t da = G.vdeg[a];
+    if (!da) continue;
+    uint64_t pos = off[a];
+    walk_up<MAP>(G, a, da, slabs, [&](uint64_t m, bool mine, uint32_t v) {
+      if (mine) up[pos + lanes_below(m)] = v;
+      pos += __popcll(m);
+    });
+  }
+  if (lane_id() == 0 && slabs) atomicAdd(slab_count, (unsigned long long)slabs);
+}
+
+constexpr uint32_t kUpChunk = 1024;     // N+(a) keys per LDS table fill
+constexpr uint32_t kHashSlots = 2048;   // LDS hash slots per wave (load <= 1/2)
+
+MK_DEV uint32_t lds_slot(uint32_t x) { return (x * 2654435761u) >> (32 - 11); }
+
+__global__ void __launch_bounds__(kBlock)
+    k_tc_intersect(const unsigned long long *off, const uint32_t *up, uint32_t n,
+                   unsigned long long *slots, unsigned long long *streamed) {
+  static_assert(kHashSlots == 1u << 11, "lds_slot assumes 2048 slots");
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  __shared__ uint32_t table[kBlock / 64][kHashSlots];
+  uint32_t *h = table[threadIdx.x >> 6];
+  uint64_t count = 0, reads = 0;
+  for (uint32_t a = gw; a < n; a += W) {
+    const uint64_t oa = off[a];
+    const uint32_t Ka = uint32_t(off[a + 1] - oa);
+    if (Ka < 2) continue;
+    for (uint32_t c0 = 0; c0 < Ka; c0 += kUpChunk) {
+      const uint32_t cn = min(kUpChunk, Ka - c0);
+      for (uint32_t i = lane; i < kHashSlots; i += 64) h[i] = kEmpty;
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t i = lane; i < cn; i += 64) {
+        const uint32_t x = up[oa + c0 + i];
+        uint32_t sl = lds_slot(x);
+        while (atomicCAS(h + sl, kEmpty, x) != kEmpty) sl = (sl + 1) & (kHashSlots - 1);
+      }
+      __builtin_amdgcn_wave_barrier();
+      // every b in N+(a): count N+(b) elements in this chunk of N+(a)
+      for (uint32_t i = 0; i < Ka; ++i) {
+        const uint32_t b = up[oa + i];
+        const uint64_t ob = off[b];
+        const uint32_t Kb = uint32_t(off[b + 1] - ob);
+        reads += Kb;
+        for (uint32_t j = lane; j < Kb; j += 64) {
+          const uint32_t c = up[ob + j];
+          uint32_t sl = lds_slot(c);
+          for (;;) {
+            const uint32_t e = h[sl];
+            if (e == c) {
+              ++count;
+              break;
+            }
+            if (e == kEmpty) break;
+            sl = (sl + 1) & (kHashSlots - 1);
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  count = wave_sum_u64(count);
   if (lane == 0) {
     add_count(slots, count);
-    if (slabs) atomicAdd(slab_count, (unsigned long long)slabs);
+    if (reads) atomicAdd(streamed, (unsigned long long)reads);
   }
 }
 
@@ -385,12 +420,13 @@ struct TC {
   mk_graph *g = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  unsigned long long *slots = nullptr;  // kTcSlots counts + 1 slab counter
+  // kTcSlots counts, then the slab counter and the streamed-elements counter
+  unsigned long long *slots = nullptr;
   uint32_t *n_owned = nullptr;
-  DevBuf tab, owned, ddeg, doff, cursor, dlist, scan_tmp;
+  DevBuf tab, owned, ddeg, doff, cursor, dlist, scan_tmp, kup, koff, uplist;
   HostBuf host;
   unsigned long long count = 0;
-  uint64_t last_slabs = 0;
+  uint64_t last_slabs = 0, last_streamed = 0;
   int max_waves = 0;
 
   int init(mk_graph *graph) {
@@ -399,9 +435,9 @@ struct TC {
     MK_HIP(hipStreamCreate(&stream));
     MK_HIP(hipEventCreate(&ev0));
     MK_HIP(hipEventCreate(&ev1));
-    MK_HIP(hipMalloc(&slots, (kTcSlots + 1) * 8));
+    MK_HIP(hipMalloc(&slots, (kTcSlots + 2) * 8));
     MK_HIP(hipMalloc(&n_owned, 4));
-    if (int r = host.ensure((kTcSlots + 2) * 8)) return r;
+    if (int r = host.ensure((kTcSlots + 3) * 8)) return r;
     max_waves = grid_waves(g->device);
     return MK_OK;
   }
@@ -411,7 +447,8 @@ struct TC {
     if (stream) (void)hipStreamSynchronize(stream);
     (void)hipFree(slots);
     (void)hipFree(n_owned);
-    for (DevBuf *b : {&tab, &owned, &ddeg, &doff, &cursor, &dlist, &scan_tmp}) b->release();
+    for (DevBuf *b : {&tab, &owned, &ddeg, &doff, &cursor, &dlist, &scan_tmp, &kup, &koff, &uplist})
+      b->release();
     host.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -421,20 +458,21 @@ struct TC {
   int start() {
     MK_HIP(hipSetDevice(g->device));
     MK_HIP(hipEventRecord(ev0, stream));
-    fill_u64(reinterpret_cast<uint64_t *>(slots), 0ull, kTcSlots + 1, stream);
+    fill_u64(reinterpret_cast<uint64_t *>(slots), 0ull, kTcSlots + 2, stream);
     return MK_OK;
   }
 
   // Sum of the count slots; also ends timing.
   int finish(uint64_t &total, float *ms) {
     MK_HIP(hipGetLastError());
-    MK_HIP(hipMemcpyAsync(host.ptr, slots, (kTcSlots + 1) * 8, hipMemcpyDeviceToHost, stream));
+    MK_HIP(hipMemcpyAsync(host.ptr, slots, (kTcSlots + 2) * 8, hipMemcpyDeviceToHost, stream));
     MK_HIP(hipEventRecord(ev1, stream));
     MK_HIP(hipEventSynchronize(ev1));
     const unsigned long long *h = host.as<unsigned long long>();
     total = 0;
     for (uint32_t k = 0; k < kTcSlots; ++k) total += h[k];
     last_slabs = h[kTcSlots];
+    last_streamed = h[kTcSlots + 1];
     float t = 0.f;
     MK_HIP(hipEventElapsedTime(&t, ev0, ev1));
     if (ms) *ms = t;
@@ -443,17 +481,48 @@ struct TC {
 
   int run_static(float *ms) {
     if (int r = start()) return r;
-    const unsigned grid = blocks_for_waves(std::min<uint64_t>(g->n, uint64_t(max_waves) * 2));
+    const size_t n = g->n;
+    if (int r = kup.ensure((n + 1) * 8)) return r;
+    if (int r = koff.ensure((n + 1) * 8)) return r;
+    unsigned long long *K = kup.as<unsigned long long>(), *off = koff.as<unsigned long long>();
+    const unsigned grid = blocks_for_waves(std::min<uint64_t>(n, uint64_t(max_waves) * 2));
     {
-      ProfScope prof(stream, "k_tc_static", 0.0);
+      // higher-ranked neighbour lists: count, scan, fill
+      ProfScope prof(stream, "k_tc_lists", 0.0);
+      fill_u64(reinterpret_cast<uint64_t *>(K) + n, 0ull, 1, stream);
       if (g->weighted)
-        k_tc_static<true><<<grid, kBlock, 0, stream>>>(g->view(), slots, slots + kTcSlots);
+        k_tc_count_up<true><<<grid, kBlock, 0, stream>>>(g->view(), K, slots + kTcSlots);
       else
-        k_tc_static<false><<<grid, kBlock, 0, stream>>>(g->view(), slots, slots + kTcSlots);
+        k_tc_count_up<false><<<grid, kBlock, 0, stream>>>(g->view(), K, slots + kTcSlots);
+      size_t tmp = 0;
+      MK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, K, off, int(n + 1), stream));
+      if (int r = scan_tmp.ensure(tmp)) return r;
+      MK_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.ptr, tmp, K, off, int(n + 1), stream));
+      MK_HIP(hipMemcpyAsync(host.ptr, off + n, 8, hipMemcpyDeviceToHost, stream));
+      MK_HIP(hipStreamSynchronize(stream));
+      const uint64_t total = *host.as<uint64_t>();
+      if (int r = uplist.ensure(std::max<uint64_t>(total, 1) * 4)) return r;
+      if (g->weighted)
+        k_tc_fill_up<true><<<grid, kBlock, 0, stream>>>(g->view(), off, uplist.as<uint32_t>(),
+                                                       slots + kTcSlots);
+      else
+        k_tc_fill_up<false><<<grid, kBlock, 0, stream>>>(g->view(), off, uplist.as<uint32_t>(),
+                                                        slots + kTcSlots);
+    }
+    {
+      ProfScope prof(stream, "k_tc_intersect", 0.0);
+      k_tc_intersect<<<grid, kBlock, 0, stream>>>(off, uplist.as<uint32_t>(), g->n, slots,
+                                                  slots + kTcSlots + 1);
     }
     uint64_t t = 0;
     if (int r = finish(t, ms)) return r;
-    if (profiling_on()) prof_add_bytes("k_tc_static", 256.0 * double(last_slabs));
+    if (profiling_on()) {
+      // lists: 256 B per slab read (twice), 4 B per neighbour degree, 4 B
+      // per list entry written; intersection: 4 B per streamed element and
+      // per staged entry, 16 B per vertex (offsets)
+      prof_add_bytes("k_tc_lists", 256.0 * double(last_slabs));
+      prof_add_bytes("k_tc_intersect", 4.0 * double(last_streamed) + 16.0 * double(n));
+    }
     count = t;
     return MK_OK;
   }

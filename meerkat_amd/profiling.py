@@ -34,45 +34,18 @@ def totals():
     return out
 
 
-# ------------------------------------------------------------ rocprofv3 regions
-_roctx = None
+# ------------------------------------------------------------ profile markers
+MARKER_KERNEL = "FillFunctor<short>"
 
 
-def _roctx_lib():
-    """The roctx library, or False when it cannot be loaded (then profiler
-    regions are no-ops)."""
-    global _roctx
-    if _roctx is None:
-        _roctx = False
-        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
-            try:
-                L = ctypes.CDLL(name)
-                L.roctxProfilerPause.argtypes = [ctypes.c_uint64]
-                L.roctxProfilerResume.argtypes = [ctypes.c_uint64]
-                _roctx = L
-                break
-            except (OSError, AttributeError):
-                continue
-    return _roctx
+def marker(device=0):
+    """Launch a one-element int16 fill kernel and wait for it: a dispatch no
+    other code here launches, so pairs of markers delimit a region of a
+    rocprofv3 trace or counter collection (see ``region_rows``)."""
+    import torch
 
-
-class region:
-    """Context manager marking a region for ``rocprofv3 --selected-regions``:
-    collection resumes on entry and pauses on exit (roctxProfilerResume /
-    roctxProfilerPause with thread id 0).  Without the profiler these calls
-    do nothing."""
-
-    def __enter__(self):
-        L = _roctx_lib()
-        if L:
-            L.roctxProfilerResume(0)
-        return self
-
-    def __exit__(self, *exc):
-        L = _roctx_lib()
-        if L:
-            L.roctxProfilerPause(0)
-        return False
+    torch.zeros(1, dtype=torch.int16, device=device)
+    torch.cuda.synchronize()
 
 
 # ------------------------------------------------------------ rocprofv3 output
@@ -95,11 +68,43 @@ def kernel_stats(path, top=20):
     return rows[:top]
 
 
-def pmc_per_kernel(path, counter):
-    """Mean per-dispatch value of `counter` for each kernel name in a
-    rocprofv3 ``*_counter_collection.csv``: {name: (dispatches, mean)}."""
+def region_rows(rows, region, marker=MARKER_KERNEL):
+    """Rows of a rocprofv3 per-dispatch CSV (kernel trace or counter
+    collection) strictly between the (2*region)-th and (2*region+1)-th
+    marker dispatches, in dispatch order.  region None: all rows."""
+    if region is None:
+        return rows
+    rows = sorted(rows, key=lambda r: int(r["Dispatch_Id"]))
+    ids = []
+    for r in rows:
+        if marker in r["Kernel_Name"] and (not ids or ids[-1] != r["Dispatch_Id"]):
+            ids.append(r["Dispatch_Id"])
+    if len(ids) < 2 * region + 2:
+        raise ValueError(f"only {len(ids)} marker dispatches; region {region} needs "
+                         f"{2 * region + 2}")
+    lo, hi = int(ids[2 * region]), int(ids[2 * region + 1])
+    return [r for r in rows if lo < int(r["Dispatch_Id"]) < hi]
+
+
+def trace_stats(path, region=None):
+    """Per-kernel (dispatches, total_ms, avg_us, min_us, max_us) from a
+    rocprofv3 ``*_kernel_trace.csv``, restricted to a marker region."""
     acc = {}
-    for r in _read_csv(path):
+    for r in region_rows(_read_csv(path), region):
+        dt = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        n, tot, mn, mx = acc.get(r["Kernel_Name"], (0, 0.0, float("inf"), 0.0))
+        acc[r["Kernel_Name"]] = (n + 1, tot + dt, min(mn, dt), max(mx, dt))
+    out = [(k, n, tot / 1e3, tot / n, mn, mx) for k, (n, tot, mn, mx) in acc.items()]
+    out.sort(key=lambda x: -x[2])
+    return out
+
+
+def pmc_per_kernel(path, counter, region=None):
+    """Mean per-dispatch value of `counter` for each kernel name in a
+    rocprofv3 ``*_counter_collection.csv`` (optionally restricted to a marker
+    region): {name: (dispatches, mean)}."""
+    acc = {}
+    for r in region_rows(_read_csv(path), region):
         if r.get("Counter_Name") != counter:
             continue
         k = r["Kernel_Name"]
@@ -117,18 +122,29 @@ def _main(argv):
     a = sub.add_parser("stats", help="kernel time table from *_kernel_stats.csv")
     a.add_argument("csv")
     a.add_argument("--top", type=int, default=20)
+    t = sub.add_parser("trace", help="kernel time table from *_kernel_trace.csv")
+    t.add_argument("csv")
+    t.add_argument("--region", type=int, default=None,
+                   help="only dispatches between marker pair REGION (0 = first)")
+    t.add_argument("--top", type=int, default=20)
     b = sub.add_parser("pmc", help="mean per-dispatch counter value per kernel")
     b.add_argument("csv")
     b.add_argument("--counter", required=True)
+    b.add_argument("--region", type=int, default=None)
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(f"{'calls':>6} {'total_ms':>10} {'avg_us':>10} {'min_us':>9} {'max_us':>10} "
               f"{'%':>6}  kernel")
         for name, calls, tot, avg, mn, mx, pct in kernel_stats(args.csv, args.top):
             print(f"{calls:6d} {tot:10.3f} {avg:10.2f} {mn:9.2f} {mx:10.2f} {pct:6.2f}  {name}")
+    elif args.cmd == "trace":
+        print(f"{'calls':>6} {'total_ms':>10} {'avg_us':>10} {'min_us':>9} {'max_us':>10}  kernel")
+        for name, calls, tot, avg, mn, mx in trace_stats(args.csv, args.region)[:args.top]:
+            print(f"{calls:6d} {tot:10.3f} {avg:10.2f} {mn:9.2f} {mx:10.2f}  {name}")
     else:
         print(f"{'dispatches':>10} {'mean ' + args.counter:>20}  kernel")
-        for name, (n, mean) in sorted(pmc_per_kernel(args.csv, args.counter).items(),
+        for name, (n, mean) in sorted(pmc_per_kernel(args.csv, args.counter,
+                                                     args.region).items(),
                                       key=lambda kv: -kv[1][1] * kv[1][0]):
             print(f"{n:10d} {mean:20.1f}  {name}")
 

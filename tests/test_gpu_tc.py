@@ -65,10 +65,10 @@ def test_static_incremental_decremental_config4_shape(gpu, weighted, hints):
         assert tc.GetCount() == oracle.tc(oracle.csr(n, s, d)), i
 
 
-def test_clique_overflow_path(gpu):
-    # a 600-clique: vertex 0 has 599 higher-ranked neighbours (> the LDS
-    # staging capacity), so the overflow path is taken; C(600, 3) triangles
-    k = 600
+@pytest.mark.parametrize("k", [600, 1500])
+def test_clique_counts(gpu, k):
+    # a k-clique: vertex 0 has k - 1 higher-ranked neighbours; at k = 1500
+    # they fill more than one 1024-key LDS table; C(k, 3) triangles
     a, b = np.meshgrid(np.arange(k, dtype=np.uint32), np.arange(k, dtype=np.uint32))
     m = a != b
     s, d = a[m].ravel(), b[m].ravel()
