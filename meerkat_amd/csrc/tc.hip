@@ -156,18 +156,33 @@ MK_DEV void walk_up(const GraphView &G, uint32_t a, uint32_t da, uint32_t &slabs
   }
 }
 
+// Vertices are handed to waves kGrab at a time from a device counter: RMAT
+// puts its heavy vertices at ids with many low zero bits, so a fixed
+// grid stride (a power of two) would give them all to the same few waves.
+constexpr uint32_t kGrab = 16;
+
+// 64-bit so the counter cannot wrap past n while waves are still taking.
+MK_DEV uint64_t grab(unsigned long long *next) {
+  unsigned long long a0 = 0;
+  if (lane_id() == 0) a0 = atomicAdd(next, (unsigned long long)kGrab);
+  const uint32_t lo = read_lane(uint32_t(a0), 0), hi = read_lane(uint32_t(a0 >> 32), 0);
+  return (uint64_t(hi) << 32) | lo;
+}
+
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
-    k_tc_count_up(GraphView G, unsigned long long *K, unsigned long long *slab_count) {
+    k_tc_count_up(GraphView G, unsigned long long *K, unsigned long long *slab_count,
+                  unsigned long long *next) {
   const uint32_t lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
   uint32_t slabs = 0;
-  for (uint32_t a = gw; a < G.n; a += W) {
-    const uint32_t da = G.vdeg[a];
-    uint64_t k = 0;
-    if (da) walk_up<MAP>(G, a, da, slabs, [&](uint64_t m, bool, uint32_t) { k += __popcll(m); });
-    if (lane == 0) K[a] = k;
+  for (uint64_t a0 = grab(next); a0 < G.n; a0 = grab(next)) {
+    const uint32_t a1 = uint32_t(min<uint64_t>(a0 + kGrab, G.n));
+    for (uint32_t a = uint32_t(a0); a < a1; ++a) {
+      const uint32_t da = G.vdeg[a];
+      uint64_t k = 0;
+      if (da) walk_up<MAP>(G, a, da, slabs, [&](uint64_t m, bool, uint32_t) { k += __popcll(m); });
+      if (lane == 0) K[a] = k;
+    }
   }
   if (lane == 0 && slabs) atomicAdd(slab_count, (unsigned long long)slabs);
 }
@@ -175,11 +190,10 @@ __global__ void __launch_bounds__(kBlock)
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_tc_fill_up(GraphView G, const unsigned long long *off, uint32_t *up,
-                 unsigned long long *slab_count) {
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+                 unsigned long long *slab_count, unsigned long long *next) {
   uint32_t slabs = 0;
-  for (uint32_t a = gw; a < G.n; a += W) {
+  for (uint64_t a0 = grab(next); a0 < G.n; a0 = grab(next))
+  for (uint32_t a = uint32_t(a0); a < uint32_t(min<uint64_t>(a0 + kGrab, G.n)); ++a) {
     const uint32_t da = G.vdeg[a];
     if (!da) continue;
     uint64_t pos = off[a];
@@ -198,15 +212,15 @@ MK_DEV uint32_t lds_slot(uint32_t x) { return (x * 2654435761u) >> (32 - 11); }
 
 __global__ void __launch_bounds__(kBlock)
     k_tc_intersect(const unsigned long long *off, const uint32_t *up, uint32_t n,
-                   unsigned long long *slots, unsigned long long *streamed) {
+                   unsigned long long *slots, unsigned long long *streamed,
+                   unsigned long long *next) {
   static_assert(kHashSlots == 1u << 11, "lds_slot assumes 2048 slots");
   const uint32_t lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
   __shared__ uint32_t table[kBlock / 64][kHashSlots];
   uint32_t *h = table[threadIdx.x >> 6];
   uint64_t count = 0, reads = 0;
-  for (uint32_t a = gw; a < n; a += W) {
+  for (uint64_t a0 = grab(next); a0 < n; a0 = grab(next))
+  for (uint32_t a = uint32_t(a0); a < uint32_t(min<uint64_t>(a0 + kGrab, n)); ++a) {
     const uint64_t oa = off[a];
     const uint32_t Ka = uint32_t(off[a + 1] - oa);
     if (Ka < 2) continue;
@@ -220,23 +234,40 @@ __global__ void __launch_bounds__(kBlock)
         while (atomicCAS(h + sl, kEmpty, x) != kEmpty) sl = (sl + 1) & (kHashSlots - 1);
       }
       __builtin_amdgcn_wave_barrier();
-      // every b in N+(a): count N+(b) elements in this chunk of N+(a)
+      // every b in N+(a): count N+(b) elements in this chunk of N+(a); the
+      // next b's list bounds load while the current list streams, four
+      // 64-element rows at a time
+      uint32_t bn = up[oa];
+      uint64_t obn = off[bn], oen = off[bn + 1];
       for (uint32_t i = 0; i < Ka; ++i) {
-        const uint32_t b = up[oa + i];
-        const uint64_t ob = off[b];
-        const uint32_t Kb = uint32_t(off[b + 1] - ob);
+        const uint64_t ob = obn;
+        const uint32_t Kb = uint32_t(oen - obn);
+        if (i + 1 < Ka) {
+          bn = up[oa + i + 1];
+          obn = off[bn];
+          oen = off[bn + 1];
+        }
         reads += Kb;
-        for (uint32_t j = lane; j < Kb; j += 64) {
-          const uint32_t c = up[ob + j];
-          uint32_t sl = lds_slot(c);
-          for (;;) {
-            const uint32_t e = h[sl];
-            if (e == c) {
-              ++count;
-              break;
+        for (uint32_t j0 = 0; j0 < Kb; j0 += 256) {
+          uint32_t c[4];
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t j = j0 + k * 64 + lane;
+            c[k] = j < Kb ? up[ob + j] : kEmpty;  // ids are < kEmpty
+          }
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) {
+            if (c[k] == kEmpty) continue;
+            uint32_t sl = lds_slot(c[k]);
+            for (;;) {
+              const uint32_t e = h[sl];
+              if (e == c[k]) {
+                ++count;
+                break;
+              }
+              if (e == kEmpty) break;
+              sl = (sl + 1) & (kHashSlots - 1);
             }
-            if (e == kEmpty) break;
-            sl = (sl + 1) & (kHashSlots - 1);
           }
         }
       }
@@ -423,7 +454,7 @@ struct TC {
   // kTcSlots counts, then the slab counter and the streamed-elements counter
   unsigned long long *slots = nullptr;
   uint32_t *n_owned = nullptr;
-  DevBuf tab, owned, ddeg, doff, cursor, dlist, scan_tmp, kup, koff, uplist;
+  DevBuf tab, owned, ddeg, doff, cursor, dlist, scan_tmp, kup, koff, uplist, grabs;
   HostBuf host;
   unsigned long long count = 0;
   uint64_t last_slabs = 0, last_streamed = 0;
@@ -447,7 +478,8 @@ struct TC {
     if (stream) (void)hipStreamSynchronize(stream);
     (void)hipFree(slots);
     (void)hipFree(n_owned);
-    for (DevBuf *b : {&tab, &owned, &ddeg, &doff, &cursor, &dlist, &scan_tmp, &kup, &koff, &uplist})
+    for (DevBuf *b :
+         {&tab, &owned, &ddeg, &doff, &cursor, &dlist, &scan_tmp, &kup, &koff, &uplist, &grabs})
       b->release();
     host.release();
     if (ev0) (void)hipEventDestroy(ev0);
@@ -484,16 +516,21 @@ struct TC {
     const size_t n = g->n;
     if (int r = kup.ensure((n + 1) * 8)) return r;
     if (int r = koff.ensure((n + 1) * 8)) return r;
+    if (int r = grabs.ensure(3 * 8)) return r;
     unsigned long long *K = kup.as<unsigned long long>(), *off = koff.as<unsigned long long>();
-    const unsigned grid = blocks_for_waves(std::min<uint64_t>(n, uint64_t(max_waves) * 2));
+    // one vertex counter per pass
+    unsigned long long *next = grabs.as<unsigned long long>();
+    fill_u64(reinterpret_cast<uint64_t *>(next), 0ull, 3, stream);
+    const unsigned grid = blocks_for_waves(
+        std::min<uint64_t>((n + kGrab - 1) / kGrab, uint64_t(max_waves) * 2));
     {
       // higher-ranked neighbour lists: count, scan, fill
       ProfScope prof(stream, "k_tc_lists", 0.0);
       fill_u64(reinterpret_cast<uint64_t *>(K) + n, 0ull, 1, stream);
       if (g->weighted)
-        k_tc_count_up<true><<<grid, kBlock, 0, stream>>>(g->view(), K, slots + kTcSlots);
+        k_tc_count_up<true><<<grid, kBlock, 0, stream>>>(g->view(), K, slots + kTcSlots, next);
       else
-        k_tc_count_up<false><<<grid, kBlock, 0, stream>>>(g->view(), K, slots + kTcSlots);
+        k_tc_count_up<false><<<grid, kBlock, 0, stream>>>(g->view(), K, slots + kTcSlots, next);
       size_t tmp = 0;
       MK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, K, off, int(n + 1), stream));
       if (int r = scan_tmp.ensure(tmp)) return r;
@@ -504,15 +541,15 @@ struct TC {
       if (int r = uplist.ensure(std::max<uint64_t>(total, 1) * 4)) return r;
       if (g->weighted)
         k_tc_fill_up<true><<<grid, kBlock, 0, stream>>>(g->view(), off, uplist.as<uint32_t>(),
-                                                       slots + kTcSlots);
+                                                       slots + kTcSlots, next + 1);
       else
         k_tc_fill_up<false><<<grid, kBlock, 0, stream>>>(g->view(), off, uplist.as<uint32_t>(),
-                                                        slots + kTcSlots);
+                                                        slots + kTcSlots, next + 1);
     }
     {
       ProfScope prof(stream, "k_tc_intersect", 0.0);
       k_tc_intersect<<<grid, kBlock, 0, stream>>>(off, uplist.as<uint32_t>(), g->n, slots,
-                                                  slots + kTcSlots + 1);
+                                                  slots + kTcSlots + 1, next + 2);
     }
     uint64_t t = 0;
     if (int r = finish(t, ms)) return r;
