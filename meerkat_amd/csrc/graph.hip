@@ -134,23 +134,39 @@ __global__ void k_validate(const uint32_t *src, const uint32_t *dst, uint32_t n,
   if (m && lane_id() == ffs64(m)) atomicAdd(counters, uint32_t(__popcll(m)));
 }
 
-// Each wave takes `epw` (<= 64) consecutive edges, lane i holding edge i.
-// The four 16-lane groups of the wave apply their own lanes' edges one after
-// another, each group walking and claiming in its own slab chain (the
-// reference runs one edge at a time per warp, include/Graph/warp/
-// InsertEdge.cuh:66-103).
+// One side of a batch mutation: the graph it applies to, that graph's
+// per-edge status and counters, and whether it takes (dst, src) (the in-edge
+// mirror) instead of (src, dst).
+struct MutSide {
+  GraphView G;
+  uint8_t *status;
+  uint32_t *counters;
+  uint32_t swap;
+};
+
+// Waves [0, waves_a) apply the batch to side A, the rest to side B, so the
+// forward graph and its mirror are updated by one launch with their
+// (latency-bound) chain walks overlapping.  Each wave takes `epw` (<= 64)
+// consecutive edges, lane i holding edge i; the four 16-lane groups of the
+// wave apply their own lanes' edges one after another, each group walking
+// and claiming in its own slab chain (the reference runs one edge at a time
+// per warp, include/Graph/warp/InsertEdge.cuh:66-103).
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
-    k_insert(GraphView G, PoolView P, const uint32_t *src, const uint32_t *dst, const uint32_t *wt,
-             uint32_t n, uint32_t epw, uint8_t *status, uint32_t *counters, const uint32_t *bad) {
+    k_insert(MutSide A, MutSide B, uint32_t waves_a, PoolView P, const uint32_t *src,
+             const uint32_t *dst, const uint32_t *wt, uint32_t n, uint32_t epw,
+             const uint32_t *bad) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const bool second = wave_all >= waves_a;
+  const MutSide &M = second ? B : A;
+  const uint32_t wave = second ? wave_all - waves_a : wave_all;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
   if (*bad) {  // the batch failed the range check: leave the graph as it is
-    if (lane < cnt) status[base + lane] = MK_EDGE_UNCHANGED;
+    if (lane < cnt) M.status[base + lane] = MK_EDGE_UNCHANGED;
     return;
   }
   uint32_t s = 0, d = 0, w = 0;
@@ -159,17 +175,24 @@ __global__ void __launch_bounds__(kBlock)
   if (lane < cnt) {
     s = src[base + lane];
     d = dst[base + lane];
+    if (M.swap) {
+      const uint32_t t = s;
+      s = d;
+      d = t;
+    }
     w = (MAP && wt) ? wt[base + lane] : 0u;
     act = s != d;
     if (!act) st = MK_EDGE_SELF_LOOP;
   }
+  const GraphView &G = M.G;
   uint32_t pending = group_bits(ballot(act));
   uint32_t applied = 0, pool_fail = 0;
+  const uint32_t seed_base = (second ? 0x9E3779B9u : 0u) + wave * 64u + group_first_lane();
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     const uint32_t sj = group_read(s, j), dj = group_read(d, j), wj = group_read(w, j);
-    const int r = slab_insert_group<MAP>(G, P, sj, dj, wj, wave * 64u + group_first_lane() + j);
+    const int r = slab_insert_group<MAP>(G, P, sj, dj, wj, seed_base + j);
     if (gl == j) {
       if (r == 1) st = MK_EDGE_APPLIED;
       else if (r < 0) st = MK_EDGE_POOL_FULL;  // rolled back by the host
@@ -181,33 +204,34 @@ __global__ void __launch_bounds__(kBlock)
       ++pool_fail;
     }
   }
-  if (lane < cnt) status[base + lane] = st;
+  if (lane < cnt) M.status[base + lane] = st;
   // group totals (held by every lane of a group) summed over the 4 groups
-  applied = gl == 0 ? applied : 0u;
-  pool_fail = gl == 0 ? pool_fail : 0u;
-  applied = wave_sum_u32(applied);
-  pool_fail = wave_sum_u32(pool_fail);
+  applied = wave_sum_u32(gl == 0 ? applied : 0u);
+  pool_fail = wave_sum_u32(gl == 0 ? pool_fail : 0u);
   if (lane == 0) {
-    if (applied) atomicAdd(counters + 2, applied);
-    if (pool_fail) atomicAdd(counters + 1, pool_fail);
+    if (applied) atomicAdd(M.counters + 2, applied);
+    if (pool_fail) atomicAdd(M.counters + 1, pool_fail);
   }
 }
 
 // Delete the batch edges whose status equals `only_status` (all edges when
-// only_status == 0xFF); records MK_EDGE_APPLIED / MK_EDGE_UNCHANGED.  Edges
-// are applied per 16-lane group as in k_insert.
+// only_status == 0xFF); records MK_EDGE_APPLIED / MK_EDGE_UNCHANGED.  Sides
+// and groups as in k_insert.
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
-    k_delete(GraphView G, const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t epw,
-             uint8_t *status, uint32_t only_status, uint32_t *counters, const uint32_t *bad) {
+    k_delete(MutSide A, MutSide B, uint32_t waves_a, const uint32_t *src, const uint32_t *dst,
+             uint32_t n, uint32_t epw, uint32_t only_status, const uint32_t *bad) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const bool second = wave_all >= waves_a;
+  const MutSide &M = second ? B : A;
+  const uint32_t wave = second ? wave_all - waves_a : wave_all;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
   if (*bad) {  // the batch failed the range check: leave the graph as it is
-    if (lane < cnt && only_status == 0xFFu) status[base + lane] = MK_EDGE_UNCHANGED;
+    if (lane < cnt && only_status == 0xFFu) M.status[base + lane] = MK_EDGE_UNCHANGED;
     return;
   }
   uint32_t s = 0, d = 0;
@@ -216,13 +240,19 @@ __global__ void __launch_bounds__(kBlock)
   if (lane < cnt) {
     s = src[base + lane];
     d = dst[base + lane];
+    if (M.swap) {
+      const uint32_t t = s;
+      s = d;
+      d = t;
+    }
     if (only_status != 0xFFu) {
-      act = status[base + lane] == only_status;
+      act = M.status[base + lane] == only_status;
     } else {
       act = s != d;
       if (!act) st = MK_EDGE_SELF_LOOP;
     }
   }
+  const GraphView &G = M.G;
   uint32_t pending = group_bits(ballot(act));
   uint32_t applied = 0;
   while (pending) {
@@ -237,12 +267,12 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   if (only_status != 0xFFu) {
-    if (lane < cnt && act) status[base + lane] = MK_EDGE_UNCHANGED;
+    if (lane < cnt && act) M.status[base + lane] = MK_EDGE_UNCHANGED;
   } else if (lane < cnt) {
-    status[base + lane] = st;
+    M.status[base + lane] = st;
   }
   applied = wave_sum_u32(gl == 0 ? applied : 0u);
-  if (lane == 0 && applied) atomicAdd(counters + 2, applied);
+  if (lane == 0 && applied) atomicAdd(M.counters + 2, applied);
 }
 
 template <bool MAP>
@@ -603,43 +633,43 @@ static int range_error(mk_graph *g, const char *who) {
                                 " edges have a vertex id >= n; graph unchanged");
 }
 
-// Launch the insert kernel on graph g (forward or mirror) with its status
-// buffer and counters.
+// The mutation side of graph g (its own view, status and counters).
+static MutSide side_of(mk_graph *g, bool swap) {
+  return MutSide{g->view(), g->status.as<uint8_t>(), g->counters, swap ? 1u : 0u};
+}
+
+// Launch the insert kernel over g and, when it has one, its mirror (one
+// launch, half the waves each).
 static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
                           const uint32_t *w, uint32_t n, hipStream_t s, const char *name,
                           const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
-  uint8_t *st = g->status.as<uint8_t>();
-  // per edge: batch entry, one slab probe, key (+value) write, degree atomic,
-  // status byte
-  ProfScope prof(s, name, double(n) * (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1));
-  if (g->weighted)
-    k_insert<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src, dst,
-                                                              w, n, epw, st, g->counters, bad);
-  else
-    k_insert<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), g->pool->view, src,
-                                                               dst, nullptr, n, epw, st,
-                                                               g->counters, bad);
+  const MutSide a = side_of(g, false), b = g->rev ? side_of(g->rev, true) : a;
+  const uint64_t total = g->rev ? 2 * waves : waves;
+  // per edge and graph: batch entry, one slab probe, key (+value) write,
+  // degree atomic, status byte
+  const double bytes = double(n) * (g->rev ? 2 : 1) *
+                       (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1);
+  prof_launch(name, bytes, g->weighted ? k_insert<true> : k_insert<false>,
+              dim3(blocks_for_waves(total)), dim3(kBlock), 0u, s, a, b, uint32_t(waves),
+              g->pool->view, src, dst, g->weighted ? w : nullptr, n, epw, bad);
 }
 
-// Launch the delete kernel on graph g; only_status != 0xFF restricts it to
-// edges whose status equals it (roll-back of an insert).
+// Launch the delete kernel over g (and its mirror); only_status != 0xFF
+// restricts it to edges whose status equals it (roll-back of an insert).
 static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
                           hipStream_t s, uint32_t only_status, const char *name,
                           const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
-  uint8_t *st = g->status.as<uint8_t>();
-  // per edge: batch entry, one slab probe, tombstone write, degree atomic,
-  // status byte
-  ProfScope prof(s, name, double(n) * (8 + 256 + 4 + 8 + 1));
-  if (g->weighted)
-    k_delete<true><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                              only_status, g->counters, bad);
-  else
-    k_delete<false><<<blocks_for_waves(waves), kBlock, 0, s>>>(g->view(), src, dst, n, epw, st,
-                                                               only_status, g->counters, bad);
+  const MutSide a = side_of(g, false), b = g->rev ? side_of(g->rev, true) : a;
+  const uint64_t total = g->rev ? 2 * waves : waves;
+  // per edge and graph: batch entry, one slab probe, tombstone write, degree
+  // atomic, status byte
+  prof_launch(name, double(n) * (g->rev ? 2 : 1) * (8 + 256 + 4 + 8 + 1),
+              g->weighted ? k_delete<true> : k_delete<false>, dim3(blocks_for_waves(total)),
+              dim3(kBlock), 0u, s, a, b, uint32_t(waves), src, dst, n, epw, only_status, bad);
 }
 
 // Status buffer and counters of the in-edge mirror for a batch of n edges.
@@ -660,7 +690,6 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n, s)) return r;
   launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
-  if (g->rev) launch_insert(g->rev, dst, src, w, n, s, "k_insert_mirror", g->counters);
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
   if (g->rev)
@@ -674,8 +703,6 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
     // roll back: delete what this batch inserted, restoring the edge set
     // (counters[0] is zero here, so the kernels run)
     launch_delete(g, src, dst, n, s, MK_EDGE_APPLIED, "k_insert_rollback", g->counters);
-    if (g->rev)
-      launch_delete(g->rev, dst, src, n, s, MK_EDGE_APPLIED, "k_insert_rollback", g->counters);
     MK_HIP(hipStreamSynchronize(s));
     return fail(MK_ERR_POOL_FULL, "mk_graph_insert: slab pool exhausted (" +
                                       std::to_string(pool_fail) +
@@ -696,7 +723,6 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n, s)) return r;
   launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);
-  if (g->rev) launch_delete(g->rev, dst, src, n, s, 0xFFu, "k_delete_mirror", g->counters);
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
   MK_HIP(hipStreamSynchronize(s));

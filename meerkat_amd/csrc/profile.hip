@@ -84,6 +84,22 @@ void prof_end(hipEvent_t start, hipStream_t s, const char *name, double algo_byt
   if (g_pending.size() > 4096) resolve_locked();
 }
 
+bool prof_events(hipEvent_t &start, hipEvent_t &stop) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  start = take_event();
+  stop = start ? take_event() : nullptr;
+  if (start && stop) return true;
+  if (start) g_free.push_back(start);
+  start = stop = nullptr;
+  return false;
+}
+
+void prof_commit(hipEvent_t start, hipEvent_t stop, const char *name, double algo_bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_pending.push_back({start, stop, name, algo_bytes});
+  if (g_pending.size() > 4096) resolve_locked();
+}
+
 void prof_add_bytes(const char *name, double algo_bytes) {
   if (!profiling_on()) return;
   std::lock_guard<std::mutex> lk(g_mu);

@@ -58,6 +58,8 @@ enum { kStatSlabs = 0, kStatEdges = 1, kStatVisited = 2, kStatPushed = 3 };
 struct Counts {
   uint32_t small[3];
   uint32_t steps;                 // steps that found a non-empty frontier
+  uint32_t ninv;                  // entries of the invalidated-vertex list
+  uint32_t pad;
   unsigned long long large[3];    // (entries << 32) | bucket items
   unsigned long long stats[kStatSlots][4];
 };
@@ -97,6 +99,7 @@ struct TravView {
   uint64_t *dist;
   uint32_t *stamp;
   uint32_t *invbits;
+  uint32_t *inv_list;  // invalidated vertices, compacted from invbits
   uint32_t *q_small[2];
   uint2 *q_large[2];
   Counts *cnt;
@@ -448,29 +451,27 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
   __shared__ uint32_t lq[kBlock / 64][kWaveQ];
   WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
-  const uint32_t nwords = (G.n + 31) / 32;
+  const uint32_t total = S.cnt->ninv;
+  const uint32_t per = per_wave(total, W);
   uint32_t slabs = 0, edges = 0, visited = 0;
-  for (uint32_t w0 = gw * 64; w0 < nwords; w0 += W * 64) {
-    uint32_t word = w0 + lane < nwords ? S.invbits[w0 + lane] : 0u;
-    while (ballot(word != 0)) {
-      bool act = word != 0;
-      uint32_t v = 0, rb0 = 0, rnb = 0;
-      if (act) {
-        v = (w0 + lane) * 32u + uint32_t(__ffs(word) - 1);
-        word &= word - 1;
-        rb0 = R.bucket_off[v];
-        rnb = R.bucket_off[v + 1] - rb0;
-        if (rnb > kSmallBuckets) {
-          const unsigned long long r = atomicAdd(&S.cnt->large[1], (1ull << 32) | rnb);
-          S.q_large[1][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
-          act = false;
-        }
+  for (uint32_t i0 = gw * per; i0 < total; i0 += W * per) {
+    const uint32_t i = i0 + lane;
+    bool act = lane < per && i < total;
+    uint32_t v = 0, rb0 = 0, rnb = 0;
+    if (act) {
+      v = S.inv_list[i];
+      rb0 = R.bucket_off[v];
+      rnb = R.bucket_off[v + 1] - rb0;
+      if (rnb > kSmallBuckets) {
+        const unsigned long long r = atomicAdd(&S.cnt->large[1], (1ull << 32) | rnb);
+        S.q_large[1][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+        act = false;
       }
-      visited += uint32_t(__popcll(ballot(act)));
-      for (uint32_t k = 0; ballot(act && rnb > k); ++k)
-        pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act && rnb > k), rb0 + k, v, slabs,
-                                   edges);
     }
+    visited += uint32_t(__popcll(ballot(act)));
+    for (uint32_t k = 0; ballot(act && rnb > k); ++k)
+      pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act && rnb > k), rb0 + k, v, slabs,
+                                 edges);
   }
   wq_flush(q, S.q_small[0], &S.cnt->small[0]);
   if (lane == 0) {
@@ -478,6 +479,31 @@ __global__ void __launch_bounds__(kBlock)
     add_stat(S.cnt, kStatEdges, edges);
     add_stat(S.cnt, kStatVisited, visited);
     add_stat(S.cnt, kStatPushed, q.pushed);
+  }
+}
+
+// Compact the invalidation bitmap into S.inv_list (count in S.cnt->ninv):
+// one bitmap word per thread, a wave prefix sum of the words' popcounts and
+// one atomicAdd per wave.
+__global__ void __launch_bounds__(kBlock) k_inv_compact(TravView S, uint32_t nwords) {
+  const uint32_t lane = lane_id();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t word = i < nwords ? S.invbits[i] : 0u;
+  const uint32_t c = uint32_t(__popc(word));
+  uint32_t x = c;  // inclusive prefix sum over the wave
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  const uint32_t total = read_lane(x, 63);
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&S.cnt->ninv, total);
+  uint32_t pos = read_lane(base, 0) + x - c;
+  while (word) {
+    S.inv_list[pos++] = i * 32u + uint32_t(__ffs(word) - 1);
+    word &= word - 1;
   }
 }
 
@@ -632,6 +658,7 @@ struct Traversal {
   uint64_t *dist = nullptr;
   uint32_t *stamp = nullptr;
   uint32_t *invbits = nullptr;
+  uint32_t *inv_list = nullptr;
   uint32_t *qs[2] = {nullptr, nullptr};
   uint2 *ql[2] = {nullptr, nullptr};
   Counts *cnt = nullptr;  // device
@@ -647,6 +674,7 @@ struct Traversal {
     v.dist = dist;
     v.stamp = stamp;
     v.invbits = invbits;
+    v.inv_list = inv_list;
     v.q_small[0] = qs[0];
     v.q_small[1] = qs[1];
     v.q_large[0] = ql[0];
@@ -667,6 +695,7 @@ struct Traversal {
     MK_HIP(hipMalloc(&dist, n * 8));
     MK_HIP(hipMalloc(&stamp, n * 4));
     MK_HIP(hipMalloc(&invbits, ((n + 31) / 32) * 4));
+    MK_HIP(hipMalloc(&inv_list, n * 4));
     for (int i = 0; i < 2; ++i) {
       MK_HIP(hipMalloc(&qs[i], n * 4));
       MK_HIP(hipMalloc(&ql[i], n * 8));
@@ -689,6 +718,7 @@ struct Traversal {
     (void)hipFree(dist);
     (void)hipFree(stamp);
     (void)hipFree(invbits);
+    (void)hipFree(inv_list);
     for (int i = 0; i < 2; ++i) {
       (void)hipFree(qs[i]);
       (void)hipFree(ql[i]);
@@ -741,15 +771,10 @@ struct Traversal {
     const unsigned grid = blocks_for_waves(uint64_t(max_waves));
     const GraphView G = g->view();
     const TravView S = view();
-    ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
-    if (g->weighted) {
-      if (weights)
-        k_expand<true, true, MODE><<<grid, kBlock, 0, stream>>>(G, S, side, slot, ep);
-      else
-        k_expand<true, false, MODE><<<grid, kBlock, 0, stream>>>(G, S, side, slot, ep);
-    } else {
-      k_expand<false, false, MODE><<<grid, kBlock, 0, stream>>>(G, S, side, slot, ep);
-    }
+    const char *name = MODE == 0 ? "k_expand_relax" : "k_expand_invalidate";
+    auto k = g->weighted ? (weights ? k_expand<true, true, MODE> : k_expand<true, false, MODE>)
+                         : k_expand<false, false, MODE>;
+    prof_launch(name, 0.0, k, dim3(grid), dim3(kBlock), 0u, stream, G, S, side, slot, ep);
   }
 
   // Run frontier steps from queue side 0 / slot 0 until the frontier is
@@ -802,13 +827,13 @@ struct Traversal {
   }
 
   int end(float *ms) {
+    // the stop event precedes the counter copy; one stream sync covers both
     MK_HIP(hipEventRecord(ev1, stream));
-    MK_HIP(hipEventSynchronize(ev1));
+    Totals h;
+    if (int r = read_counts(h)) return r;
     float t = 0.f;
     MK_HIP(hipEventElapsedTime(&t, ev0, ev1));
     if (ms) *ms = t;
-    Totals h;
-    if (int r = read_counts(h)) return r;
     last_slabs = h.slabs;
     last_edges = h.edges;
     last_iters = h.steps;
@@ -836,17 +861,11 @@ struct Traversal {
                         g->status_n == n && g->last_w == w;
       const uint8_t *st = same ? g->status.as<uint8_t>() : nullptr;
       {
-        ProfScope prof(stream, "k_seed_batch", 24.0 * n);
         const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
-        if (weights)
-          k_seed_batch<true, true><<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst, w, st, n,
-                                                                   ep);
-        else if (g->weighted)
-          k_seed_batch<true, false><<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst, nullptr,
-                                                                    nullptr, n, ep);
-        else
-          k_seed_batch<false, false><<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst,
-                                                                     nullptr, nullptr, n, ep);
+        auto k = weights ? k_seed_batch<true, true>
+                         : (g->weighted ? k_seed_batch<true, false> : k_seed_batch<false, false>);
+        prof_launch("k_seed_batch", 24.0 * n, k, dim3(grid), dim3(kItemBlock), 0u, stream, G,
+                    view(), src, dst, weights ? w : nullptr, weights ? st : nullptr, n, ep);
       }
       MK_HIP(hipGetLastError());
       Totals h;
@@ -861,9 +880,9 @@ struct Traversal {
     if (n) {
       const GraphView G = g->view();
       {
-        ProfScope prof(stream, "k_invalidate_batch", 24.0 * n);
         const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
-        k_invalidate_batch<<<grid, kItemBlock, 0, stream>>>(G, view(), src, dst, n, source);
+        prof_launch("k_invalidate_batch", 24.0 * n, k_invalidate_batch, dim3(grid),
+                    dim3(kItemBlock), 0u, stream, G, view(), src, dst, n, source);
       }
       MK_HIP(hipGetLastError());
       // propagate invalidation down the tree; any step ran = something was
@@ -879,31 +898,20 @@ struct Traversal {
         if (g->rev) {
           // pull from in-neighbours of the invalidated vertices
           const GraphView R = g->rev->view();
-          const unsigned grid = blocks_for_waves(std::min<uint64_t>(
-              (uint64_t(g->n) / 32 + 64) / 64, uint64_t(max_waves)));
+          const uint32_t nwords = uint32_t((uint64_t(g->n) + 31) / 32);
+          fill_u32(&cnt->ninv, 0u, 1, stream);
+          k_inv_compact<<<blocks_for_threads(nwords), kBlock, 0, stream>>>(view(), nwords);
+          const unsigned grid = blocks_for_waves(uint64_t(max_waves));
           const unsigned hgrid = blocks_for_waves(uint64_t(max_waves));
-          {
-            ProfScope prof(stream, "k_reseed_pull", 0.0);
-            if (g->weighted) {
-              if (weights)
-                k_reseed_pull<true, true><<<grid, kBlock, 0, stream>>>(G, R, view(), ep);
-              else
-                k_reseed_pull<true, false><<<grid, kBlock, 0, stream>>>(G, R, view(), ep);
-            } else {
-              k_reseed_pull<false, false><<<grid, kBlock, 0, stream>>>(G, R, view(), ep);
-            }
-          }
-          {
-            ProfScope prof(stream, "k_reseed_pull", 0.0);  // hub half of the pull
-            if (g->weighted) {
-              if (weights)
-                k_pull_hubs<true, true><<<hgrid, kBlock, 0, stream>>>(G, R, view(), ep);
-              else
-                k_pull_hubs<true, false><<<hgrid, kBlock, 0, stream>>>(G, R, view(), ep);
-            } else {
-              k_pull_hubs<false, false><<<hgrid, kBlock, 0, stream>>>(G, R, view(), ep);
-            }
-          }
+          auto kp = g->weighted ? (weights ? k_reseed_pull<true, true> : k_reseed_pull<true, false>)
+                                : k_reseed_pull<false, false>;
+          auto kh = g->weighted ? (weights ? k_pull_hubs<true, true> : k_pull_hubs<true, false>)
+                                : k_pull_hubs<false, false>;
+          prof_launch("k_reseed_pull", 0.0, kp, dim3(grid), dim3(kBlock), 0u, stream, G, R, view(),
+                      ep);
+          // hub half of the pull
+          prof_launch("k_pull_hubs", 0.0, kh, dim3(hgrid), dim3(kBlock), 0u, stream, G, R, view(),
+                      ep);
           // the hub list used slot 1, which the first relax step appends to
           fill_u32(cnt->small + 1, 0u, 1, stream);
           fill_u64(reinterpret_cast<uint64_t *>(cnt->large + 1), 0ull, 1, stream);

@@ -113,6 +113,74 @@ def pmc_per_kernel(path, counter, region=None):
     return {k: (n, s / n) for k, (n, s) in acc.items()}
 
 
+# Names the library's HIP-event profiler uses for the kernels a rocprofv3
+# trace names by their full signature (the bench reports the former).
+SCOPE_OF = [
+    ("k_expand<true, true, 0>", "k_expand_relax"),
+    ("k_expand<true, true, 1>", "k_expand_invalidate"),
+    ("k_insert<", "k_insert"),
+    ("k_delete<", "k_delete"),
+    ("k_seed_batch<", "k_seed_batch"),
+    ("k_invalidate_batch(", "k_invalidate_batch"),
+    ("k_reseed_pull<", "k_reseed_pull"),
+    ("k_pull_hubs<", "k_pull_hubs"),
+]
+
+
+def scope_name(kernel):
+    for pat, name in SCOPE_OF:
+        if pat in kernel:
+            return name
+    return None
+
+
+def traffic(fetch_csv, write_csv, region=0, calib_region=1, calib_bytes=1 << 30):
+    """HBM bytes per dispatch of each kernel in marker region `region`, from
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) collected in two runs, each
+    scaled by the factor that makes the calibration copy in `calib_region`
+    (reading and writing `calib_bytes`) read exactly.  Returns ({kernel:
+    (fetch_bytes, write_bytes, dispatches)}, fetch_factor, write_factor)."""
+    f = pmc_per_kernel(fetch_csv, "FETCH_SIZE", region)
+    w = pmc_per_kernel(write_csv, "WRITE_SIZE", region)
+    fc = list(pmc_per_kernel(fetch_csv, "FETCH_SIZE", calib_region).values())
+    wc = list(pmc_per_kernel(write_csv, "WRITE_SIZE", calib_region).values())
+    if len(fc) != 1 or len(wc) != 1:
+        raise ValueError("the calibration region must hold exactly one kernel")
+    ff = calib_bytes / (fc[0][1] * 1024.0)
+    wf = calib_bytes / (wc[0][1] * 1024.0)
+    out = {}
+    for k in set(f) | set(w):
+        fb = f.get(k, (0, 0.0))[1] * 1024.0 * ff
+        wb = w.get(k, (0, 0.0))[1] * 1024.0 * wf
+        out[k] = (fb, wb, max(f.get(k, (0, 0))[0], w.get(k, (0, 0))[0]))
+    return out, ff, wf
+
+
+def report(trace_csv, fetch_csv, write_csv, commands, top=12):
+    """Text summary of the timed region (marker pair 0): kernel times from the
+    kernel trace and HBM traffic per dispatch from the counter passes."""
+    lines = ["# rocprofv3 summary: timed region of bench.py (between marker pair 0)", ""]
+    lines += ["Commands:"] + [f"  {c}" for c in commands] + [""]
+    lines.append("Kernel trace, timed region:")
+    lines.append(f"{'calls':>6} {'total_ms':>10} {'avg_us':>9} {'min_us':>8} {'max_us':>9}  kernel")
+    for name, n, tot, avg, mn, mx in trace_stats(trace_csv, 0)[:top]:
+        lines.append(f"{n:6d} {tot:10.3f} {avg:9.2f} {mn:8.2f} {mx:9.2f}  {name[:110]}")
+    tr, ff, wf = traffic(fetch_csv, write_csv)
+    lines += ["", f"Counter scale from the 1 GiB calibration copy (marker pair 1): "
+              f"FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {wf:.3f}", "",
+              "HBM traffic per dispatch, timed region (scaled):",
+              f"{'dispatches':>10} {'fetch_MB':>10} {'write_MB':>10} {'total_MB':>10}  kernel"]
+    machine = []
+    for k, (fb, wb, n) in sorted(tr.items(), key=lambda kv: -(kv[1][0] + kv[1][1]) * kv[1][2]):
+        if fb + wb < 1e5:
+            continue
+        lines.append(f"{n:10d} {fb / 1e6:10.3f} {wb / 1e6:10.3f} {(fb + wb) / 1e6:10.3f}  {k[:100]}")
+        sc = scope_name(k)
+        if sc:
+            machine.append(f"{sc} {int(round(fb + wb))}")
+    return "\n".join(lines) + "\n", "\n".join(machine) + "\n"
+
+
 def _main(argv):
     import argparse
 
@@ -127,6 +195,13 @@ def _main(argv):
     t.add_argument("--region", type=int, default=None,
                    help="only dispatches between marker pair REGION (0 = first)")
     t.add_argument("--top", type=int, default=20)
+    rp = sub.add_parser("report", help="timed-region summary and traffic table")
+    rp.add_argument("--trace", required=True)
+    rp.add_argument("--fetch", required=True)
+    rp.add_argument("--write", required=True)
+    rp.add_argument("--command", action="append", default=[])
+    rp.add_argument("--out", required=True, help="summary text file")
+    rp.add_argument("--traffic-out", required=True, help="'<kernel> <bytes>' lines")
     b = sub.add_parser("pmc", help="mean per-dispatch counter value per kernel")
     b.add_argument("csv")
     b.add_argument("--counter", required=True)
@@ -137,6 +212,13 @@ def _main(argv):
               f"{'%':>6}  kernel")
         for name, calls, tot, avg, mn, mx, pct in kernel_stats(args.csv, args.top):
             print(f"{calls:6d} {tot:10.3f} {avg:10.2f} {mn:9.2f} {mx:10.2f} {pct:6.2f}  {name}")
+    elif args.cmd == "report":
+        text, machine = report(args.trace, args.fetch, args.write, args.command)
+        with open(args.out, "w") as f:
+            f.write(text)
+        with open(args.traffic_out, "w") as f:
+            f.write(machine)
+        print(text)
     elif args.cmd == "trace":
         print(f"{'calls':>6} {'total_ms':>10} {'avg_us':>10} {'min_us':>9} {'max_us':>10}  kernel")
         for name, calls, tot, avg, mn, mx in trace_stats(args.csv, args.region)[:args.top]:
