@@ -127,6 +127,70 @@ MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t si
   q.pushed += uint32_t(__popcll(pm));
 }
 
+// Push targets of the expansion helpers below (wave-collective calls).
+// GlobalSink appends to a step's global queues through the wave's LDS
+// staging (enqueue); TailSink appends to a block's LDS queue (tail mode of
+// k_expand), spilling to the global queues of the next step when the LDS
+// queue is full or the vertex is a hub.
+struct GlobalSink {
+  const TravView &S;
+  WaveQ &q;
+  uint32_t side, slot;
+  MK_DEV void operator()(const GraphView &G, bool push, uint32_t v) {
+    enqueue(G, S, q, side, slot, push, v);
+  }
+};
+
+constexpr uint32_t kTailCap = 2048;    // LDS queue entries per tail-mode block
+constexpr uint32_t kTailEnter = 512;   // frontiers up to this size run in tail mode
+constexpr uint32_t kTailStay = 768;    // a tail frontier above this is handed back
+constexpr uint32_t kEpochStride = 256; // epochs reserved per launch (tail steps)
+
+struct TailSink {
+  const TravView &S;
+  uint32_t *lq;     // LDS queue, kTailCap entries
+  uint32_t *ctl;    // LDS: [0] entries claimed, [1] handover flag
+  uint32_t side, slot;  // global queues of the next step
+  uint32_t pushed;      // wave-uniform statistics
+  MK_DEV void operator()(const GraphView &G, bool push, uint32_t v) {
+    const uint64_t pm = ballot(push);
+    if (pm == 0) return;
+    const uint32_t lane = lane_id();
+    uint32_t nb = 0;
+    if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
+    const bool small = push && nb <= kSmallBuckets;
+    const uint64_t sm = ballot(small);
+    const uint32_t c = uint32_t(__popcll(sm));
+    if (c) {
+      const uint32_t leader = ffs64(sm);
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(ctl, c);
+      base = read_lane(base, leader);
+      if (base + c <= kTailCap) {
+        if (small) lq[base + lanes_below(sm)] = v;
+      } else {
+        // LDS queue full: these go to the global queue (positions claimed
+        // in LDS are marked empty) and the block hands the frontier back
+        uint32_t gb = 0;
+        if (lane == leader) gb = atomicAdd(&S.cnt->small[slot], c);
+        gb = read_lane(gb, leader);
+        if (small) {
+          const uint32_t pos = base + lanes_below(sm);
+          if (pos < kTailCap) lq[pos] = kEmpty;
+          S.q_small[side][gb + lanes_below(sm)] = v;
+        }
+        if (lane == leader) ctl[1] = 1u;
+      }
+    }
+    if (push && !small) {  // hubs go to the next step's bucket work list
+      const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
+      S.q_large[side][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+      ctl[1] = 1u;
+    }
+    pushed += uint32_t(__popcll(pm));
+  }
+};
+
 // Block-aggregated append for kernels with one item per thread: one global
 // atomicAdd per block.  Called by every thread of the block.
 template <uint32_t NW>
@@ -162,9 +226,9 @@ MK_DEV bool inv_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> 
 
 // Per-slab work of the three passes.  `v` is this lane's word of one slab of
 // vertex u (distance du).  Wave-collective; returns live keys seen.
-template <bool MAP, bool WEIGHTS, int MODE>
-MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
-                          uint32_t slot, uint32_t epoch, uint32_t u, uint32_t du, uint32_t v) {
+template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
+MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
+                          uint32_t u, uint32_t du, uint32_t v) {
   const uint32_t lane = lane_id();
   const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
   const bool live = key_lane && v < kTomb;
@@ -201,18 +265,17 @@ MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, WaveQ &q, uint3
       }
     }
   }
-  enqueue(G, S, q, side, slot, push, v);
+  sink(G, push, v);
   return uint32_t(__popcll(ballot(live)));
 }
 
 // Work of a whole chain given its first slab's word `v` (already loaded).
-template <bool MAP, bool WEIGHTS, int MODE>
-MK_DEV void chain_work(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
-                       uint32_t slot, uint32_t epoch, uint32_t u, uint32_t du, uint32_t v,
-                       uint32_t &slabs, uint32_t &edges) {
+template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
+MK_DEV void chain_work(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
+                       uint32_t u, uint32_t du, uint32_t v, uint32_t &slabs, uint32_t &edges) {
   for (;;) {
     ++slabs;
-    edges += slab_work<MAP, WEIGHTS, MODE>(G, S, q, side, slot, epoch, u, du, v);
+    edges += slab_work<MAP, WEIGHTS, MODE>(G, S, sink, epoch, u, du, v);
     const uint32_t nx = read_lane(v, kNextLane);
     if (nx == kEmpty) break;
     v = slab_ptr(G.pool, nx)[lane_id()];
@@ -221,10 +284,10 @@ MK_DEV void chain_work(const GraphView &G, const TravView &S, WaveQ &q, uint32_t
 
 // Process the buckets named by lanes of `m`: lane j owns bucket bk (its
 // vertex uj with distance dj).  Slabs are loaded kBatch at a time.
-template <bool MAP, bool WEIGHTS, int MODE>
-MK_DEV void batched_buckets(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
-                            uint32_t slot, uint32_t epoch, uint64_t m, uint32_t bk, uint32_t uj,
-                            uint32_t dj, uint32_t &slabs, uint32_t &edges) {
+template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
+MK_DEV void batched_buckets(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
+                            uint64_t m, uint32_t bk, uint32_t uj, uint32_t dj, uint32_t &slabs,
+                            uint32_t &edges) {
   const uint32_t lane = lane_id();
   while (m) {
     uint32_t js[kBatch];
@@ -244,13 +307,101 @@ MK_DEV void batched_buckets(const GraphView &G, const TravView &S, WaveQ &q, uin
 #pragma unroll
     for (uint32_t k = 0; k < kBatch; ++k)
       if (k < c)
-        chain_work<MAP, WEIGHTS, MODE>(G, S, q, side, slot, epoch, read_lane(uj, js[k]),
+        chain_work<MAP, WEIGHTS, MODE>(G, S, sink, epoch, read_lane(uj, js[k]),
                                        read_lane(dj, js[k]), v[k], slabs, edges);
   }
 }
 
+// Expand frontier vertices queue[0..n) (non-hub entries), waves wave,
+// wave + NW, ... of the caller taking up to 64 each.
+template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
+MK_DEV void expand_rows(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
+                        const uint32_t *queue, uint32_t n, uint32_t wave, uint32_t NW,
+                        uint32_t &slabs, uint32_t &edges, uint32_t &visited) {
+  const uint32_t lane = lane_id();
+  const uint32_t per = per_wave(n, NW);
+  for (uint32_t i0 = wave * per; i0 < n; i0 += NW * per) {
+    const uint32_t i = i0 + lane;
+    bool act = lane < per && i < n;
+    uint32_t u = 0, b0 = 0, nb = 0, du = 0;
+    if (act) {
+      u = queue[i];
+      b0 = G.bucket_off[u];
+      nb = G.bucket_off[u + 1] - b0;
+      du = MODE == 1 ? 0u : dist_of(S.dist[u]);
+      if (MODE == 0 && du == kInf) act = false;
+    }
+    visited += uint32_t(__popcll(ballot(act)));
+    batched_buckets<MAP, WEIGHTS, MODE>(G, S, sink, epoch, ballot(act), b0, u, du, slabs, edges);
+    // further buckets (at most kSmallBuckets - 1 for queued vertices)
+    for (uint32_t k = 1; ballot(act && nb > k); ++k)
+      batched_buckets<MAP, WEIGHTS, MODE>(G, S, sink, epoch, ballot(act && nb > k), b0 + k, u,
+                                          du, slabs, edges);
+  }
+}
+
+// Tail mode of a frontier step: block 0 alone runs this step and the
+// following ones, frontiers ping-ponging between two LDS queues, while they
+// stay small; step j uses epoch + j.  The frontier is handed back (written
+// to the global queues of the next step, where the next launch reads it)
+// when it grows past kTailStay, overflows the LDS queue, reaches a hub, or
+// the epoch range reserved for this launch runs out.
+template <bool MAP, bool WEIGHTS, int MODE>
+MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                        uint32_t epoch, uint32_t n0, uint32_t *lds) {
+  uint32_t *qa = lds, *qb = lds + kTailCap, *ctl = lds + 2 * kTailCap;  // ctl: 2 x {count, flag}
+  const uint32_t nside = side ^ 1u, nslot = slot == 2 ? 0 : slot + 1;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, NW = blockDim.x >> 6;
+  if (tid < 4) ctl[tid] = 0u;
+  for (uint32_t i = tid; i < n0; i += blockDim.x) qa[i] = S.q_small[side][i];
+  uint32_t nin = n0, slabs = 0, edges = 0, visited = 0, pushed = 0;
+  for (uint32_t j = 0;; ++j) {
+    __syncthreads();  // the previous step is complete everywhere; qa holds its frontier
+    uint32_t *c = ctl + 2 * (j & 1u);
+    if (tid == 0) {  // the other parity is read next step; nobody reads it now
+      ctl[2 * ((j + 1) & 1u)] = 0u;
+      ctl[2 * ((j + 1) & 1u) + 1] = 0u;
+    }
+    TailSink sink{S, qb, c, nside, nslot, 0u};
+    expand_rows<MAP, WEIGHTS, MODE>(G, S, sink, epoch + j, qa, nin, wave, NW, slabs, edges,
+                                    visited);
+    pushed += sink.pushed;
+    __syncthreads();
+    const uint32_t nb = c[0], handover = c[1];
+    if (nb == 0 && handover == 0) break;  // the traversal is done
+    if (handover || nb > kTailStay || j + 2 >= kEpochStride) {
+      // append the LDS part of the frontier to the next step's global queue
+      const uint32_t m = min(nb, kTailCap);
+      for (uint32_t i0 = wave * 64; i0 < m; i0 += NW * 64) {
+        const uint32_t i = i0 + lane;
+        const uint32_t v = i < m ? qb[i] : kEmpty;
+        const uint64_t vm = ballot(v != kEmpty);
+        if (vm == 0) continue;
+        const uint32_t leader = ffs64(vm);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&S.cnt->small[nslot], uint32_t(__popcll(vm)));
+        base = read_lane(base, leader);
+        if (v != kEmpty) S.q_small[nside][base + lanes_below(vm)] = v;
+      }
+      break;
+    }
+    if (tid == 0) atomicAdd(&S.cnt->steps, 1u);  // one more step runs here
+    uint32_t *t = qa;
+    qa = qb;
+    qb = t;
+    nin = nb;
+  }
+  if (lane == 0) {
+    add_stat(S.cnt, kStatSlabs, slabs);
+    add_stat(S.cnt, kStatEdges, edges);
+    add_stat(S.cnt, kStatVisited, visited);
+    add_stat(S.cnt, kStatPushed, pushed);
+  }
+}
+
 // One frontier step: read queue side `side`, counter slot `slot`; append to
-// side^1 / slot+1; clear slot+2.  MODE 0 relaxes, MODE 1 invalidates.
+// side^1 / slot+1; clear slot+2.  MODE 0 relaxes, MODE 1 invalidates.  A
+// small frontier without hub entries is run in tail mode by block 0.
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_expand(GraphView G, TravView S, uint32_t side, uint32_t slot, uint32_t epoch) {
@@ -269,8 +420,15 @@ __global__ void __launch_bounds__(kBlock)
     if (n_small + n_large) atomicAdd(&S.cnt->steps, 1u);
   }
   if (n_small + n_large == 0) return;
-  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
-  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  constexpr uint32_t kLdsWords =
+      2 * kTailCap + 4 > (kBlock / 64) * kWaveQ ? 2 * kTailCap + 4 : (kBlock / 64) * kWaveQ;
+  __shared__ uint32_t lds[kLdsWords];
+  if (n_large == 0 && n_small <= kTailEnter) {
+    if (blockIdx.x == 0) expand_tail<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, n_small, lds);
+    return;
+  }
+  WaveQ q{lds + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
+  GlobalSink sink{S, q, nside, nslot};
   uint32_t slabs = 0, edges = 0, visited = 0;
   // hub vertices: bucket work items spread over all waves
   const uint2 *ql = S.q_large[side];
@@ -295,31 +453,11 @@ __global__ void __launch_bounds__(kBlock)
       du = MODE == 1 ? 0u : dist_of(S.dist[u]);
       if (MODE == 0 && du == kInf) act = false;
     }
-    batched_buckets<MAP, WEIGHTS, MODE>(G, S, q, nside, nslot, epoch, ballot(act), b, u, du,
-                                        slabs, edges);
+    batched_buckets<MAP, WEIGHTS, MODE>(G, S, sink, epoch, ballot(act), b, u, du, slabs, edges);
   }
   // small vertices: up to 64 per wave step, spread over all waves when few
-  const uint32_t *qs = S.q_small[side];
-  const uint32_t per = per_wave(n_small, W);
-  for (uint32_t i0 = gw * per; i0 < n_small; i0 += W * per) {
-    const uint32_t i = i0 + lane;
-    bool act = lane < per && i < n_small;
-    uint32_t u = 0, b0 = 0, nb = 0, du = 0;
-    if (act) {
-      u = qs[i];
-      b0 = G.bucket_off[u];
-      nb = G.bucket_off[u + 1] - b0;
-      du = MODE == 1 ? 0u : dist_of(S.dist[u]);
-      if (MODE == 0 && du == kInf) act = false;
-    }
-    visited += uint32_t(__popcll(ballot(act)));
-    batched_buckets<MAP, WEIGHTS, MODE>(G, S, q, nside, nslot, epoch, ballot(act), b0, u, du,
-                                        slabs, edges);
-    // further buckets (at most kSmallBuckets - 1 for queued vertices)
-    for (uint32_t k = 1; ballot(act && nb > k); ++k)
-      batched_buckets<MAP, WEIGHTS, MODE>(G, S, q, nside, nslot, epoch, ballot(act && nb > k),
-                                          b0 + k, u, du, slabs, edges);
-  }
+  expand_rows<MAP, WEIGHTS, MODE>(G, S, sink, epoch, S.q_small[side], n_small, gw, W, slabs, edges,
+                                  visited);
   wq_flush(q, S.q_small[nside], &S.cnt->small[nslot]);
   if (lane == 0) {
     add_stat(S.cnt, kStatSlabs, slabs);
@@ -353,8 +491,9 @@ __global__ void __launch_bounds__(kBlock)
         act = du != kInf && !inv_bit(S.invbits, u);
       }
     }
-    batched_buckets<MAP, WEIGHTS, 2>(G, S, q, 0, 0, epoch, ballot(act), uint32_t(b), u, du,
-                                     slabs, edges);
+    GlobalSink sink{S, q, 0u, 0u};
+    batched_buckets<MAP, WEIGHTS, 2>(G, S, sink, epoch, ballot(act), uint32_t(b), u, du, slabs,
+                                     edges);
   }
   wq_flush(q, S.q_small[0], &S.cnt->small[0]);
   if (lane == 0) {
@@ -730,11 +869,15 @@ struct Traversal {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // Epoch base of the next launch: kEpochStride values are reserved per
+  // launch (a tail-mode launch uses one per inner step).  Stamps start at 0,
+  // so epochs are never 0; on wrap-around the stamps are cleared.
   uint32_t next_epoch() {
-    if (++epoch == 0xFFFFFFFFu) {  // wrap: clear stamps once per 4G steps
+    if (epoch > 0xFFFFFFFFu - 2 * kEpochStride) {
       fill_u32(stamp, 0u, g->n, stream);
-      epoch = 1;
+      epoch = 0;
     }
+    epoch += kEpochStride;
     return epoch;
   }
 

@@ -1,0 +1,113 @@
+"""GPU parity of frontier steps run in tail mode (a small frontier expanded
+by one block, step after step, frontiers in LDS) and of every way the tail
+hands the frontier back to full-grid steps: the frontier grows past the
+LDS queue (spill), reaches a hub vertex, or the epochs reserved per launch
+run out (a long path).  Distances bit-exact against the oracle, parents
+valid."""
+import numpy as np
+import pytest
+
+import meerkat_amd as mk
+import oracle
+from meerkat_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+
+def check(algo, n, s, d, w=None, source=0):
+    ref = oracle.csr(n, s, d, w)
+    dist, par = algo.read()
+    exp = oracle.dijkstra(ref, source) if w is not None else oracle.bfs(ref, source)
+    assert np.array_equal(dist, exp), int((dist != exp).sum())
+    assert oracle.check_parents_fast(ref, dist, par, source, weights=w is not None) == 0
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_long_path_static_decremental_incremental(gpu, weighted):
+    # 1000 levels: every step's frontier is one vertex, so the tail runs
+    # ~254 steps per launch and hands over when its epoch range runs out
+    n = 1000
+    s = np.arange(n - 1, dtype=np.uint32)
+    d = s + 1
+    w = (np.arange(n - 1, dtype=np.uint32) % 7 + 1) if weighted else None
+    pool = mk.SlabPool(1 << 22)
+    g = mk.DynamicGraph(n, pool, 0.7, None, weighted=weighted)
+    g.InsertEdges(s, d, w)
+    algo = mk.SSSP(g, 0) if weighted else mk.BFS(g, 0)
+    algo.Static()
+    check(algo, n, s, d, w)
+    assert algo.last_stats()["iterations"] >= n - 1
+    # cut the path at 500: the tail invalidates 499 vertices one level a step
+    cs, cd = np.array([500], np.uint32), np.array([501], np.uint32)
+    g.DeleteEdges(cs, cd)
+    algo.Decremental(cs, cd) if not weighted else algo.Decremental(cs, cd, None)
+    keep = ~((s == 500) & (d == 501))
+    s1, d1 = s[keep], d[keep]
+    w1 = w[keep] if weighted else None
+    check(algo, n, s1, d1, w1)
+    # reconnect through a detour: 499 -> 700 and 10 -> 520
+    bs, bd = np.array([499, 10], np.uint32), np.array([700, 520], np.uint32)
+    bw = np.array([3, 5], np.uint32) if weighted else None
+    g.InsertEdges(bs, bd, bw)
+    if weighted:
+        algo.Incremental(bs, bd, bw)
+    else:
+        algo.Incremental(bs, bd)
+    s2, d2 = np.r_[s1, bs], np.r_[d1, bd]
+    w2 = np.r_[w1, bw] if weighted else None
+    check(algo, n, s2, d2, w2)
+
+
+@pytest.mark.parametrize("hints", [False, True])
+def test_broom_spill_and_hub_handover(gpu, hints):
+    # 0 -> 1 -> {2 .. 3001}, each leaf -> one of 50 tail vertices: the
+    # frontier {1} (tail mode) pushes 3000 leaves, more than the LDS queue
+    # holds (spill); with degree hints vertex 1 is a hub (hand-over)
+    k = 3000
+    leaves = np.arange(2, 2 + k, dtype=np.uint32)
+    tails = (2 + k + leaves % 50).astype(np.uint32)
+    s = np.r_[np.uint32(0), np.full(k, 1, np.uint32), leaves]
+    d = np.r_[np.uint32(1), leaves, tails]
+    n = 2 + k + 50
+    h = gen.degree_hints(n, s) if hints else None
+    pool = mk.SlabPool(1 << 24)
+    g = mk.DynamicGraph(n, pool, 0.7, h)
+    g.InsertEdges(s, d)
+    b = mk.BFS(g, 0)
+    b.Static()
+    check(b, n, s, d)
+    # delete the hand edge and put it back: decremental then incremental
+    es, ed = np.array([0], np.uint32), np.array([1], np.uint32)
+    g.DeleteEdges(es, ed)
+    b.Decremental(es, ed)
+    keep = ~((s == 0) & (d == 1))
+    check(b, n, s[keep], d[keep])
+    g.InsertEdges(es, ed)
+    b.Incremental(es, ed)
+    check(b, n, s, d)
+
+
+def test_small_batches_on_rmat_stay_exact(gpu):
+    # small update batches give small frontiers (tail mode) on a skewed graph
+    n, s, d, w = gen.rmat(14, 8, seed=41, weighted=True)
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=True,
+                        in_degree_hints=gen.degree_hints(n, d))
+    g.InsertEdges(s, d, w)
+    sp = mk.SSSP(g, 0)
+    sp.Static()
+    keys, wt = gen.keys_of(s, d), w.copy()
+    for i in range(6):
+        ds, dd = gen.delete_batch(keys, 20, seed=300 + i)
+        g.DeleteEdges(ds, dd)
+        sp.Decremental(ds, dd, None)
+        keep = ~np.isin(keys, gen.keys_of(ds, dd))
+        keys, wt = keys[keep], wt[keep]
+        bs, bd, bw = gen.insert_batch(keys, n, 20, seed=400 + i, weighted=True, wseed=500 + i)
+        g.InsertEdges(bs, bd, bw)
+        sp.Incremental(bs, bd, bw)
+        keys, wt = np.r_[keys, gen.keys_of(bs, bd)], np.r_[wt, bw]
+        order = np.argsort(keys)
+        keys, wt = keys[order], wt[order]
+        s2, d2 = gen.split_keys(keys)
+        check(sp, n, s2, d2, wt)
