@@ -153,13 +153,15 @@ struct MutSide {
 // per warp, include/Graph/warp/InsertEdge.cuh:66-103).
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
-    k_insert(MutSide A, MutSide B, uint32_t waves_a, PoolView P, const uint32_t *src,
-             const uint32_t *dst, const uint32_t *wt, uint32_t n, uint32_t epw,
-             const uint32_t *bad) {
+    k_insert(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b, PoolView P,
+             const uint32_t *src, const uint32_t *dst, const uint32_t *wt, uint32_t n,
+             uint32_t epw, const uint32_t *bad) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const bool second = wave_all >= waves_a;
+  // waves past both sides (the last block's spare waves) have nothing to do
+  if (second && wave_all - waves_a >= waves_b) return;
   const MutSide &M = second ? B : A;
   const uint32_t wave = second ? wave_all - waves_a : wave_all;
   const uint64_t base = uint64_t(wave) * epw;
@@ -219,12 +221,15 @@ __global__ void __launch_bounds__(kBlock)
 // and groups as in k_insert.
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
-    k_delete(MutSide A, MutSide B, uint32_t waves_a, const uint32_t *src, const uint32_t *dst,
-             uint32_t n, uint32_t epw, uint32_t only_status, const uint32_t *bad) {
+    k_delete(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b, const uint32_t *src,
+             const uint32_t *dst, uint32_t n, uint32_t epw, uint32_t only_status,
+             const uint32_t *bad) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const bool second = wave_all >= waves_a;
+  // waves past both sides (the last block's spare waves) have nothing to do
+  if (second && wave_all - waves_a >= waves_b) return;
   const MutSide &M = second ? B : A;
   const uint32_t wave = second ? wave_all - waves_a : wave_all;
   const uint64_t base = uint64_t(wave) * epw;
@@ -653,7 +658,8 @@ static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
                        (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1);
   prof_launch(name, bytes, g->weighted ? k_insert<true> : k_insert<false>,
               dim3(blocks_for_waves(total)), dim3(kBlock), 0u, s, a, b, uint32_t(waves),
-              g->pool->view, src, dst, g->weighted ? w : nullptr, n, epw, bad);
+              uint32_t(total - waves), g->pool->view, src, dst, g->weighted ? w : nullptr, n, epw,
+              bad);
 }
 
 // Launch the delete kernel over g (and its mirror); only_status != 0xFF
@@ -669,7 +675,8 @@ static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   // atomic, status byte
   prof_launch(name, double(n) * (g->rev ? 2 : 1) * (8 + 256 + 4 + 8 + 1),
               g->weighted ? k_delete<true> : k_delete<false>, dim3(blocks_for_waves(total)),
-              dim3(kBlock), 0u, s, a, b, uint32_t(waves), src, dst, n, epw, only_status, bad);
+              dim3(kBlock), 0u, s, a, b, uint32_t(waves), uint32_t(total - waves), src, dst, n,
+              epw, only_status, bad);
 }
 
 // Status buffer and counters of the in-edge mirror for a batch of n edges.

@@ -219,3 +219,32 @@ def test_mirror_rolls_back_with_the_graph(gpu):
     after_f, after_r = g.export_csr(), g.mirror().export_csr()
     assert np.array_equal(before_f[1], after_f[1]) and np.array_equal(before_r[1], after_r[1])
     assert np.array_equal(before_r[0], after_r[0])
+
+
+@pytest.mark.parametrize("mirror", [False, True])
+@pytest.mark.parametrize("count", [1, 100, 4097, 300_001])
+def test_status_of_every_edge_when_the_grid_has_spare_waves(gpu, mirror, count):
+    # batch sizes whose wave count leaves spare waves in the last block: every
+    # new edge must report APPLIED (insert) and APPLIED again (delete), and
+    # no edge may be applied twice
+    n = 50_000
+    rng = np.random.default_rng(count)
+    keys = np.unique(rng.integers(0, n, 2 * count + 10, dtype=np.uint64) * np.uint64(n)
+                     + rng.integers(0, n, 2 * count + 10, dtype=np.uint64))
+    s = (keys // np.uint64(n)).astype(np.uint32)
+    d = (keys % np.uint64(n)).astype(np.uint32)
+    keep = s != d
+    s, d = s[keep][:count], d[keep][:count]
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, None, reverse=mirror)
+    sd, dd = mk.device_u32(s), mk.device_u32(d)
+    g.InsertEdges(sd, dd)
+    st = g.last_status()
+    assert len(st) == len(s) and (st == _lib.EDGE_APPLIED).all(), np.bincount(st)
+    assert g.num_edges() == len(s)
+    if mirror:
+        assert g.mirror().num_edges() == len(s)
+    g.DeleteEdges(sd, dd)
+    st = g.last_status()
+    assert (st == _lib.EDGE_APPLIED).all(), np.bincount(st)
+    assert g.num_edges() == 0
