@@ -621,28 +621,49 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Compact the invalidation bitmap into S.inv_list (count in S.cnt->ninv):
-// one bitmap word per thread, a wave prefix sum of the words' popcounts and
-// one atomicAdd per wave.
+// Compact the invalidation bitmap into S.inv_list (count in S.cnt->ninv).
+// Each thread scans kInvWords consecutive bitmap words; a block-wide prefix
+// sum of their popcounts (wave scans plus one LDS pass) places the entries
+// with a single atomicAdd per block.
+constexpr uint32_t kInvWords = 8;
+
 __global__ void __launch_bounds__(kBlock) k_inv_compact(TravView S, uint32_t nwords) {
-  const uint32_t lane = lane_id();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t word = i < nwords ? S.invbits[i] : 0u;
-  const uint32_t c = uint32_t(__popc(word));
+  __shared__ uint32_t wsum[kBlock / 64 + 1];
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  const uint64_t first = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * kInvWords;
+  uint32_t words[kInvWords];
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kInvWords; ++k) {
+    words[k] = first + k < nwords ? S.invbits[first + k] : 0u;
+    c += uint32_t(__popc(words[k]));
+  }
   uint32_t x = c;  // inclusive prefix sum over the wave
 #pragma unroll
   for (uint32_t off = 1; off < 64; off <<= 1) {
     const uint32_t y = __shfl_up(x, off, 64);
     if (lane >= off) x += y;
   }
-  const uint32_t total = read_lane(x, 63);
-  if (total == 0) return;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(&S.cnt->ninv, total);
-  uint32_t pos = read_lane(base, 0) + x - c;
-  while (word) {
-    S.inv_list[pos++] = i * 32u + uint32_t(__ffs(word) - 1);
-    word &= word - 1;
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+      const uint32_t v = wsum[k];
+      wsum[k] = t;
+      t += v;
+    }
+    wsum[kBlock / 64] = t ? atomicAdd(&S.cnt->ninv, t) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = wsum[kBlock / 64] + wsum[w] + x - c;
+#pragma unroll
+  for (uint32_t k = 0; k < kInvWords; ++k) {
+    uint32_t word = words[k];
+    while (word) {
+      S.inv_list[pos++] = uint32_t(first + k) * 32u + uint32_t(__ffs(word) - 1);
+      word &= word - 1;
+    }
   }
 }
 
@@ -802,6 +823,8 @@ struct Traversal {
   uint2 *ql[2] = {nullptr, nullptr};
   Counts *cnt = nullptr;  // device
   HostBuf host;           // pinned Counts
+  HostBuf snap[2];        // pinned Counts snapshots of successive step groups
+  hipEvent_t snap_ev[2] = {nullptr, nullptr};
   uint32_t epoch = 0;
   uint32_t last_iters = 0;
   uint64_t last_slabs = 0;
@@ -841,6 +864,10 @@ struct Traversal {
     }
     MK_HIP(hipMalloc(&cnt, sizeof(Counts)));
     if (int r = host.ensure(sizeof(Counts))) return r;
+    for (int i = 0; i < 2; ++i) {
+      if (int r = snap[i].ensure(sizeof(Counts))) return r;
+      MK_HIP(hipEventCreateWithFlags(&snap_ev[i], hipEventDisableTiming));
+    }
     fill_u64(dist, kInfInf, n, stream);
     fill_u32(stamp, 0u, n, stream);
     fill_u32(invbits, 0u, (n + 31) / 32, stream);
@@ -864,6 +891,10 @@ struct Traversal {
     }
     (void)hipFree(cnt);
     host.release();
+    for (int i = 0; i < 2; ++i) {
+      snap[i].release();
+      if (snap_ev[i]) (void)hipEventDestroy(snap_ev[i]);
+    }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -894,7 +925,11 @@ struct Traversal {
   int read_counts(Totals &h) {
     MK_HIP(hipMemcpyAsync(host.ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
     MK_HIP(hipStreamSynchronize(stream));
-    const Counts &c = *host.as<Counts>();
+    parse(*host.as<Counts>(), h);
+    return MK_OK;
+  }
+
+  static void parse(const Counts &c, Totals &h) {
     for (int i = 0; i < 3; ++i) {
       h.small[i] = c.small[i];
       h.large[i] = c.large[i];
@@ -907,7 +942,6 @@ struct Traversal {
       h.visited += c.stats[k][kStatVisited];
       h.pushed += c.stats[k][kStatPushed];
     }
-    return MK_OK;
   }
 
   template <int MODE> void launch_expand(uint32_t side, uint32_t slot, uint32_t ep) {
@@ -921,43 +955,54 @@ struct Traversal {
   }
 
   // Run frontier steps from queue side 0 / slot 0 until the frontier is
-  // empty.  Steps are launched in groups without waiting; the counters are
-  // read after each group.  Algorithmic bytes (profiling) per group: 256 B
-  // per slab, 8 B per key (neighbour distance), 24 B per expanded vertex
-  // (entry, bucket range, distance), 16 B per push (atomic, stamp, entry).
+  // empty.  Steps are launched in groups without waiting; after each group
+  // the counters are copied to a pinned snapshot.  The host queues the next
+  // group before it waits for the previous group's snapshot, so the device
+  // keeps working through the host's round trip; groups launched after the
+  // frontier emptied are no-op steps.  Algorithmic bytes (profiling) per
+  // group: 256 B per slab, 8 B per key (neighbour distance), 24 B per
+  // expanded vertex (entry, bucket range, distance), 16 B per push (atomic,
+  // stamp, entry).
   template <int MODE> int iterate(Totals &h) {
     uint32_t side = 0, slot = 0;
-    uint32_t group = 2;
     uint64_t launched = 0;
     const uint64_t limit = uint64_t(g->n) + 8;
-    // the first group is launched without reading the counters (steps on an
-    // empty frontier return at once); later groups only while the frontier
-    // the last group left is non-empty
     Totals prev;
     if (profiling_on()) {
       if (int r = read_counts(prev)) return r;
     }
-    bool more = true;
-    while (more) {
-      if (launched > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
-      for (uint32_t k = 0; k < group; ++k) {
+    uint32_t slot_after[2] = {0, 0};
+    auto group = [&](uint32_t k, int b) -> int {
+      for (uint32_t i = 0; i < k; ++i) {
         launch_expand<MODE>(side, slot, MODE == 0 ? next_epoch() : 0u);
         side ^= 1u;
         slot = slot == 2 ? 0 : slot + 1;
         ++launched;
       }
       MK_HIP(hipGetLastError());
-      if (int r = read_counts(h)) return r;
-      more = h.small[slot] != 0 || (h.large[slot] >> 32) != 0;
+      MK_HIP(hipMemcpyAsync(snap[b].ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
+      MK_HIP(hipEventRecord(snap_ev[b], stream));
+      slot_after[b] = slot;
+      return MK_OK;
+    };
+    if (int r = group(2, 0)) return r;
+    uint32_t k = 2;
+    for (int b = 0;; b ^= 1) {
+      if (launched > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
+      k = std::min<uint32_t>(k * 2, 4);
+      if (int r = group(k, b ^ 1)) return r;  // queued before waiting
+      MK_HIP(hipEventSynchronize(snap_ev[b]));
+      parse(*snap[b].as<Counts>(), h);
       if (profiling_on()) {
         const double bytes = 256.0 * double(h.slabs - prev.slabs) +
                              8.0 * double(h.edges - prev.edges) +
                              24.0 * double(h.visited - prev.visited) +
                              16.0 * double(h.pushed - prev.pushed);
         prof_add_bytes(MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", bytes);
+        prev = h;
       }
-      prev = h;
-      group = std::min<uint32_t>(group * 2, 8);
+      const uint32_t sl = slot_after[b];
+      if (h.small[sl] == 0 && (h.large[sl] >> 32) == 0) break;
     }
     return MK_OK;
   }
@@ -1043,7 +1088,8 @@ struct Traversal {
           const GraphView R = g->rev->view();
           const uint32_t nwords = uint32_t((uint64_t(g->n) + 31) / 32);
           fill_u32(&cnt->ninv, 0u, 1, stream);
-          k_inv_compact<<<blocks_for_threads(nwords), kBlock, 0, stream>>>(view(), nwords);
+          k_inv_compact<<<blocks_for_threads((uint64_t(nwords) + kInvWords - 1) / kInvWords),
+                          kBlock, 0, stream>>>(view(), nwords);
           const unsigned grid = blocks_for_waves(uint64_t(max_waves));
           const unsigned hgrid = blocks_for_waves(uint64_t(max_waves));
           auto kp = g->weighted ? (weights ? k_reseed_pull<true, true> : k_reseed_pull<true, false>)
