@@ -41,6 +41,8 @@ def lib():
         L.orc_bfs_par.restype = _u32
         L.orc_dijkstra.argtypes = [_u32, _vp, _vp, _vp, _u32, _vp]
         L.orc_dijkstra.restype = ctypes.c_int
+        L.orc_sssp_par.argtypes = [_u32, _vp, _vp, _vp, _u32, _vp, ctypes.c_int]
+        L.orc_sssp_par.restype = ctypes.c_int64
         L.orc_pagerank.argtypes = [_u32, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _u32,
                                    _vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.orc_pagerank.restype = _u32
@@ -113,6 +115,16 @@ def dijkstra(g, source):
     d = np.empty(g.n, np.uint32)
     rc = lib().orc_dijkstra(g.n, _p(g.row_off), _p(g.col), _p(g.w), source, _p(d))
     if rc != 0:
+        raise OverflowError("distance exceeds uint32")
+    return d
+
+
+def sssp_par(g, source, threads=0):
+    """Multithreaded label-correcting SSSP (the CPU baseline); same distances
+    as dijkstra()."""
+    d = np.empty(g.n, np.uint32)
+    rc = lib().orc_sssp_par(g.n, _p(g.row_off), _p(g.col), _p(g.w), source, _p(d), threads)
+    if rc < 0:
         raise OverflowError("distance exceeds uint32")
     return d
 

@@ -18,7 +18,10 @@
  *                   FIFO queue.  orc_bfs_par: level-synchronous, OpenMP.
  *   orc_dijkstra    distances with uint32 weights, INF = 0xFFFFFFFF
  *                   (impl/SSSPImpl.cuh:286,571; definitions.cuh:3-6);
- *                   binary heap, 64-bit accumulation.
+ *                   binary heap, 64-bit accumulation.  orc_sssp_par:
+ *                   frontier label-correcting relaxation (the reference's
+ *                   SSSP method, SSSPImpl.cuh:490-605) on OpenMP threads, the
+ *                   multithreaded CPU baseline; equal distances.
  *   orc_pagerank    PR'[u] = (1-d)/N + d*sum_{v in in(u)} PR[v]/outdeg[v]
  *                            + d*sum_{outdeg(x)=0} PR[x]/N,
  *                   delta = sum |PR' - PR|, stop when delta <= margin or
@@ -260,6 +263,82 @@ int orc_dijkstra(uint32_t n, const uint64_t *row_off, const uint32_t *col, const
   free(d);
   free(h);
   return rc;
+}
+
+/* Parallel label-correcting SSSP: every step relaxes the out-edges of the
+ * vertices whose distance dropped in the previous step, with an atomic
+ * compare-and-swap minimum; a per-vertex stamp keeps each vertex once per
+ * frontier.  Returns the number of steps, or -1 when a distance would
+ * exceed uint32 (INF excluded). */
+int64_t orc_sssp_par(uint32_t n, const uint64_t *row_off, const uint32_t *col, const uint32_t *w,
+                     uint32_t src, uint32_t *dist, int threads) {
+  if (threads > 0) omp_set_num_threads(threads);
+  uint32_t *stamp = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+  uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *nxt = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  if (!stamp || !cur || !nxt) {
+    free(stamp);
+    free(cur);
+    free(nxt);
+    return -1;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < (int64_t)n; ++v) dist[v] = ORC_INF;
+  dist[src] = 0;
+  cur[0] = src;
+  uint64_t ncur = 1, nnxt = 0;
+  int overflow = 0;
+  int64_t steps = 0;
+  while (ncur) {
+    nnxt = 0;
+    const uint32_t tag = (uint32_t)steps + 1u;
+#pragma omp parallel
+    {
+      uint32_t local[256];
+      int nl = 0;
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t i = 0; i < (int64_t)ncur; ++i) {
+        const uint32_t u = cur[i];
+        const uint32_t du = __atomic_load_n(&dist[u], __ATOMIC_RELAXED);
+        for (uint64_t j = row_off[u]; j < row_off[u + 1]; ++j) {
+          const uint32_t v = col[j];
+          const uint64_t nd = (uint64_t)du + w[j];
+          if (nd >= ORC_INF) {
+            __atomic_store_n(&overflow, 1, __ATOMIC_RELAXED);
+            continue;
+          }
+          uint32_t old = __atomic_load_n(&dist[v], __ATOMIC_RELAXED);
+          while ((uint32_t)nd < old) {
+            if (__atomic_compare_exchange_n(&dist[v], &old, (uint32_t)nd, 1, __ATOMIC_RELAXED,
+                                            __ATOMIC_RELAXED)) {
+              if (__atomic_exchange_n(&stamp[v], tag, __ATOMIC_RELAXED) != tag) {
+                local[nl++] = v;
+                if (nl == 256) {
+                  uint64_t at = __atomic_fetch_add(&nnxt, 256, __ATOMIC_RELAXED);
+                  memcpy(nxt + at, local, sizeof(local));
+                  nl = 0;
+                }
+              }
+              break;
+            }
+          }
+        }
+      }
+      if (nl) {
+        uint64_t at = __atomic_fetch_add(&nnxt, (uint64_t)nl, __ATOMIC_RELAXED);
+        memcpy(nxt + at, local, sizeof(uint32_t) * nl);
+      }
+    }
+    uint32_t *t = cur;
+    cur = nxt;
+    nxt = t;
+    ncur = nnxt;
+    ++steps;
+  }
+  free(stamp);
+  free(cur);
+  free(nxt);
+  return overflow ? -1 : steps;
 }
 
 /* ------------------------------------------------------------- PageRank */

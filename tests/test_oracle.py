@@ -270,3 +270,18 @@ def test_planted_mistakes_are_caught():
     gunit = oracle.csr(n, z["src"], z["dst"], np.ones(len(z["src"]), np.uint32))
     assert not np.array_equal(oracle.dijkstra(gunit, int(z["source"])), z["sssp"])
     del gin
+
+
+@pytest.mark.parametrize("name", ["sparse_directed", "dense_directed", "undirected", "components"])
+def test_parallel_sssp_matches_golden(name):
+    z = load_golden(name)
+    g = oracle.csr(int(z["n"]), z["src"], z["dst"], z["w"])
+    for threads in (1, 4):
+        assert np.array_equal(oracle.sssp_par(g, int(z["source"]), threads), z["sssp"])
+
+
+def test_parallel_sssp_matches_dijkstra_rmat():
+    n, s, d, w = rmat_graph(13, 8, weighted=True)
+    g = oracle.csr(n, s, d, w)
+    for src in (0, 77, n - 1):
+        assert np.array_equal(oracle.sssp_par(g, src, 4), oracle.dijkstra(g, src))
