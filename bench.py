@@ -111,6 +111,7 @@ def run_serving(args, n, g, s, d, del_idx, used, log):
         done = loop.run(reqs, mode=mode)
         el = time.perf_counter() - t0
         res[mode] = serving.summarize(done, el)
+        res[mode]["phase_s"] = {k: round(v, 3) for k, v in loop.phase_s.items()}
         log(f"serving {mode}: {res[mode]}")
     cap = res["batched"]["requests_per_s"]
     rate = 0.5 * cap
@@ -133,6 +134,8 @@ def run_serving(args, n, g, s, d, del_idx, used, log):
         "query_p99_ms": round(lat.get("query_p99_ms", 0.0), 2),
         "update_p99_ms": round(lat.get("update_p99_ms", 0.0), 2),
         "sequential_query_p50_ms": round(res["sequential"].get("query_p50_ms", 0.0), 2),
+        "phase_seconds": {"sequential": res["sequential"]["phase_s"],
+                          "batched": res["batched"]["phase_s"]},
     }
 
 

@@ -325,12 +325,15 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
         c = qq + 1;
       }
     }
+    // only the lanes of sources active at u read their distance word, so a
+    // sparse mask touches one line of the 256-byte row, not both
+    const bool reads = mine && du_lane != kInf;
     uint32_t v[U], w[U], cur[U];
 #pragma unroll
     for (uint32_t qq = 0; qq < U; ++qq) {
       v[qq] = read_lane(x, ks[qq]);
       w[qq] = MAP ? read_lane(x, ks[qq] + 1) : 1u;
-      cur[qq] = qq < c ? S.dist[size_t(v[qq]) * 64 + lane] : 0u;
+      cur[qq] = (qq < c && reads) ? S.dist[size_t(v[qq]) * 64 + lane] : 0u;
     }
 #pragma unroll
     for (uint32_t qq = 0; qq < U; ++qq) {

@@ -80,6 +80,7 @@ class ServingLoop:
         self.sssp1 = SSSP(graph, 0) if graph.weighted else None
         self.version = 0
         self.rejected_rounds = 0
+        self.phase_s = {"updates": 0.0, "queries": 0.0}  # wall time per phase, last run
 
     # ------------------------------------------------------------ updates
     def _slabs_needed(self, n_ins):
@@ -95,6 +96,7 @@ class ServingLoop:
         return total - used >= self._slabs_needed(n_ins)
 
     def _apply(self, req, now):
+        t = time.perf_counter()
         req.start = now()
         if req.dele_dev is not None:
             self.g.DeleteEdges(*req.dele_dev)
@@ -104,6 +106,7 @@ class ServingLoop:
         self.version += 1
         req.version = self.version
         req.done = now()
+        self.phase_s["updates"] += time.perf_counter() - t
 
     # ------------------------------------------------------------ queries
     def _run_queries(self, engine, reqs, now):
@@ -134,6 +137,7 @@ class ServingLoop:
     def run(self, requests, mode="batched"):
         """Serve `requests` (sorted by arrival) and return them, completed."""
         reqs = [_stage(r, self.g.device) for r in sorted(requests, key=lambda r: r.arrival)]
+        self.phase_s = {"updates": 0.0, "queries": 0.0}
         t0 = time.perf_counter()
 
         def now():
@@ -149,7 +153,9 @@ class ServingLoop:
                     r.admitted = now()
                     self._apply(r, now)
                 else:
+                    tq = time.perf_counter()
                     self._run_single(r, now)
+                    self.phase_s["queries"] += time.perf_counter() - tq
             return reqs
 
         pending_u = collections.deque()
@@ -181,6 +187,7 @@ class ServingLoop:
             qb = [pending["bfs"].popleft() for _ in range(min(self.MAX_BATCH, len(pending["bfs"])))]
             qs = [pending["sssp"].popleft()
                   for _ in range(min(self.MAX_BATCH, len(pending["sssp"])))]
+            tq = time.perf_counter()
             if qb and qs:
                 th = threading.Thread(target=self._run_queries, args=(self.sssp, qs, now))
                 th.start()
@@ -189,6 +196,7 @@ class ServingLoop:
             else:
                 self._run_queries(self.bfs, qb, now)
                 self._run_queries(self.sssp, qs, now)
+            self.phase_s["queries"] += time.perf_counter() - tq
             n_done += len(qb) + len(qs)
         return reqs
 
