@@ -335,22 +335,38 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
       w[qq] = MAP ? read_lane(x, ks[qq] + 1) : 1u;
       cur[qq] = (qq < c && reads) ? S.dist[size_t(v[qq]) * 64 + lane] : 0u;
     }
+    // the U neighbours' atomics are issued back to back (their round trips
+    // overlap), then the mask updates, then one push for all of them
+    uint32_t cand[U], old[U];
+    bool imp[U];
 #pragma unroll
     for (uint32_t qq = 0; qq < U; ++qq) {
-      if (qq >= c) continue;
-      const uint32_t cand = du_lane + w[qq];
-      bool imp = mine && du_lane != kInf && cand >= du_lane && cand < cur[qq];
-      if (imp) {
-        const uint32_t old = atomicMin(S.dist + size_t(v[qq]) * 64 + lane, cand);
-        imp = cand < old;
-      }
-      const uint64_t im = ballot(imp);
-      bool push = false;
-      if (im && lane == 0)
-        push = atomicOr((unsigned long long *)(Fn + v[qq]), (unsigned long long)im) == 0;
-      ms_push(G, S, q, push, v[qq], qn, 0, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
-              S.n, false);
+      cand[qq] = du_lane + w[qq];
+      imp[qq] = qq < c && reads && cand[qq] >= du_lane && cand[qq] < cur[qq];
     }
+#pragma unroll
+    for (uint32_t qq = 0; qq < U; ++qq)
+      old[qq] = imp[qq] ? atomicMin(S.dist + size_t(v[qq]) * 64 + lane, cand[qq]) : 0u;
+    uint64_t im[U];
+#pragma unroll
+    for (uint32_t qq = 0; qq < U; ++qq) im[qq] = ballot(imp[qq] && cand[qq] < old[qq]);
+    unsigned long long po[U];
+#pragma unroll
+    for (uint32_t qq = 0; qq < U; ++qq)
+      po[qq] = (im[qq] && lane == 0)
+                   ? atomicOr((unsigned long long *)(Fn + v[qq]), (unsigned long long)im[qq])
+                   : 1ull;
+    // neighbour qq is pushed (by lane qq) when its mask was empty before
+    uint32_t pm = 0;
+#pragma unroll
+    for (uint32_t qq = 0; qq < U; ++qq)
+      if (im[qq] && read_lane(uint32_t(po[qq] == 0), 0)) pm |= 1u << qq;
+    uint32_t vv = 0;
+#pragma unroll
+    for (uint32_t qq = 0; qq < U; ++qq)
+      if (lane == qq) vv = v[qq];
+    ms_push(G, S, q, lane < U && ((pm >> lane) & 1u), vv, qn, 0, &S.c->cnt[nslot], S.ql[nslot],
+            &S.c->large[nslot], S.n, false);
   }
 }
 
