@@ -306,6 +306,11 @@ __global__ void k_msbfs_read(MsView S, uint32_t s, uint32_t total, uint32_t leve
 // Relax one slab of u for all sources in mu: per live key (neighbour v,
 // weight w), lanes are sources; four neighbours' distance rows are loaded at
 // a time.
+// A frontier vertex with at most this many active sources relaxes its slabs
+// key-parallel (lanes = neighbours, one pass per source) instead of
+// row-parallel (lanes = sources, neighbours a few at a time).
+constexpr uint32_t kMsSparse = 4;
+
 template <bool MAP, uint32_t U>
 MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t du_lane,
                       uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
@@ -313,6 +318,26 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
   const uint64_t km = key_mask<MAP>();
   uint64_t live = ballot(x < kTomb) & km;
   edges += uint32_t(__popcll(live));
+  if (uint32_t(__popcll(mu)) <= kMsSparse) {
+    // key-parallel: lane j relaxes key j of this slab for each active source
+    const bool lv = (live >> lane) & 1ull;
+    const uint32_t wv = MAP ? __shfl_down(x, 1, 64) : 1u;
+    for (uint64_t m = mu; m; m &= m - 1) {
+      const uint32_t src = ffs64(m);
+      const uint32_t du = read_lane(du_lane, src);
+      if (du == kInf) continue;
+      bool push = false;
+      if (lv) {
+        const uint32_t cand = du + wv;
+        uint32_t *dp = S.dist + size_t(x) * 64 + src;
+        if (cand >= du && cand < *dp && cand < atomicMin(dp, cand))
+          push = atomicOr((unsigned long long *)(Fn + x), 1ull << src) == 0;
+      }
+      ms_push(G, S, q, push, x, qn, 0, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot], S.n,
+              false);
+    }
+    return;
+  }
   const bool mine = (mu >> lane) & 1ull;
   while (live) {
     uint32_t ks[U];
