@@ -97,7 +97,8 @@ int mk_graph_destroy(mk_graph *g);
  * PageRank.cuh:11, and updates it by hand, src/PageRank.cu:137). */
 int mk_graph_create_ex(mk_pool *pool, uint32_t n, uint32_t flags, double load_factor,
                        const uint32_t *out_hints, const uint32_t *in_hints, mk_graph **out);
-/* The in-edge mirror of g (read-only: inspection calls only). */
+/* The in-edge mirror of g (read-only: inspection calls only; its degrees,
+ * the in-degrees of g, are counted from its slabs on request). */
 int mk_graph_reverse(mk_graph *g, mk_graph **rev);
 
 /* DynamicGraph::InsertEdges(src, dst, n[, w])

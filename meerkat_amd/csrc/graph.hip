@@ -142,6 +142,7 @@ struct MutSide {
   uint8_t *status;
   uint32_t *counters;
   uint32_t swap;
+  uint32_t degrees;  // maintain G.vdeg (the mirror's in-degrees are counted on request)
 };
 
 // Waves [0, waves_a) apply the batch to side A, the rest to side B, so the
@@ -201,7 +202,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (r == 1) {
       ++applied;
-      if (gl == 0) atomicAdd(G.vdeg + sj, 1u);
+      if (gl == 0 && M.degrees) atomicAdd(G.vdeg + sj, 1u);
     } else if (r < 0) {
       ++pool_fail;
     }
@@ -268,7 +269,7 @@ __global__ void __launch_bounds__(kBlock)
     if (gl == j && r == 1) st = MK_EDGE_APPLIED;
     if (r == 1) {
       ++applied;
-      if (gl == 0) atomicSub(G.vdeg + sj, 1u);
+      if (gl == 0 && M.degrees) atomicSub(G.vdeg + sj, 1u);
     }
   }
   if (only_status != 0xFFu) {
@@ -639,8 +640,11 @@ static int range_error(mk_graph *g, const char *who) {
 }
 
 // The mutation side of graph g (its own view, status and counters).
+// The mirror keeps no degree array up to date: no kernel reads it, and
+// mk_graph_degrees counts its live keys instead.
 static MutSide side_of(mk_graph *g, bool swap) {
-  return MutSide{g->view(), g->status.as<uint8_t>(), g->counters, swap ? 1u : 0u};
+  return MutSide{g->view(), g->status.as<uint8_t>(), g->counters, swap ? 1u : 0u,
+                 g->mirror_of ? 0u : 1u};
 }
 
 // Launch the insert kernel over g and, when it has one, its mirror (one
@@ -652,10 +656,10 @@ static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
   const MutSide a = side_of(g, false), b = g->rev ? side_of(g->rev, true) : a;
   const uint64_t total = g->rev ? 2 * waves : waves;
-  // per edge and graph: batch entry, one slab probe, key (+value) write,
-  // degree atomic, status byte
-  const double bytes = double(n) * (g->rev ? 2 : 1) *
-                       (g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1);
+  // per edge: batch entry, one slab probe, key (+value) write, degree atomic,
+  // status byte; the mirror side the same without the degree atomic
+  const double per = g->weighted ? 12 + 256 + 8 + 8 + 1 : 8 + 256 + 4 + 8 + 1;
+  const double bytes = double(n) * (g->rev ? 2 * per - 8 : per);
   prof_launch(name, bytes, g->weighted ? k_insert<true> : k_insert<false>,
               dim3(blocks_for_waves(total)), dim3(kBlock), 0u, s, a, b, uint32_t(waves),
               uint32_t(total - waves), g->pool->view, src, dst, g->weighted ? w : nullptr, n, epw,
@@ -671,9 +675,9 @@ static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
   const MutSide a = side_of(g, false), b = g->rev ? side_of(g->rev, true) : a;
   const uint64_t total = g->rev ? 2 * waves : waves;
-  // per edge and graph: batch entry, one slab probe, tombstone write, degree
-  // atomic, status byte
-  prof_launch(name, double(n) * (g->rev ? 2 : 1) * (8 + 256 + 4 + 8 + 1),
+  // per edge: batch entry, one slab probe, tombstone write, degree atomic,
+  // status byte; the mirror side the same without the degree atomic
+  prof_launch(name, double(n) * (g->rev ? 2 * 277 - 8 : 277),
               g->weighted ? k_delete<true> : k_delete<false>, dim3(blocks_for_waves(total)),
               dim3(kBlock), 0u, s, a, b, uint32_t(waves), uint32_t(total - waves), src, dst, n,
               epw, only_status, bad);
@@ -781,6 +785,7 @@ int mk_graph_is_weighted(mk_graph *g, int *w) {
 
 int mk_graph_degrees(mk_graph *g, uint32_t *deg) {
   if (!g || !deg) return fail(MK_ERR_BAD_ARG, "mk_graph_degrees: NULL argument");
+  if (g->mirror_of) return mk_graph_count_slots(g, deg, nullptr);  // counted from the slabs
   MK_HIP(hipSetDevice(g->device));
   MK_HIP(hipMemcpy(deg, g->vdeg, size_t(g->n) * 4, hipMemcpyDeviceToHost));
   return MK_OK;
