@@ -136,7 +136,9 @@ __global__ void k_validate(const uint32_t *src, const uint32_t *dst, uint32_t n,
 
 // One side of a batch mutation: the graph it applies to, that graph's
 // per-edge status and counters, and whether it takes (dst, src) (the in-edge
-// mirror) instead of (src, dst).
+// mirror) instead of (src, dst).  Kernels pick a side field by field
+// (pick_side): selecting a reference to one of two by-value kernel arguments
+// would copy both to private memory.
 struct MutSide {
   GraphView G;
   uint8_t *status;
@@ -144,6 +146,23 @@ struct MutSide {
   uint32_t swap;
   uint32_t degrees;  // maintain G.vdeg (the mirror's in-degrees are counted on request)
 };
+
+MK_DEV MutSide pick_side(bool second, const MutSide &A, const MutSide &B) {
+  MutSide m;
+  m.G.heads = second ? B.G.heads : A.G.heads;
+  m.G.pool = second ? B.G.pool : A.G.pool;
+  m.G.bucket_off = second ? B.G.bucket_off : A.G.bucket_off;
+  m.G.bucket_owner = second ? B.G.bucket_owner : A.G.bucket_owner;
+  m.G.vdeg = second ? B.G.vdeg : A.G.vdeg;
+  m.G.n = second ? B.G.n : A.G.n;
+  m.G.total_buckets = second ? B.G.total_buckets : A.G.total_buckets;
+  m.G.weighted = second ? B.G.weighted : A.G.weighted;
+  m.status = second ? B.status : A.status;
+  m.counters = second ? B.counters : A.counters;
+  m.swap = second ? B.swap : A.swap;
+  m.degrees = second ? B.degrees : A.degrees;
+  return m;
+}
 
 // Waves [0, waves_a) apply the batch to side A, the rest to side B, so the
 // forward graph and its mirror are updated by one launch with their
@@ -163,7 +182,7 @@ __global__ void __launch_bounds__(kBlock)
   const bool second = wave_all >= waves_a;
   // waves past both sides (the last block's spare waves) have nothing to do
   if (second && wave_all - waves_a >= waves_b) return;
-  const MutSide &M = second ? B : A;
+  const MutSide M = pick_side(second, A, B);
   const uint32_t wave = second ? wave_all - waves_a : wave_all;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
@@ -231,7 +250,7 @@ __global__ void __launch_bounds__(kBlock)
   const bool second = wave_all >= waves_a;
   // waves past both sides (the last block's spare waves) have nothing to do
   if (second && wave_all - waves_a >= waves_b) return;
-  const MutSide &M = second ? B : A;
+  const MutSide M = pick_side(second, A, B);
   const uint32_t wave = second ? wave_all - waves_a : wave_all;
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;

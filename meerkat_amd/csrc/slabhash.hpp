@@ -175,47 +175,60 @@ MK_DEV int slab_search(const GraphView &G, uint32_t src, uint32_t dst, uint32_t 
 }
 
 // ================================================================ groups
-// Group-wide variants: a wave runs four independent slab operations, one per
-// group of 16 lanes.  Group lane gl holds words 4gl..4gl+3 of the current
-// slab (one 16-byte read per lane, the group reading the 256-byte slab in
-// one coalesced access), so four chain walks / claims are in flight per
-// wave instead of one.  Every value the group branches on is group-uniform
-// (broadcast with a width-16 shuffle), so the 16 lanes of a group stay
-// active together and ballots restricted to the group's 16 bits are exact.
-// Same concurrency contract as the wave versions above.
-constexpr uint32_t kGroupLanes = 16;
+// Group-wide variants: a wave runs kGroupsPerWave independent slab
+// operations, one per group of kGroupLanes lanes.  Group lane gl holds
+// words kWordsPerLane*gl .. kWordsPerLane*(gl+1)-1 of the current slab (the
+// group reads the 256-byte slab in one coalesced access), so several chain
+// walks / claims are in flight per wave instead of one.  Every value the
+// group branches on is group-uniform (broadcast with a group-width shuffle),
+// so the lanes of a group stay active together and ballots restricted to
+// the group's bits are exact.  Same concurrency contract as the wave
+// versions above.
+constexpr uint32_t kGroupLanes = 8;
 constexpr uint32_t kGroupsPerWave = 64 / kGroupLanes;
+constexpr uint32_t kWordsPerLane = 64 / kGroupLanes;
+constexpr uint32_t kGroupMask = (1u << kGroupLanes) - 1u;
 
 MK_DEV uint32_t group_lane() { return lane_id() & (kGroupLanes - 1); }
 MK_DEV uint32_t group_first_lane() { return lane_id() & ~(kGroupLanes - 1); }
-// The calling group's 16 bits of a wave ballot, bit i = group lane i.
+// The calling group's bits of a wave ballot, bit i = group lane i.
 MK_DEV uint32_t group_bits(uint64_t wave_mask) {
-  return uint32_t(wave_mask >> group_first_lane()) & 0xFFFFu;
+  return uint32_t(wave_mask >> group_first_lane()) & kGroupMask;
 }
-// Value of `v` held by group lane `src` (0..15) of the calling group.
+// Value of `v` held by group lane `src` of the calling group.
 MK_DEV uint32_t group_read(uint32_t v, uint32_t src) {
   return uint32_t(__shfl(int(v), int(src), int(kGroupLanes)));
 }
 MK_DEV uint32_t ffs32(uint32_t m) { return uint32_t(__ffs(m) - 1); }
 
-// Words 4gl..4gl+3 at p (16-byte aligned) with agent-scope loads.
-MK_DEV void load_quad_coherent(const uint32_t *p, uint32_t (&v)[4]) {
+// This lane's kWordsPerLane words at p (8-byte aligned), agent-scope loads.
+MK_DEV void load_words_coherent(const uint32_t *p, uint32_t (&v)[kWordsPerLane]) {
   const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
-  const uint64_t a = coherent_load64(q), b = coherent_load64(q + 1);
-  v[0] = uint32_t(a);
-  v[1] = uint32_t(a >> 32);
-  v[2] = uint32_t(b);
-  v[3] = uint32_t(b >> 32);
+#pragma unroll
+  for (uint32_t k = 0; k < kWordsPerLane / 2; ++k) {
+    const uint64_t a = coherent_load64(q + k);
+    v[2 * k] = uint32_t(a);
+    v[2 * k + 1] = uint32_t(a >> 32);
+  }
 }
-MK_DEV void load_quad(const uint32_t *p, uint32_t (&v)[4]) {
-  const uint4 x = *reinterpret_cast<const uint4 *>(p);
-  v[0] = x.x;
-  v[1] = x.y;
-  v[2] = x.z;
-  v[3] = x.w;
+MK_DEV void load_words(const uint32_t *p, uint32_t (&v)[kWordsPerLane]) {
+  const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+  for (uint32_t k = 0; k < kWordsPerLane / 4; ++k) {
+    const uint4 x = q[k];
+    v[4 * k] = x.x;
+    v[4 * k + 1] = x.y;
+    v[4 * k + 2] = x.z;
+    v[4 * k + 3] = x.w;
+  }
 }
-MK_DEV uint32_t pick4(const uint32_t (&v)[4], uint32_t j) {
-  return j == 0 ? v[0] : j == 1 ? v[1] : j == 2 ? v[2] : v[3];
+// v[j] for a lane-varying j (a select chain, no scratch indexing).
+MK_DEV uint32_t pick(const uint32_t (&v)[kWordsPerLane], uint32_t j) {
+  uint32_t r = v[0];
+#pragma unroll
+  for (uint32_t k = 1; k < kWordsPerLane; ++k)
+    if (j == k) r = v[k];
+  return r;
 }
 
 // Whether slab word i holds a key (set: 0..62; map: even words 0..60).
@@ -223,7 +236,8 @@ template <bool MAP> MK_DEV bool key_word(uint32_t i) {
   return MAP ? ((i & 1u) == 0 && i <= 60u) : i < kNextLane;
 }
 
-// pool_alloc for one group: lane gl scans bitmap words 4gl..4gl+3 of a block.
+// pool_alloc for one group: lane gl scans its kWordsPerLane bitmap words of
+// a block.
 MK_DEV uint32_t pool_alloc_group(const PoolView &P, uint32_t seed) {
   const uint32_t gl = group_lane();
   if (coherent_load(P.counters) >= P.total_slabs) {
@@ -234,34 +248,33 @@ MK_DEV uint32_t pool_alloc_group(const PoolView &P, uint32_t seed) {
   const uint32_t rot = (seed >> 7) & (kGroupLanes - 1);
   for (uint32_t tries = 0; tries < P.nblocks; ++tries) {
     uint32_t *wp = P.bitmap + size_t(blk) * kBlockWords;
-    uint32_t v[4];
-    load_quad_coherent(wp + 4 * gl, v);
+    uint32_t v[kWordsPerLane];
+    load_words_coherent(wp + kWordsPerLane * gl, v);
     for (;;) {
-      uint32_t fj = 4;  // this lane's first word with a clear bit
+      uint32_t fj = kWordsPerLane;  // this lane's first word with a clear bit
 #pragma unroll
-      for (int j = 3; j >= 0; --j)
+      for (int j = kWordsPerLane - 1; j >= 0; --j)
         if (v[j] != 0xFFFFFFFFu) fj = uint32_t(j);
-      const uint32_t cand = group_bits(ballot(fj < 4));
+      const uint32_t cand = group_bits(ballot(fj < kWordsPerLane));
       if (!cand) break;
       // rotate the starting lane so groups sharing a block spread out
-      const uint32_t rotated = ((cand >> rot) | (cand << (kGroupLanes - rot))) & 0xFFFFu;
+      const uint32_t rotated = ((cand >> rot) | (cand << (kGroupLanes - rot))) & kGroupMask;
       const uint32_t l = (ffs32(rotated) + rot) & (kGroupLanes - 1);
       const uint32_t wj = group_read(fj, l);
-      const uint32_t wv = group_read(fj < 4 ? pick4(v, fj) : 0u, l);
+      const uint32_t wv = group_read(fj < kWordsPerLane ? pick(v, fj) : 0u, l);
       const uint32_t bit = ffs32(~wv);
       uint32_t old = 0;
-      if (gl == l) old = atomicOr(wp + 4 * l + wj, 1u << bit);
+      if (gl == l) old = atomicOr(wp + kWordsPerLane * l + wj, 1u << bit);
       old = group_read(old, l);
       if ((old & (1u << bit)) == 0) {
         if (gl == 0) atomicAdd(P.counters, 1u);
-        return blk * kBlockSlabs + (4 * l + wj) * 32u + bit;
+        return blk * kBlockSlabs + (kWordsPerLane * l + wj) * 32u + bit;
       }
       if (gl == l) {
         const uint32_t nv = old | (1u << bit);
-        if (wj == 0) v[0] = nv;
-        else if (wj == 1) v[1] = nv;
-        else if (wj == 2) v[2] = nv;
-        else v[3] = nv;
+#pragma unroll
+        for (uint32_t k = 0; k < kWordsPerLane; ++k)
+          if (wj == k) v[k] = nv;
       }
     }
     blk = (blk + 1 == P.nblocks) ? 0 : blk + 1;
@@ -280,28 +293,28 @@ MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t src
   uint32_t *free_slab = nullptr;
   uint32_t free_word = 0, free_old = 0;
   for (;;) {
-    uint32_t v[4];
-    load_quad_coherent(slab + 4 * gl, v);
+    uint32_t v[kWordsPerLane];
+    load_words_coherent(slab + kWordsPerLane * gl, v);
     bool hit = false;
-    uint32_t fj = 4;  // this lane's first free key word
+    uint32_t fj = kWordsPerLane;  // this lane's first free key word
 #pragma unroll
-    for (int j = 3; j >= 0; --j) {
-      if (key_word<MAP>(4 * gl + j)) {
+    for (int j = kWordsPerLane - 1; j >= 0; --j) {
+      if (key_word<MAP>(kWordsPerLane * gl + j)) {
         hit |= v[j] == dst;
         if (v[j] >= kTomb) fj = uint32_t(j);
       }
     }
     if (group_bits(ballot(hit))) return 0;
     if (free_slab == nullptr) {
-      const uint32_t fb = group_bits(ballot(fj < 4));
+      const uint32_t fb = group_bits(ballot(fj < kWordsPerLane));
       if (fb) {
         const uint32_t fl = ffs32(fb);
-        free_word = 4 * fl + group_read(fj, fl);
-        free_old = group_read(fj < 4 ? pick4(v, fj) : 0u, fl);
+        free_word = kWordsPerLane * fl + group_read(fj, fl);
+        free_old = group_read(fj < kWordsPerLane ? pick(v, fj) : 0u, fl);
         free_slab = slab;
       }
     }
-    const uint32_t nx = group_read(v[3], kGroupLanes - 1);  // word 63
+    const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);  // word 63
     if (nx != kEmpty) {
       slab = slab_ptr(G.pool, nx);
       continue;
@@ -322,11 +335,14 @@ MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t src
     // chain full: link a fresh slab holding the key at its first slot
     const uint32_t ns = pool_alloc_group(P, seed);
     if (ns == kEmpty) return -1;
-    uint32_t *nsp = slab_ptr(G.pool, ns) + 4 * gl;
-    coherent_store(nsp + 0, gl == 0 ? dst : kEmpty);
-    coherent_store(nsp + 1, (MAP && gl == 0) ? w : kEmpty);
-    coherent_store(nsp + 2, kEmpty);
-    coherent_store(nsp + 3, kEmpty);
+    uint32_t *nsp = slab_ptr(G.pool, ns) + kWordsPerLane * gl;
+#pragma unroll
+    for (uint32_t k = 0; k < kWordsPerLane; ++k) {
+      uint32_t init = kEmpty;
+      if (gl == 0 && k == 0) init = dst;
+      if (MAP && gl == 0 && k == 1) init = w;
+      coherent_store(nsp + k, init);
+    }
     __atomic_thread_fence(__ATOMIC_RELEASE);  // slab contents before the link
     uint32_t old = 0;
     if (gl == 0) old = atomicCAS(slab + kNextLane, kEmpty, ns);
@@ -344,22 +360,22 @@ MK_DEV int slab_delete_group(const GraphView &G, uint32_t src, uint32_t dst) {
   const uint32_t gl = group_lane();
   uint32_t *slab = head_of(G, src, dst);
   for (;;) {
-    uint32_t v[4];
-    load_quad_coherent(slab + 4 * gl, v);
-    uint32_t hj = 4;
+    uint32_t v[kWordsPerLane];
+    load_words_coherent(slab + kWordsPerLane * gl, v);
+    uint32_t hj = kWordsPerLane;
 #pragma unroll
-    for (int j = 3; j >= 0; --j)
-      if (key_word<MAP>(4 * gl + j) && v[j] == dst) hj = uint32_t(j);
-    const uint32_t hb = group_bits(ballot(hj < 4));
+    for (int j = kWordsPerLane - 1; j >= 0; --j)
+      if (key_word<MAP>(kWordsPerLane * gl + j) && v[j] == dst) hj = uint32_t(j);
+    const uint32_t hb = group_bits(ballot(hj < kWordsPerLane));
     if (hb) {
       const uint32_t hl = ffs32(hb);
-      const uint32_t word = 4 * hl + group_read(hj, hl);
+      const uint32_t word = kWordsPerLane * hl + group_read(hj, hl);
       uint32_t old = 0;
       if (gl == 0) old = atomicCAS(slab + word, dst, kTomb);
       old = group_read(old, 0);
       return old == dst ? 1 : 0;
     }
-    const uint32_t nx = group_read(v[3], kGroupLanes - 1);
+    const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);
     if (nx == kEmpty) return 0;
     slab = slab_ptr(G.pool, nx);
   }
@@ -372,22 +388,22 @@ MK_DEV int slab_search_group(const GraphView &G, uint32_t src, uint32_t dst, uin
   const uint32_t gl = group_lane();
   const uint32_t *slab = head_of(G, src, dst);
   for (;;) {
-    uint32_t v[4];
-    load_quad(slab + 4 * gl, v);
-    uint32_t hj = 4;
+    uint32_t v[kWordsPerLane];
+    load_words(slab + kWordsPerLane * gl, v);
+    uint32_t hj = kWordsPerLane;
 #pragma unroll
-    for (int j = 3; j >= 0; --j)
-      if (key_word<MAP>(4 * gl + j) && v[j] == dst) hj = uint32_t(j);
-    const uint32_t hb = group_bits(ballot(hj < 4));
+    for (int j = kWordsPerLane - 1; j >= 0; --j)
+      if (key_word<MAP>(kWordsPerLane * gl + j) && v[j] == dst) hj = uint32_t(j);
+    const uint32_t hb = group_bits(ballot(hj < kWordsPerLane));
     if (hb) {
       const uint32_t hl = ffs32(hb);
       // a map key sits at an even word; its value is the next word, in the
-      // same lane (4gl + j + 1 with j even <= 2)
-      const uint32_t val = hj < 4 ? pick4(v, hj + 1 < 4 ? hj + 1 : 3) : 0u;
+      // same lane (kWordsPerLane is even)
+      const uint32_t val = hj < kWordsPerLane ? pick(v, hj + 1 < kWordsPerLane ? hj + 1 : hj) : 0u;
       *w = MAP ? group_read(val, hl) : 0u;
       return 1;
     }
-    const uint32_t nx = group_read(v[3], kGroupLanes - 1);
+    const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);
     if (nx == kEmpty) return 0;
     slab = slab_ptr(G.pool, nx);
   }
