@@ -760,13 +760,16 @@ __global__ void __launch_bounds__(kItemBlock)
 }
 
 // Decremental seed: batch edge (u, v) was deleted; if it was v's tree edge,
-// v becomes invalid (Invalidate, BFSImpl.cuh:3-15).
+// v becomes invalid (Invalidate, BFSImpl.cuh:3-15).  With `collect`, such a
+// v (an orphan) is appended to S.inv_list instead (count S.cnt->ninv) for
+// k_repair, which looks for another parent before invalidating.
 __global__ void __launch_bounds__(kItemBlock)
     k_invalidate_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
-                       uint32_t n, uint32_t source) {
+                       uint32_t n, uint32_t source, uint32_t collect) {
   __shared__ uint32_t sh[kItemBlock / 64 + 1];
+  const uint32_t lane = lane_id();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool push = false;
+  bool push = false, orphan = false;
   uint32_t v = 0;
   if (i < n) {
     const uint32_t u = src[i];
@@ -774,16 +777,106 @@ __global__ void __launch_bounds__(kItemBlock)
     if (u != v && u < G.n && v < G.n && v != source) {
       const uint64_t cur = S.dist[v];
       if (cur != kInfInf && parent_of(cur) == u) {
-        if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
-                      (unsigned long long)kInfInf) == cur) {
+        if (collect) {
+          orphan = true;
+        } else if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
+                             (unsigned long long)kInfInf) == cur) {
           atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
           push = true;
         }
       }
     }
   }
+  if (collect) {
+    const uint64_t om = ballot(orphan);
+    if (om) {
+      const uint32_t leader = ffs64(om);
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&S.cnt->ninv, uint32_t(__popcll(om)));
+      base = read_lane(base, leader);
+      if (orphan) S.inv_list[base + lanes_below(om)] = v;
+    }
+    return;
+  }
   block_enqueue<kItemBlock / 64>(G, S, 0, 0, push, v, sh);
 }
+
+// Orphans with at most this many in-buckets are searched for another parent.
+constexpr uint32_t kRepairBuckets = 8;
+
+// Re-parent orphans: an orphan v (tree edge deleted) keeps its distance if
+// an in-neighbour p (read from the in-edge mirror R) has dist[p] + w(p, v)
+// == dist[v]; v's parent becomes p and its subtree is left alone.  This is
+// safe even when p is itself invalidated later in the same call: the
+// invalidation walk then reaches v as p's child.  (With weights >= 1, p is
+// not in v's subtree.)  Orphans without such a p are invalidated and pushed
+// as k_invalidate_batch does.  One 8-lane group per orphan.
+template <bool MAP, bool WEIGHTS>
+__global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, TravView S) {
+  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
+  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  const uint32_t gl = group_lane();
+  const uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / kGroupLanes;
+  const uint32_t ng = (gridDim.x * blockDim.x) / kGroupLanes;
+  const uint32_t total = S.cnt->ninv;
+  // every group runs the same number of rounds (wave-collective enqueue)
+  const uint32_t rounds = (total + ng - 1) / ng;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t k = r * ng + gid;
+    bool push = false;
+    uint32_t v = 0;
+    if (k < total) {
+      v = S.inv_list[k];
+      // one read, broadcast: the group's decisions must agree
+      const uint64_t c0 = S.dist[v];
+      const uint64_t cur = (uint64_t(group_read(uint32_t(c0 >> 32), 0)) << 32) |
+                           group_read(uint32_t(c0), 0);
+      const uint32_t dv = dist_of(cur);
+      uint64_t repl = kInfInf;  // new packed value: repaired, or INF
+      if (cur != kInfInf) {
+        const uint32_t rb0 = R.bucket_off[v], rnb = R.bucket_off[v + 1] - rb0;
+        uint32_t found = kEmpty;
+        for (uint32_t bk = 0; bk < rnb && rnb <= kRepairBuckets && found == kEmpty; ++bk) {
+          const uint32_t *slab = R.heads + size_t(rb0 + bk) * kSlabWords;
+          for (;;) {
+            uint32_t x[kWordsPerLane];
+            load_words(slab + kWordsPerLane * gl, x);
+            uint32_t mine = kEmpty;
+#pragma unroll
+            for (uint32_t j = 0; j < kWordsPerLane; ++j) {
+              const uint32_t word = kWordsPerLane * gl + j;
+              if (!key_word<MAP>(word) || x[j] >= kTomb || mine != kEmpty) continue;
+              const uint32_t p = x[j];
+              const uint32_t w = WEIGHTS ? x[j + 1 < kWordsPerLane ? j + 1 : j] : 1u;
+              const uint32_t dp = dist_of(S.dist[p]);
+              if (p != parent_of(cur) && dp != kInf && uint64_t(dp) + w == dv) mine = p;
+            }
+            const uint32_t hb = group_bits(ballot(mine != kEmpty));
+            if (hb) {
+              found = group_read(mine, ffs32(hb));
+              break;
+            }
+            const uint32_t nx = group_read(x[kWordsPerLane - 1], kGroupLanes - 1);
+            if (nx == kEmpty) break;
+            slab = slab_ptr(R.pool, nx);
+          }
+        }
+        if (found != kEmpty) repl = pack(dv, found);
+        if (gl == 0) {
+          const uint64_t old = atomicCAS((unsigned long long *)(S.dist + v),
+                                         (unsigned long long)cur, (unsigned long long)repl);
+          if (old == cur && repl == kInfInf) {
+            atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
+            push = true;
+          }
+        }
+      }
+    }
+    enqueue(G, S, q, 0, 0, push, v);
+  }
+  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
+}
+
 
 // Start a static run: the source at distance 0 is the frontier (side 0,
 // slot 0), in the queue its bucket count selects.
@@ -1069,8 +1162,16 @@ struct Traversal {
       const GraphView G = g->view();
       {
         const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
+        // with the in-edge mirror, orphans are first offered another parent
+        const uint32_t collect = g->rev ? 1u : 0u;
         prof_launch("k_invalidate_batch", 24.0 * n, k_invalidate_batch, dim3(grid),
-                    dim3(kItemBlock), 0u, stream, G, view(), src, dst, n, source);
+                    dim3(kItemBlock), 0u, stream, G, view(), src, dst, n, source, collect);
+        if (collect) {
+          auto k = g->weighted ? (weights ? k_repair<true, true> : k_repair<true, false>)
+                               : k_repair<false, false>;
+          prof_launch("k_repair", 0.0, k, dim3(blocks_for_waves(uint64_t(max_waves))),
+                      dim3(kBlock), 0u, stream, G, g->rev->view(), view());
+        }
       }
       MK_HIP(hipGetLastError());
       // propagate invalidation down the tree; any step ran = something was

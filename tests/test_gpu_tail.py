@@ -111,3 +111,33 @@ def test_small_batches_on_rmat_stay_exact(gpu):
         keys, wt = keys[order], wt[order]
         s2, d2 = gen.split_keys(keys)
         check(sp, n, s2, d2, wt)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("mirror", [False, True])
+def test_deleted_tree_edge_with_equal_alternative(gpu, weighted, mirror):
+    # 0 -> 1 -> 3, 0 -> 2 -> 3 (equal lengths), then a 200-vertex chain below
+    # 3: deleting whichever edge 3's tree uses leaves every distance as it
+    # was (with the mirror, 3 is re-parented and the chain never invalidated)
+    chain = np.arange(3, 203, dtype=np.uint32)
+    s = np.r_[np.array([0, 0, 1, 2], np.uint32), chain[:-1]]
+    d = np.r_[np.array([1, 2, 3, 3], np.uint32), chain[1:]]
+    w = np.r_[np.array([2, 3, 4, 3], np.uint32), np.full(len(chain) - 1, 5, np.uint32)]
+    n = 203
+    pool = mk.SlabPool(1 << 22)
+    g = mk.DynamicGraph(n, pool, 0.7, None, weighted=weighted, reverse=mirror)
+    g.InsertEdges(s, d, w if weighted else None)
+    algo = mk.SSSP(g, 0) if weighted else mk.BFS(g, 0)
+    algo.Static()
+    ww = w if weighted else None
+    check(algo, n, s, d, ww)
+    before = algo.Distances().copy()
+    p3 = int(algo.Parents()[3])
+    assert p3 in (1, 2)
+    es, ed = np.array([p3], np.uint32), np.array([3], np.uint32)
+    g.DeleteEdges(es, ed)
+    algo.Decremental(es, ed) if not weighted else algo.Decremental(es, ed, None)
+    keep = ~((s == p3) & (d == 3))
+    check(algo, n, s[keep], d[keep], ww[keep] if weighted else None)
+    assert np.array_equal(algo.Distances(), before)
+    assert int(algo.Parents()[3]) == 3 - p3  # the other middle vertex
