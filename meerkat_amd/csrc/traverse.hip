@@ -761,8 +761,9 @@ __global__ void __launch_bounds__(kItemBlock)
 
 // Decremental seed: batch edge (u, v) was deleted; if it was v's tree edge,
 // v becomes invalid (Invalidate, BFSImpl.cuh:3-15).  With `collect`, such a
-// v (an orphan) is appended to S.inv_list instead (count S.cnt->ninv) for
-// k_repair, which looks for another parent before invalidating.
+// v (an orphan) gets its invalid bit and is appended once to S.inv_list
+// instead (count S.cnt->ninv) for k_repair, which looks for another parent
+// before invalidating.
 __global__ void __launch_bounds__(kItemBlock)
     k_invalidate_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
                        uint32_t n, uint32_t source, uint32_t collect) {
@@ -778,7 +779,10 @@ __global__ void __launch_bounds__(kItemBlock)
       const uint64_t cur = S.dist[v];
       if (cur != kInfInf && parent_of(cur) == u) {
         if (collect) {
-          orphan = true;
+          // the invalid bit marks v as listed: a batch naming (u, v) twice
+          // lists v once (the list holds at most n vertices)
+          const uint32_t bit = 1u << (v & 31u);
+          orphan = (atomicOr(S.invbits + (v >> 5), bit) & bit) == 0u;
         } else if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                              (unsigned long long)kInfInf) == cur) {
           atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
@@ -808,9 +812,12 @@ constexpr uint32_t kRepairBuckets = 8;
 // an in-neighbour p (read from the in-edge mirror R) has dist[p] + w(p, v)
 // == dist[v]; v's parent becomes p and its subtree is left alone.  This is
 // safe even when p is itself invalidated later in the same call: the
-// invalidation walk then reaches v as p's child.  (With weights >= 1, p is
-// not in v's subtree.)  Orphans without such a p are invalidated and pushed
-// as k_invalidate_batch does.  One 8-lane group per orphan.
+// invalidation walk then reaches v as p's child.  Only edges of weight >= 1
+// qualify, so dist[p] < dist[v]: p is not in v's subtree, and the new parent
+// pointers form no cycle (each repaired pointer strictly lowers the
+// distance, tree pointers never raise it).  A repaired orphan's invalid bit
+// is cleared; one without such a p keeps it, is set to INF and is pushed as
+// k_invalidate_batch does.  One 8-lane group per orphan (each listed once).
 template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, TravView S) {
   __shared__ uint32_t lq[kBlock / 64][kWaveQ];
@@ -849,7 +856,8 @@ __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, Tra
               const uint32_t p = x[j];
               const uint32_t w = WEIGHTS ? x[j + 1 < kWordsPerLane ? j + 1 : j] : 1u;
               const uint32_t dp = dist_of(S.dist[p]);
-              if (p != parent_of(cur) && dp != kInf && uint64_t(dp) + w == dv) mine = p;
+              if (w != 0u && p != parent_of(cur) && dp != kInf && uint64_t(dp) + w == dv)
+                mine = p;
             }
             const uint32_t hb = group_bits(ballot(mine != kEmpty));
             if (hb) {
@@ -865,10 +873,10 @@ __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, Tra
         if (gl == 0) {
           const uint64_t old = atomicCAS((unsigned long long *)(S.dist + v),
                                          (unsigned long long)cur, (unsigned long long)repl);
-          if (old == cur && repl == kInfInf) {
-            atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
+          if (old == cur && repl == kInfInf)
             push = true;
-          }
+          else
+            atomicAnd(S.invbits + (v >> 5), ~(1u << (v & 31u)));
         }
       }
     }
