@@ -268,8 +268,12 @@ int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
  * counterpart; the reference hard-codes its shapes, e.g. BFSImpl.cuh:
  * 620-625).  "waves_per_cu" (1..64, default 16): waves the grid-stride
  * kernels launch per compute unit.  "ms_unroll" (4 or 8, default 8):
- * neighbour distance rows in flight per wave in multi-source SSSP.  The
- * Python autotuner (meerkat_amd/autotune.py) sets them from measurements. */
+ * neighbour distance rows in flight per wave in multi-source SSSP.
+ * "orphan_repair" (0 or 1, default 1): a decremental BFS/SSSP call on a
+ * graph with the in-edge mirror first offers each vertex whose tree edge
+ * was deleted another parent at the same distance, sparing its subtree.
+ * The Python autotuner (meerkat_amd/autotune.py) sets them from
+ * measurements. */
 int mk_tune_set(const char *name, int64_t value);
 int mk_tune_get(const char *name, int64_t *value);
 

@@ -931,6 +931,7 @@ struct Traversal {
   uint64_t last_slabs = 0;
   uint64_t last_edges = 0;
   int max_waves = 0;
+  bool repair = true;  // orphan repair before invalidation (with the in-edge mirror)
 
   TravView view() const {
     TravView v;
@@ -976,6 +977,7 @@ struct Traversal {
     k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
     MK_HIP(hipStreamSynchronize(stream));
     max_waves = grid_waves(g->device);
+    repair = tune_get("orphan_repair", 1) != 0;
     return MK_OK;
   }
 
@@ -1171,7 +1173,7 @@ struct Traversal {
       {
         const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
         // with the in-edge mirror, orphans are first offered another parent
-        const uint32_t collect = g->rev ? 1u : 0u;
+        const uint32_t collect = g->rev && repair ? 1u : 0u;
         prof_launch("k_invalidate_batch", 24.0 * n, k_invalidate_batch, dim3(grid),
                     dim3(kItemBlock), 0u, stream, G, view(), src, dst, n, source, collect);
         if (collect) {

@@ -87,7 +87,7 @@ def test_broom_spill_and_hub_handover(gpu, hints):
     check(b, n, s, d)
 
 
-def test_small_batches_on_rmat_stay_exact(gpu):
+def test_small_batches_on_rmat_stay_exact(gpu, repair):
     # small update batches give small frontiers (tail mode) on a skewed graph
     n, s, d, w = gen.rmat(14, 8, seed=41, weighted=True)
     pool = mk.SlabPool(1 << 27)
@@ -113,9 +113,18 @@ def test_small_batches_on_rmat_stay_exact(gpu):
         check(sp, n, s2, d2, wt)
 
 
+@pytest.fixture(params=[0, 1], ids=["no_repair", "repair"])
+def repair(request):
+    from meerkat_amd import autotune
+    before = autotune.get_knob("orphan_repair")
+    autotune.set_knob("orphan_repair", request.param)
+    yield request.param
+    autotune.set_knob("orphan_repair", before)
+
+
 @pytest.mark.parametrize("weighted", [False, True])
 @pytest.mark.parametrize("mirror", [False, True])
-def test_deleted_tree_edge_with_equal_alternative(gpu, weighted, mirror):
+def test_deleted_tree_edge_with_equal_alternative(gpu, weighted, mirror, repair):
     # 0 -> 1 -> 3, 0 -> 2 -> 3 (equal lengths), then a 200-vertex chain below
     # 3: deleting whichever edge 3's tree uses leaves every distance as it
     # was (with the mirror, 3 is re-parented and the chain never invalidated)
