@@ -269,9 +269,11 @@ int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
  * 620-625).  "waves_per_cu" (1..64, default 16): waves the grid-stride
  * kernels launch per compute unit.  "ms_unroll" (4 or 8, default 8):
  * neighbour distance rows in flight per wave in multi-source SSSP.
- * "orphan_repair" (0 or 1, default 1): a decremental BFS/SSSP call on a
+ * "orphan_repair" (0 or 1, default 0): a decremental BFS/SSSP call on a
  * graph with the in-edge mirror first offers each vertex whose tree edge
- * was deleted another parent at the same distance, sparing its subtree.
+ * was deleted another parent at the same distance, sparing its subtree
+ * (pays on unweighted graphs with small batches; on RMAT-24 batches of
+ * 100k+ edges the extra pass costs more than the shorter walk saves).
  * The Python autotuner (meerkat_amd/autotune.py) sets them from
  * measurements. */
 int mk_tune_set(const char *name, int64_t value);

@@ -51,7 +51,7 @@ static std::mutex g_tune_mu;
 static std::map<std::string, int64_t> g_tune = {
     {"waves_per_cu", 16},   // resident waves per CU the grid-stride kernels ask for
     {"ms_unroll", 8},       // neighbour rows in flight per wave in multi-source SSSP
-    {"orphan_repair", 1},   // decremental with the in-edge mirror re-parents orphans first
+    {"orphan_repair", 0},   // decremental with the in-edge mirror re-parents orphans first
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {

@@ -931,7 +931,7 @@ struct Traversal {
   uint64_t last_slabs = 0;
   uint64_t last_edges = 0;
   int max_waves = 0;
-  bool repair = true;  // orphan repair before invalidation (with the in-edge mirror)
+  bool repair = false;  // orphan repair before invalidation (with the in-edge mirror)
 
   TravView view() const {
     TravView v;
@@ -977,7 +977,7 @@ struct Traversal {
     k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
     MK_HIP(hipStreamSynchronize(stream));
     max_waves = grid_waves(g->device);
-    repair = tune_get("orphan_repair", 1) != 0;
+    repair = tune_get("orphan_repair", 0) != 0;
     return MK_OK;
   }
 

@@ -95,7 +95,7 @@ def test_tune_knobs_validate_without_gpu():
     from meerkat_amd import autotune
 
     before = autotune.knobs()
-    assert before == {"waves_per_cu": 16, "ms_unroll": 8, "orphan_repair": 1}
+    assert before == {"waves_per_cu": 16, "ms_unroll": 8, "orphan_repair": 0}
     try:
         autotune.set_knob("waves_per_cu", 32)
         assert autotune.get_knob("waves_per_cu") == 32
