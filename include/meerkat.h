@@ -274,6 +274,9 @@ int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
  * was deleted another parent at the same distance, sparing its subtree
  * (pays on unweighted graphs with small batches; on RMAT-24 batches of
  * 100k+ edges the extra pass costs more than the shorter walk saves).
+ * "persistent_steps" (0 or 1, default 1): BFS/SSSP frontier steps run in
+ * one cooperative launch with a grid barrier between steps (0: one launch
+ * per step, the host polling the frontier counters).
  * The Python autotuner (meerkat_amd/autotune.py) sets them from
  * measurements. */
 int mk_tune_set(const char *name, int64_t value);

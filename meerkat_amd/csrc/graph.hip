@@ -52,6 +52,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"waves_per_cu", 16},   // resident waves per CU the grid-stride kernels ask for
     {"ms_unroll", 8},       // neighbour rows in flight per wave in multi-source SSSP
     {"orphan_repair", 0},   // decremental with the in-edge mirror re-parents orphans first
+    {"persistent_steps", 1},  // BFS/SSSP frontier steps in one launch with grid barriers
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {
@@ -436,8 +437,8 @@ int mk_tune_set(const char *name, int64_t value) {
     return fail(MK_ERR_BAD_ARG, "waves_per_cu must be in [1, 64]");
   if (k == "ms_unroll" && value != 4 && value != 8)
     return fail(MK_ERR_BAD_ARG, "ms_unroll must be 4 or 8");
-  if (k == "orphan_repair" && value != 0 && value != 1)
-    return fail(MK_ERR_BAD_ARG, "orphan_repair must be 0 or 1");
+  if ((k == "orphan_repair" || k == "persistent_steps") && value != 0 && value != 1)
+    return fail(MK_ERR_BAD_ARG, k + " must be 0 or 1");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(k);
   if (it == g_tune.end()) return fail(MK_ERR_BAD_ARG, "mk_tune_set: unknown knob " + k);

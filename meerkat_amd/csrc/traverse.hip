@@ -59,6 +59,10 @@ struct Counts {
   uint32_t small[3];
   uint32_t steps;                 // steps that found a non-empty frontier
   uint32_t ninv;                  // entries of the invalidated-vertex list
+  uint32_t bar_count;             // grid barrier of k_expand_loop: arrivals
+  uint32_t bar_gen;               //   and generation
+  uint32_t abort;                 //   set when a barrier wait timed out
+  uint32_t loop_end;              // step index at which k_expand_loop stopped
   uint32_t pad;
   unsigned long long large[3];    // (entries << 32) | bucket items
   unsigned long long stats[kStatSlots][4];
@@ -68,6 +72,7 @@ struct Counts {
 struct Totals {
   uint32_t small[3];
   uint32_t steps;
+  uint32_t abort, loop_end;
   unsigned long long large[3];
   unsigned long long slabs, edges, visited, pushed;
 };
@@ -402,9 +407,14 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
 // One frontier step: read queue side `side`, counter slot `slot`; append to
 // side^1 / slot+1; clear slot+2.  MODE 0 relaxes, MODE 1 invalidates.  A
 // small frontier without hub entries is run in tail mode by block 0.
+// Called by every thread of every block of the grid; `lds` holds
+// kExpandLds words.
+constexpr uint32_t kExpandLds =
+    2 * kTailCap + 4 > (kBlock / 64) * kWaveQ ? 2 * kTailCap + 4 : (kBlock / 64) * kWaveQ;
+
 template <bool MAP, bool WEIGHTS, int MODE>
-__global__ void __launch_bounds__(kBlock)
-    k_expand(GraphView G, TravView S, uint32_t side, uint32_t slot, uint32_t epoch) {
+MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                        uint32_t epoch, uint32_t *lds) {
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
@@ -420,9 +430,6 @@ __global__ void __launch_bounds__(kBlock)
     if (n_small + n_large) atomicAdd(&S.cnt->steps, 1u);
   }
   if (n_small + n_large == 0) return;
-  constexpr uint32_t kLdsWords =
-      2 * kTailCap + 4 > (kBlock / 64) * kWaveQ ? 2 * kTailCap + 4 : (kBlock / 64) * kWaveQ;
-  __shared__ uint32_t lds[kLdsWords];
   if (n_large == 0 && n_small <= kTailEnter) {
     if (blockIdx.x == 0) expand_tail<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, n_small, lds);
     return;
@@ -465,6 +472,81 @@ __global__ void __launch_bounds__(kBlock)
     add_stat(S.cnt, kStatVisited, visited);
     add_stat(S.cnt, kStatPushed, q.pushed);
   }
+}
+
+template <bool MAP, bool WEIGHTS, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_expand(GraphView G, TravView S, uint32_t side, uint32_t slot, uint32_t epoch) {
+  __shared__ uint32_t lds[kExpandLds];
+  expand_step<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, lds);
+}
+
+// Grid-wide barrier of a kernel whose blocks are all resident (a
+// cooperative launch).  Thread 0 of each block publishes the block's writes
+// (agent-scope fence), counts its arrival, and waits for the generation to
+// move on; the last block to arrive resets the count and advances the
+// generation.  A wait longer than `timeout` wall-clock ticks sets
+// cnt->abort instead (so every wave still exits); returns false once abort
+// is set.  `gen` is the generation the block last saw (thread 0's copy).
+MK_DEV bool grid_barrier(Counts *c, uint32_t &gen, uint64_t timeout, uint32_t *flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t my = gen;
+    bool ok = true;
+    if (atomicAdd(&c->bar_count, 1u) == gridDim.x - 1) {
+      __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&c->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const uint64_t t0 = wall_clock64();
+      // relaxed polls; the fence below is the acquire
+      while (__hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == my) {
+        __builtin_amdgcn_s_sleep(4);
+        if (wall_clock64() - t0 > timeout) {
+          atomicExch(&c->abort, 1u);
+          ok = false;
+          break;
+        }
+      }
+    }
+    if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = false;
+    __threadfence();
+    gen = my + 1;
+    *flag = ok ? 1u : 0u;
+  }
+  __syncthreads();
+  return *flag != 0u;
+}
+
+// Frontier steps j0, j0 + 1, ... < j_end in one launch (all blocks
+// resident), a grid barrier between steps: step j reads queue side j & 1,
+// counter slot j % 3, as k_expand does.  Stops before the first step whose
+// frontier is empty and records its index in cnt->loop_end (j_end when the
+// range ran out first).  MODE 0 step j uses epochs epoch0 + (j - j0) *
+// kEpochStride onwards.
+template <bool MAP, bool WEIGHTS, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
+                  uint64_t timeout) {
+  __shared__ uint32_t lds[kExpandLds];
+  __shared__ uint32_t flag;
+  uint32_t gen = 0;
+  if (threadIdx.x == 0)
+    gen = __hip_atomic_load(&S.cnt->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t j = j0;
+  for (; j < j_end; ++j) {
+    const uint32_t slot = j % 3;
+    // every block reads the same settled counts: the uniform stop decision
+    const uint32_t ns =
+        __hip_atomic_load(&S.cnt->small[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long nl =
+        __hip_atomic_load(&S.cnt->large[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ns == 0 && (nl >> 32) == 0) break;
+    expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot,
+                                    MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
+    if (!grid_barrier(S.cnt, gen, timeout, &flag)) return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) S.cnt->loop_end = j;
 }
 
 // Re-seed invalidated vertices from valid in-neighbours: waves stream over
@@ -932,6 +1014,9 @@ struct Traversal {
   uint64_t last_edges = 0;
   int max_waves = 0;
   bool repair = false;  // orphan repair before invalidation (with the in-edge mirror)
+  bool persistent = true;        // frontier steps in one launch (k_expand_loop)
+  unsigned loop_blocks[2] = {0, 0};  // resident-grid size per MODE
+  uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
 
   TravView view() const {
     TravView v;
@@ -978,6 +1063,16 @@ struct Traversal {
     MK_HIP(hipStreamSynchronize(stream));
     max_waves = grid_waves(g->device);
     repair = tune_get("orphan_repair", 0) != 0;
+    persistent = tune_get("persistent_steps", 1) != 0;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
+        khz <= 0)
+      khz = 100000;
+    barrier_timeout = uint64_t(khz) * 5000ull;  // five seconds
+    if (persistent) {
+      if (int r = loop_grid<0>(loop_blocks[0])) return r;
+      if (int r = loop_grid<1>(loop_blocks[1])) return r;
+    }
     return MK_OK;
   }
 
@@ -1006,13 +1101,52 @@ struct Traversal {
   // Epoch base of the next launch: kEpochStride values are reserved per
   // launch (a tail-mode launch uses one per inner step).  Stamps start at 0,
   // so epochs are never 0; on wrap-around the stamps are cleared.
-  uint32_t next_epoch() {
-    if (epoch > 0xFFFFFFFFu - 2 * kEpochStride) {
+  uint32_t next_epoch() { return reserve_epochs(1); }
+
+  // Base of `steps` consecutive epoch ranges of kEpochStride values each.
+  uint32_t reserve_epochs(uint32_t steps) {
+    const uint64_t need = uint64_t(steps + 1) * kEpochStride;
+    if (uint64_t(epoch) + need > 0xFFFFFFFFull) {
       fill_u32(stamp, 0u, g->n, stream);
       epoch = 0;
     }
-    epoch += kEpochStride;
-    return epoch;
+    const uint32_t base = epoch + kEpochStride;
+    epoch = base + (steps - 1) * kEpochStride;
+    return base;
+  }
+
+  template <bool MAP, bool WEIGHTS, int MODE> static const void *loop_kernel() {
+    return reinterpret_cast<const void *>(&k_expand_loop<MAP, WEIGHTS, MODE>);
+  }
+  template <int MODE> const void *loop_kernel_for() const {
+    return g->weighted ? (weights ? loop_kernel<true, true, MODE>()
+                                  : loop_kernel<true, false, MODE>())
+                       : loop_kernel<false, false, MODE>();
+  }
+
+  // Blocks of a k_expand_loop grid: the launch shape of k_expand, capped at
+  // what the device keeps resident at once (a grid barrier needs every block
+  // running).
+  template <int MODE> int loop_grid(unsigned &blocks) {
+    int per_cu = 0;
+    MK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, loop_kernel_for<MODE>(), kBlock,
+                                                        0));
+    const uint64_t resident = uint64_t(per_cu > 0 ? per_cu : 0) * uint64_t(device_cus(g->device));
+    if (resident == 0) return fail(MK_ERR_HIP, "frontier loop kernel cannot be resident");
+    blocks = unsigned(std::min<uint64_t>(blocks_for_waves(uint64_t(max_waves)), resident));
+    return MK_OK;
+  }
+
+  // One cooperative launch of k_expand_loop over steps [j0, j1).
+  template <int MODE> int launch_loop(uint32_t j0, uint32_t j1, uint32_t ep) {
+    GraphView G = g->view();
+    TravView S = view();
+    uint64_t to = barrier_timeout;
+    void *args[] = {&G, &S, &j0, &j1, &ep, &to};
+    ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
+    MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
+                                      dim3(kBlock), args, 0u, stream));
+    return MK_OK;
   }
 
   // Zero every counter (statistics included).
@@ -1038,6 +1172,8 @@ struct Traversal {
       h.large[i] = c.large[i];
     }
     h.steps = c.steps;
+    h.abort = c.abort;
+    h.loop_end = c.loop_end;
     h.slabs = h.edges = h.visited = h.pushed = 0;
     for (uint32_t k = 0; k < kStatSlots; ++k) {
       h.slabs += c.stats[k][kStatSlabs];
@@ -1058,6 +1194,35 @@ struct Traversal {
   }
 
   // Run frontier steps from queue side 0 / slot 0 until the frontier is
+  // empty, kLoopSteps steps per k_expand_loop launch (a further launch only
+  // when a launch used up its range); one host wait per launch.
+  // Algorithmic bytes as in iterate.
+  static constexpr uint32_t kLoopSteps = 256;
+  template <int MODE> int iterate_loop(Totals &h) {
+    Totals prev;
+    if (profiling_on()) {
+      if (int r = read_counts(prev)) return r;
+    }
+    const uint64_t limit = uint64_t(g->n) + 8;
+    for (uint32_t j0 = 0;; j0 += kLoopSteps) {
+      if (j0 > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
+      const uint32_t ep = MODE == 0 ? reserve_epochs(kLoopSteps) : 0u;
+      if (int r = launch_loop<MODE>(j0, j0 + kLoopSteps, ep)) return r;
+      MK_HIP(hipGetLastError());
+      if (int r = read_counts(h)) return r;
+      if (h.abort) return fail(MK_ERR_STATE, "frontier loop: a grid barrier wait timed out");
+      if (h.loop_end < j0 + kLoopSteps) break;
+    }
+    if (profiling_on()) {
+      const double bytes = 256.0 * double(h.slabs - prev.slabs) + 8.0 * double(h.edges - prev.edges) +
+                           24.0 * double(h.visited - prev.visited) +
+                           16.0 * double(h.pushed - prev.pushed);
+      prof_add_bytes(MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", bytes);
+    }
+    return MK_OK;
+  }
+
+  // Run frontier steps from queue side 0 / slot 0 until the frontier is
   // empty.  Steps are launched in groups without waiting; after each group
   // the counters are copied to a pinned snapshot.  The host queues the next
   // group before it waits for the previous group's snapshot, so the device
@@ -1067,6 +1232,7 @@ struct Traversal {
   // expanded vertex (entry, bucket range, distance), 16 B per push (atomic,
   // stamp, entry).
   template <int MODE> int iterate(Totals &h) {
+    if (persistent) return iterate_loop<MODE>(h);
     uint32_t side = 0, slot = 0;
     uint64_t launched = 0;
     const uint64_t limit = uint64_t(g->n) + 8;

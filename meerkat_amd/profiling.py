@@ -118,6 +118,8 @@ def pmc_per_kernel(path, counter, region=None):
 SCOPE_OF = [
     ("k_expand<true, true, 0>", "k_expand_relax"),
     ("k_expand<true, true, 1>", "k_expand_invalidate"),
+    ("k_expand_loop<true, true, 0>", "k_expand_relax"),
+    ("k_expand_loop<true, true, 1>", "k_expand_invalidate"),
     ("k_insert<", "k_insert"),
     ("k_delete<", "k_delete"),
     ("k_seed_batch<", "k_seed_batch"),

@@ -23,7 +23,7 @@ def check(algo, n, s, d, w=None, source=0):
 
 
 @pytest.mark.parametrize("weighted", [False, True])
-def test_long_path_static_decremental_incremental(gpu, weighted):
+def test_long_path_static_decremental_incremental(gpu, weighted, persistent):
     # 1000 levels: every step's frontier is one vertex, so the tail runs
     # ~254 steps per launch and hands over when its epoch range runs out
     n = 1000
@@ -87,7 +87,16 @@ def test_broom_spill_and_hub_handover(gpu, hints):
     check(b, n, s, d)
 
 
-def test_small_batches_on_rmat_stay_exact(gpu, repair):
+@pytest.fixture(params=[1, 0], ids=["one_launch", "launch_per_step"])
+def persistent(request):
+    from meerkat_amd import autotune
+    before = autotune.get_knob("persistent_steps")
+    autotune.set_knob("persistent_steps", request.param)
+    yield request.param
+    autotune.set_knob("persistent_steps", before)
+
+
+def test_small_batches_on_rmat_stay_exact(gpu, repair, persistent):
     # small update batches give small frontiers (tail mode) on a skewed graph
     n, s, d, w = gen.rmat(14, 8, seed=41, weighted=True)
     pool = mk.SlabPool(1 << 27)
