@@ -59,8 +59,7 @@ struct Counts {
   uint32_t small[3];
   uint32_t steps;                 // steps that found a non-empty frontier
   uint32_t ninv;                  // entries of the invalidated-vertex list
-  uint32_t bar_count;             // grid barrier of k_expand_loop: arrivals
-  uint32_t bar_gen;               //   and generation
+  uint32_t bar_count;             // grid-barrier arrivals in k_expand_loop
   uint32_t abort;                 //   set when a barrier wait timed out
   uint32_t loop_end;              // step index at which k_expand_loop stopped
   uint32_t pad;
@@ -89,13 +88,29 @@ struct WaveQ {
   uint32_t pushed;  // wave-uniform statistics
 };
 
+// Frontier queue entries and counters cross the steps of one k_expand_loop
+// launch, between blocks on different XCDs (each with its own L2): they are
+// written and read with agent-scope coherent accesses, so a grid barrier
+// needs no L2 write-back or invalidation.
+MK_DEV void qstore(uint32_t *p, uint32_t v) { coherent_store(p, v); }
+MK_DEV void qstore2(uint2 *p, uint2 v) {
+  __hip_atomic_store(reinterpret_cast<uint64_t *>(p), (uint64_t(v.y) << 32) | v.x,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MK_DEV uint32_t qload(const uint32_t *p) { return coherent_load(p); }
+MK_DEV uint2 qload2(const uint2 *p) {
+  const uint64_t x = coherent_load64(reinterpret_cast<const uint64_t *>(p));
+  return make_uint2(uint32_t(x), uint32_t(x >> 32));
+}
+MK_DEV uint32_t qload2_y(const uint2 *p) { return qload(reinterpret_cast<const uint32_t *>(p) + 1); }
+
 MK_DEV void wq_flush(WaveQ &q, uint32_t *queue, uint32_t *count) {
   if (q.n == 0) return;
   __builtin_amdgcn_wave_barrier();
   uint32_t base = 0;
   if (lane_id() == 0) base = atomicAdd(count, q.n);
   base = read_lane(base, 0);
-  for (uint32_t i = lane_id(); i < q.n; i += 64) queue[base + i] = q.buf[i];
+  for (uint32_t i = lane_id(); i < q.n; i += 64) qstore(queue + base + i, q.buf[i]);
   __builtin_amdgcn_wave_barrier();
   q.n = 0;
 }
@@ -127,7 +142,7 @@ MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t si
   q.n += c;
   if (push && !small) {
     const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
-    S.q_large[side][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+    qstore2(S.q_large[side] + uint32_t(r >> 32), make_uint2(v, uint32_t(r)));
   }
   q.pushed += uint32_t(__popcll(pm));
 }
@@ -182,14 +197,14 @@ struct TailSink {
         if (small) {
           const uint32_t pos = base + lanes_below(sm);
           if (pos < kTailCap) lq[pos] = kEmpty;
-          S.q_small[side][gb + lanes_below(sm)] = v;
+          qstore(S.q_small[side] + gb + lanes_below(sm), v);
         }
         if (lane == leader) ctl[1] = 1u;
       }
     }
     if (push && !small) {  // hubs go to the next step's bucket work list
       const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
-      S.q_large[side][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+      qstore2(S.q_large[side] + uint32_t(r >> 32), make_uint2(v, uint32_t(r)));
       ctl[1] = 1u;
     }
     pushed += uint32_t(__popcll(pm));
@@ -330,10 +345,10 @@ MK_DEV void expand_rows(const GraphView &G, const TravView &S, Sink &sink, uint3
     bool act = lane < per && i < n;
     uint32_t u = 0, b0 = 0, nb = 0, du = 0;
     if (act) {
-      u = queue[i];
+      u = qload(queue + i);
       b0 = G.bucket_off[u];
       nb = G.bucket_off[u + 1] - b0;
-      du = MODE == 1 ? 0u : dist_of(S.dist[u]);
+      du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
       if (MODE == 0 && du == kInf) act = false;
     }
     visited += uint32_t(__popcll(ballot(act)));
@@ -358,7 +373,7 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
   const uint32_t nside = side ^ 1u, nslot = slot == 2 ? 0 : slot + 1;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, NW = blockDim.x >> 6;
   if (tid < 4) ctl[tid] = 0u;
-  for (uint32_t i = tid; i < n0; i += blockDim.x) qa[i] = S.q_small[side][i];
+  for (uint32_t i = tid; i < n0; i += blockDim.x) qa[i] = qload(S.q_small[side] + i);
   uint32_t nin = n0, slabs = 0, edges = 0, visited = 0, pushed = 0;
   for (uint32_t j = 0;; ++j) {
     __syncthreads();  // the previous step is complete everywhere; qa holds its frontier
@@ -386,7 +401,7 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(&S.cnt->small[nslot], uint32_t(__popcll(vm)));
         base = read_lane(base, leader);
-        if (v != kEmpty) S.q_small[nside][base + lanes_below(vm)] = v;
+        if (v != kEmpty) qstore(S.q_small[nside] + base + lanes_below(vm), v);
       }
       break;
     }
@@ -421,12 +436,13 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
   const uint32_t nslot = slot == 2 ? 0 : slot + 1;
   const uint32_t cslot = nslot == 2 ? 0 : nslot + 1;
   const uint32_t nside = side ^ 1u;
-  const uint32_t n_small = S.cnt->small[slot];
-  const unsigned long long lg = S.cnt->large[slot];
+  const uint32_t n_small = coherent_load(&S.cnt->small[slot]);
+  const unsigned long long lg = coherent_load64(reinterpret_cast<const uint64_t *>(&S.cnt->large[slot]));
   const uint32_t n_large = uint32_t(lg >> 32), large_items = uint32_t(lg);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.cnt->small[cslot] = 0;
-    S.cnt->large[cslot] = 0ull;
+    coherent_store(&S.cnt->small[cslot], 0u);
+    __hip_atomic_store(reinterpret_cast<uint64_t *>(&S.cnt->large[cslot]), uint64_t(0),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (n_small + n_large) atomicAdd(&S.cnt->steps, 1u);
   }
   if (n_small + n_large == 0) return;
@@ -443,7 +459,7 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
     uint32_t lo = 0, hi = n_large;  // last entry with start <= t0
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (ql[mid].y <= t0) lo = mid;
+      if (qload2_y(ql + mid) <= t0) lo = mid;
       else hi = mid;
     }
     // lane k < chunk takes work item t0 + k
@@ -451,13 +467,13 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
     bool act = lane < kLargeChunk && t < large_items;
     uint32_t e = lo;
     if (act)
-      while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
+      while (e + 1 < n_large && qload2_y(ql + e + 1) <= t) ++e;
     uint32_t u = 0, b = 0, du = 0;
     if (act) {
-      const uint2 ent = ql[e];
+      const uint2 ent = qload2(ql + e);
       u = ent.x;
       b = G.bucket_off[u] + (t - ent.y);
-      du = MODE == 1 ? 0u : dist_of(S.dist[u]);
+      du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
       if (MODE == 0 && du == kInf) act = false;
     }
     batched_buckets<MAP, WEIGHTS, MODE>(G, S, sink, epoch, ballot(act), b, u, du, slabs, edges);
@@ -482,25 +498,22 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Grid-wide barrier of a kernel whose blocks are all resident (a
-// cooperative launch).  Thread 0 of each block publishes the block's writes
-// (agent-scope fence), counts its arrival, and waits for the generation to
-// move on; the last block to arrive resets the count and advances the
-// generation.  A wait longer than `timeout` wall-clock ticks sets
-// cnt->abort instead (so every wave still exits); returns false once abort
-// is set.  `gen` is the generation the block last saw (thread 0's copy).
-MK_DEV bool grid_barrier(Counts *c, uint32_t &gen, uint64_t timeout, uint32_t *flag) {
+// cooperative launch).  Every wave first waits for its own memory
+// operations to complete (the data that crosses steps is written with
+// coherent stores and atomics, so complete means visible to every XCD);
+// thread 0 of each block then counts the block's arrival on cnt->bar_count
+// (zero at launch) and waits until it reaches `target`, the number of
+// arrivals that completes this barrier.  A wait longer than `timeout`
+// wall-clock ticks sets cnt->abort instead (so every wave still exits);
+// returns false once abort is set.
+MK_DEV bool grid_barrier(Counts *c, uint32_t target, uint64_t timeout, uint32_t *flag) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t my = gen;
     bool ok = true;
-    if (atomicAdd(&c->bar_count, 1u) == gridDim.x - 1) {
-      __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&c->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
+    if (atomicAdd(&c->bar_count, 1u) + 1u < target) {
       const uint64_t t0 = wall_clock64();
-      // relaxed polls; the fence below is the acquire
-      while (__hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == my) {
+      while (coherent_load(&c->bar_count) < target) {
         __builtin_amdgcn_s_sleep(4);
         if (wall_clock64() - t0 > timeout) {
           atomicExch(&c->abort, 1u);
@@ -509,9 +522,7 @@ MK_DEV bool grid_barrier(Counts *c, uint32_t &gen, uint64_t timeout, uint32_t *f
         }
       }
     }
-    if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = false;
-    __threadfence();
-    gen = my + 1;
+    if (coherent_load(&c->abort)) ok = false;
     *flag = ok ? 1u : 0u;
   }
   __syncthreads();
@@ -519,32 +530,27 @@ MK_DEV bool grid_barrier(Counts *c, uint32_t &gen, uint64_t timeout, uint32_t *f
 }
 
 // Frontier steps j0, j0 + 1, ... < j_end in one launch (all blocks
-// resident), a grid barrier between steps: step j reads queue side j & 1,
-// counter slot j % 3, as k_expand does.  Stops before the first step whose
-// frontier is empty and records its index in cnt->loop_end (j_end when the
-// range ran out first).  MODE 0 step j uses epochs epoch0 + (j - j0) *
-// kEpochStride onwards.
+// resident, cnt->bar_count zero), a grid barrier between steps: step j
+// reads queue side j & 1, counter slot j % 3, as k_expand does.  Stops
+// before the first step whose frontier is empty and records its index in
+// cnt->loop_end (j_end when the range ran out first).  MODE 0 step j uses
+// epochs epoch0 + (j - j0) * kEpochStride onwards.
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
                   uint64_t timeout) {
   __shared__ uint32_t lds[kExpandLds];
   __shared__ uint32_t flag;
-  uint32_t gen = 0;
-  if (threadIdx.x == 0)
-    gen = __hip_atomic_load(&S.cnt->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t j = j0;
   for (; j < j_end; ++j) {
     const uint32_t slot = j % 3;
     // every block reads the same settled counts: the uniform stop decision
-    const uint32_t ns =
-        __hip_atomic_load(&S.cnt->small[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long nl =
-        __hip_atomic_load(&S.cnt->large[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t ns = coherent_load(&S.cnt->small[slot]);
+    const uint64_t nl = coherent_load64(reinterpret_cast<const uint64_t *>(&S.cnt->large[slot]));
     if (ns == 0 && (nl >> 32) == 0) break;
     expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot,
                                     MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
-    if (!grid_barrier(S.cnt, gen, timeout, &flag)) return;
+    if (!grid_barrier(S.cnt, (j - j0 + 1) * gridDim.x, timeout, &flag)) return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) S.cnt->loop_end = j;
 }
@@ -1143,6 +1149,7 @@ struct Traversal {
     TravView S = view();
     uint64_t to = barrier_timeout;
     void *args[] = {&G, &S, &j0, &j1, &ep, &to};
+    fill_u32(&cnt->bar_count, 0u, 1, stream);
     ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
     MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
                                       dim3(kBlock), args, 0u, stream));
