@@ -169,7 +169,7 @@ MK_DEV MutSide pick_side(bool second, const MutSide &A, const MutSide &B) {
 // Waves [0, waves_a) apply the batch to side A, the rest to side B, so the
 // forward graph and its mirror are updated by one launch with their
 // (latency-bound) chain walks overlapping.  Each wave takes `epw` (<= 64)
-// consecutive edges, lane i holding edge i; the four 16-lane groups of the
+// consecutive edges, lane i holding edge i; the eight 8-lane groups of the
 // wave apply their own lanes' edges one after another, each group walking
 // and claiming in its own slab chain (the reference runs one edge at a time
 // per warp, include/Graph/warp/InsertEdge.cuh:66-103).
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   const GraphView &G = M.G;
   uint32_t pending = group_bits(ballot(act));
-  uint32_t applied = 0, pool_fail = 0;
+  uint32_t pool_fail = 0;
   const uint32_t seed_base = (second ? 0x9E3779B9u : 0u) + wave * 64u + group_first_lane();
   while (pending) {
     const uint32_t j = ffs32(pending);
@@ -222,20 +222,14 @@ __global__ void __launch_bounds__(kBlock)
       else if (r < 0) st = MK_EDGE_POOL_FULL;  // rolled back by the host
     }
     if (r == 1) {
-      ++applied;
       if (gl == 0 && M.degrees) atomicAdd(G.vdeg + sj, 1u);
     } else if (r < 0) {
       ++pool_fail;
     }
   }
   if (lane < cnt) M.status[base + lane] = st;
-  // group totals (held by every lane of a group) summed over the 4 groups
-  applied = wave_sum_u32(gl == 0 ? applied : 0u);
-  pool_fail = wave_sum_u32(gl == 0 ? pool_fail : 0u);
-  if (lane == 0) {
-    if (applied) atomicAdd(M.counters + 2, applied);
-    if (pool_fail) atomicAdd(M.counters + 1, pool_fail);
-  }
+  // pool failures (rare): one atomic per group that had any
+  if (gl == 0 && pool_fail) atomicAdd(M.counters + 1, pool_fail);
 }
 
 // Delete the batch edges whose status equals `only_status` (all edges when
@@ -281,25 +275,19 @@ __global__ void __launch_bounds__(kBlock)
   }
   const GraphView &G = M.G;
   uint32_t pending = group_bits(ballot(act));
-  uint32_t applied = 0;
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     const uint32_t sj = group_read(s, j), dj = group_read(d, j);
     const int r = slab_delete_group<MAP>(G, sj, dj);
     if (gl == j && r == 1) st = MK_EDGE_APPLIED;
-    if (r == 1) {
-      ++applied;
-      if (gl == 0 && M.degrees) atomicSub(G.vdeg + sj, 1u);
-    }
+    if (r == 1 && gl == 0 && M.degrees) atomicSub(G.vdeg + sj, 1u);
   }
   if (only_status != 0xFFu) {
     if (lane < cnt && act) M.status[base + lane] = MK_EDGE_UNCHANGED;
   } else if (lane < cnt) {
     M.status[base + lane] = st;
   }
-  applied = wave_sum_u32(gl == 0 ? applied : 0u);
-  if (lane == 0 && applied) atomicAdd(M.counters + 2, applied);
 }
 
 template <bool MAP>

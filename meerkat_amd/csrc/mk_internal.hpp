@@ -220,7 +220,7 @@ struct mk_graph {
   uint32_t *bucket_off = nullptr;
   uint32_t *bucket_owner = nullptr;
   uint32_t *vdeg = nullptr;
-  uint32_t *counters = nullptr;  // [0] range errors [1] pool failures [2] applied
+  uint32_t *counters = nullptr;  // [0] range errors [1] pool failures
   std::vector<uint32_t> bucket_off_host;
   mk::DevBuf status;             // uint8 per edge of the last batch
   uint32_t status_n = 0;

@@ -59,8 +59,7 @@ struct Counts {
   uint32_t small[3];
   uint32_t steps;                 // steps that found a non-empty frontier
   uint32_t ninv;                  // entries of the invalidated-vertex list
-  uint32_t bar_count;             // grid-barrier arrivals in k_expand_loop
-  uint32_t abort;                 //   set when a barrier wait timed out
+  uint32_t abort;                 // set when a grid-barrier wait timed out
   uint32_t loop_end;              // step index at which k_expand_loop stopped
   uint32_t pad;
   unsigned long long large[3];    // (entries << 32) | bucket items
@@ -115,6 +114,21 @@ MK_DEV void wq_flush(WaveQ &q, uint32_t *queue, uint32_t *count) {
   q.n = 0;
 }
 
+// Grid-barrier state of k_expand_loop, one 128-byte line per counter so
+// that arrivals on different counters do not serialise: blocks arrive in
+// groups of kBarGroup on their group's line; the last of a group arrives on
+// the top line and, once all groups are in, writes the barrier's number to
+// its group's release line.  Counts only grow (zeroed with the frontier
+// counters at the start of a compute call); barrier k is complete when the
+// top count reaches k x groups.
+constexpr uint32_t kBarGroup = 32;
+constexpr uint32_t kMaxBarGroups = 128;  // 4096 blocks
+struct BarLines {
+  uint32_t top[32];
+  uint32_t grp[kMaxBarGroups][32];
+  uint32_t rel[kMaxBarGroups][32];
+};
+
 struct TravView {
   uint64_t *dist;
   uint32_t *stamp;
@@ -123,6 +137,7 @@ struct TravView {
   uint32_t *q_small[2];
   uint2 *q_large[2];
   Counts *cnt;
+  struct BarLines *bar;  // grid-barrier counters (k_expand_loop)
 };
 
 // Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
@@ -497,30 +512,40 @@ __global__ void __launch_bounds__(kBlock)
   expand_step<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, lds);
 }
 
-// Grid-wide barrier of a kernel whose blocks are all resident (a
-// cooperative launch).  Every wave first waits for its own memory
-// operations to complete (the data that crosses steps is written with
-// coherent stores and atomics, so complete means visible to every XCD);
-// thread 0 of each block then counts the block's arrival on cnt->bar_count
-// (zero at launch) and waits until it reaches `target`, the number of
-// arrivals that completes this barrier.  A wait longer than `timeout`
-// wall-clock ticks sets cnt->abort instead (so every wave still exits);
-// returns false once abort is set.
-MK_DEV bool grid_barrier(Counts *c, uint32_t target, uint64_t timeout, uint32_t *flag) {
+// Grid-wide barrier number `k` (1, 2, ... since the counters were zeroed)
+// of a kernel whose blocks are all resident (a cooperative launch).  Every
+// wave first waits for its own memory operations to complete (the data that
+// crosses steps is written with coherent stores and atomics, so complete
+// means visible to every XCD); thread 0 of each block then arrives as
+// BarLines describes.  A wait longer than `timeout` wall-clock ticks sets
+// cnt->abort instead (so every wave still exits); returns false once abort
+// is set.
+MK_DEV bool barrier_wait(const uint32_t *p, uint32_t target, uint64_t timeout, Counts *c) {
+  const uint64_t t0 = wall_clock64();
+  while (coherent_load(p) < target) {
+    __builtin_amdgcn_s_sleep(2);
+    if (wall_clock64() - t0 > timeout) {
+      atomicExch(&c->abort, 1u);
+      return false;
+    }
+  }
+  return true;
+}
+
+MK_DEV bool grid_barrier(BarLines *bl, Counts *c, uint32_t k, uint64_t timeout, uint32_t *flag) {
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   if (threadIdx.x == 0) {
+    const uint32_t g = blockIdx.x / kBarGroup;
+    const uint32_t ng = (gridDim.x + kBarGroup - 1) / kBarGroup;
+    const uint32_t gs = min(kBarGroup, gridDim.x - g * kBarGroup);
     bool ok = true;
-    if (atomicAdd(&c->bar_count, 1u) + 1u < target) {
-      const uint64_t t0 = wall_clock64();
-      while (coherent_load(&c->bar_count) < target) {
-        __builtin_amdgcn_s_sleep(4);
-        if (wall_clock64() - t0 > timeout) {
-          atomicExch(&c->abort, 1u);
-          ok = false;
-          break;
-        }
-      }
+    if (atomicAdd(&bl->grp[g][0], 1u) + 1u == k * gs) {
+      // last of the group: wait for every group, then release this one
+      if (atomicAdd(&bl->top[0], 1u) + 1u < k * ng) ok = barrier_wait(bl->top, k * ng, timeout, c);
+      coherent_store(&bl->rel[g][0], k);
+    } else {
+      ok = barrier_wait(bl->rel[g], k, timeout, c);
     }
     if (coherent_load(&c->abort)) ok = false;
     *flag = ok ? 1u : 0u;
@@ -530,15 +555,16 @@ MK_DEV bool grid_barrier(Counts *c, uint32_t target, uint64_t timeout, uint32_t 
 }
 
 // Frontier steps j0, j0 + 1, ... < j_end in one launch (all blocks
-// resident, cnt->bar_count zero), a grid barrier between steps: step j
-// reads queue side j & 1, counter slot j % 3, as k_expand does.  Stops
-// before the first step whose frontier is empty and records its index in
-// cnt->loop_end (j_end when the range ran out first).  MODE 0 step j uses
-// epochs epoch0 + (j - j0) * kEpochStride onwards.
+// resident), a grid barrier between steps: step j reads queue side j & 1,
+// counter slot j % 3, as k_expand does.  Stops before the first step whose
+// frontier is empty and records its index in cnt->loop_end (j_end when the
+// range ran out first).  MODE 0 step j uses epochs epoch0 + (j - j0) *
+// kEpochStride onwards.  `bdone` grid barriers have completed since the
+// barrier counters were zeroed.
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
-                  uint64_t timeout) {
+                  uint32_t bdone, uint64_t timeout) {
   __shared__ uint32_t lds[kExpandLds];
   __shared__ uint32_t flag;
   uint32_t j = j0;
@@ -550,7 +576,7 @@ __global__ void __launch_bounds__(kBlock)
     if (ns == 0 && (nl >> 32) == 0) break;
     expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot,
                                     MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
-    if (!grid_barrier(S.cnt, (j - j0 + 1) * gridDim.x, timeout, &flag)) return;
+    if (!grid_barrier(S.bar, S.cnt, bdone + (j - j0) + 1, timeout, &flag)) return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) S.cnt->loop_end = j;
 }
@@ -1010,7 +1036,9 @@ struct Traversal {
   uint32_t *inv_list = nullptr;
   uint32_t *qs[2] = {nullptr, nullptr};
   uint2 *ql[2] = {nullptr, nullptr};
-  Counts *cnt = nullptr;  // device
+  Counts *cnt = nullptr;  // device; BarLines follow at bar_offset
+  BarLines *bar = nullptr;
+  uint32_t bar_done = 0;  // grid barriers completed since the counters were zeroed
   HostBuf host;           // pinned Counts
   HostBuf snap[2];        // pinned Counts snapshots of successive step groups
   hipEvent_t snap_ev[2] = {nullptr, nullptr};
@@ -1035,6 +1063,7 @@ struct Traversal {
     v.q_large[0] = ql[0];
     v.q_large[1] = ql[1];
     v.cnt = cnt;
+    v.bar = bar;
     return v;
   }
 
@@ -1055,7 +1084,8 @@ struct Traversal {
       MK_HIP(hipMalloc(&qs[i], n * 4));
       MK_HIP(hipMalloc(&ql[i], n * 8));
     }
-    MK_HIP(hipMalloc(&cnt, sizeof(Counts)));
+    MK_HIP(hipMalloc(&cnt, bar_offset() + sizeof(BarLines)));
+    bar = reinterpret_cast<BarLines *>(reinterpret_cast<char *>(cnt) + bar_offset());
     if (int r = host.ensure(sizeof(Counts))) return r;
     for (int i = 0; i < 2; ++i) {
       if (int r = snap[i].ensure(sizeof(Counts))) return r;
@@ -1139,7 +1169,9 @@ struct Traversal {
                                                         0));
     const uint64_t resident = uint64_t(per_cu > 0 ? per_cu : 0) * uint64_t(device_cus(g->device));
     if (resident == 0) return fail(MK_ERR_HIP, "frontier loop kernel cannot be resident");
-    blocks = unsigned(std::min<uint64_t>(blocks_for_waves(uint64_t(max_waves)), resident));
+    blocks = unsigned(std::min<uint64_t>(
+        std::min<uint64_t>(blocks_for_waves(uint64_t(max_waves)), resident),
+        uint64_t(kMaxBarGroups) * kBarGroup));  // the barrier's group lines
     return MK_OK;
   }
 
@@ -1148,18 +1180,20 @@ struct Traversal {
     GraphView G = g->view();
     TravView S = view();
     uint64_t to = barrier_timeout;
-    void *args[] = {&G, &S, &j0, &j1, &ep, &to};
-    fill_u32(&cnt->bar_count, 0u, 1, stream);
+    uint32_t bd = bar_done;
+    void *args[] = {&G, &S, &j0, &j1, &ep, &bd, &to};
     ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
     MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
                                       dim3(kBlock), args, 0u, stream));
     return MK_OK;
   }
 
-  // Zero every counter (statistics included).
+  // Zero every counter (statistics and grid-barrier lines included).
   void clear_counts() {
-    fill_u32(reinterpret_cast<uint32_t *>(cnt), 0u, sizeof(Counts) / 4, stream);
+    fill_u32(reinterpret_cast<uint32_t *>(cnt), 0u, (bar_offset() + sizeof(BarLines)) / 4, stream);
+    bar_done = 0;
   }
+  static constexpr size_t bar_offset() { return (sizeof(Counts) + 127) / 128 * 128; }
   // Zero the frontier slots only.
   void clear_slots() {
     fill_u32(cnt->small, 0u, 3, stream);
@@ -1218,6 +1252,7 @@ struct Traversal {
       MK_HIP(hipGetLastError());
       if (int r = read_counts(h)) return r;
       if (h.abort) return fail(MK_ERR_STATE, "frontier loop: a grid barrier wait timed out");
+      bar_done += h.loop_end - j0;  // one barrier after each step run
       if (h.loop_end < j0 + kLoopSteps) break;
     }
     if (profiling_on()) {
