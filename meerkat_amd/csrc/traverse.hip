@@ -49,7 +49,7 @@ constexpr uint32_t kLargeChunk = 8;  // bucket work items per wave step
 constexpr uint32_t kBatch = 4;       // slab loads in flight per wave
 
 constexpr uint32_t kWaveQ = 512;     // LDS staging entries per wave
-constexpr uint32_t kStatSlots = 64;  // statistics spread over this many slots
+constexpr uint32_t kStatSlots = 128;  // statistics spread over this many 128-byte lines
 enum { kStatSlabs = 0, kStatEdges = 1, kStatVisited = 2, kStatPushed = 3 };
 
 // Device counters.  Frontier slots rotate per step; the statistics
@@ -63,7 +63,7 @@ struct Counts {
   uint32_t loop_end;              // step index at which k_expand_loop stopped
   uint32_t pad;
   unsigned long long large[3];    // (entries << 32) | bucket items
-  unsigned long long stats[kStatSlots][4];
+  unsigned long long stats[kStatSlots][16];  // [slot][which], one line per slot
 };
 
 // Host-side totals of a Counts snapshot.
@@ -140,6 +140,32 @@ struct TravView {
   struct BarLines *bar;  // grid-barrier counters (k_expand_loop)
 };
 
+// Append the hub vertices v (lanes with `hub`, nb buckets each) to hub
+// queue side `side`, counter slot `slot`: one 64-bit atomic per call claims
+// the entries and the bucket work items of all of them (an exclusive prefix
+// of nb over the lanes gives each its first item).  Wave-collective.
+MK_DEV void push_hubs(const TravView &S, uint32_t side, uint32_t slot, bool hub, uint32_t v,
+                      uint32_t nb) {
+  const uint64_t hm = ballot(hub);
+  if (hm == 0) return;
+  const uint32_t lane = lane_id();
+  uint32_t x = hub ? nb : 0u;  // inclusive prefix of nb over the wave
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  const uint32_t total = read_lane(x, 63);
+  const uint32_t leader = ffs64(hm);
+  unsigned long long r = 0;
+  if (lane == leader)
+    r = atomicAdd(&S.cnt->large[slot], (uint64_t(__popcll(hm)) << 32) | total);
+  const uint32_t first = read_lane(uint32_t(r >> 32), leader);
+  const uint32_t item0 = read_lane(uint32_t(r), leader);
+  if (hub)
+    qstore2(S.q_large[side] + first + lanes_below(hm), make_uint2(v, item0 + x - nb));
+}
+
 // Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
 // small vertices through the wave's LDS staging, hub vertices directly.
 // Wave-collective.
@@ -155,11 +181,30 @@ MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t si
   if (q.n + c > kWaveQ) wq_flush(q, S.q_small[side], &S.cnt->small[slot]);
   if (small) q.buf[q.n + lanes_below(sm)] = v;
   q.n += c;
-  if (push && !small) {
-    const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
-    qstore2(S.q_large[side] + uint32_t(r >> 32), make_uint2(v, uint32_t(r)));
-  }
+  push_hubs(S, side, slot, push && !small, v, nb);
   q.pushed += uint32_t(__popcll(pm));
+}
+
+// wq_flush for every wave of the block at once, one atomic per block;
+// `sh` holds kBlock / 64 + 1 words of LDS.  Called by every thread.
+MK_DEV void block_flush(WaveQ &q, uint32_t *queue, uint32_t *count, uint32_t *sh) {
+  constexpr uint32_t NW = kBlock / 64;
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 0) sh[w] = q.n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < NW; ++k) {
+      const uint32_t c = sh[k];
+      sh[k] = tot;
+      tot += c;
+    }
+    sh[NW] = tot ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  const uint32_t base = sh[NW] + sh[w];
+  for (uint32_t i = lane_id(); i < q.n; i += 64) qstore(queue + base + i, q.buf[i]);
+  q.n = 0;
 }
 
 // Push targets of the expansion helpers below (wave-collective calls).
@@ -217,11 +262,9 @@ struct TailSink {
         if (lane == leader) ctl[1] = 1u;
       }
     }
-    if (push && !small) {  // hubs go to the next step's bucket work list
-      const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
-      qstore2(S.q_large[side] + uint32_t(r >> 32), make_uint2(v, uint32_t(r)));
-      ctl[1] = 1u;
-    }
+    // hubs go to the next step's bucket work list
+    push_hubs(S, side, slot, push && !small, v, nb);
+    if (push && !small) ctl[1] = 1u;
     pushed += uint32_t(__popcll(pm));
   }
 };
@@ -248,11 +291,8 @@ MK_DEV void block_enqueue(const GraphView &G, const TravView &S, uint32_t side, 
     sh[NW] = tot ? atomicAdd(&S.cnt->small[slot], tot) : 0u;
   }
   __syncthreads();
-  if (small) S.q_small[side][sh[NW] + sh[w] + lanes_below(sm)] = v;
-  if (push && !small) {
-    const unsigned long long r = atomicAdd(&S.cnt->large[slot], (1ull << 32) | nb);
-    S.q_large[side][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
-  }
+  if (small) qstore(S.q_small[side] + sh[NW] + sh[w] + lanes_below(sm), v);
+  push_hubs(S, side, slot, push && !small, v, nb);
   const uint64_t pm = ballot(push);
   if (lane == 0) add_stat(S.cnt, kStatPushed, uint32_t(__popcll(pm)));
 }
@@ -439,8 +479,8 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
 // small frontier without hub entries is run in tail mode by block 0.
 // Called by every thread of every block of the grid; `lds` holds
 // kExpandLds words.
-constexpr uint32_t kExpandLds =
-    2 * kTailCap + 4 > (kBlock / 64) * kWaveQ ? 2 * kTailCap + 4 : (kBlock / 64) * kWaveQ;
+constexpr uint32_t kWaveLds = (kBlock / 64) * kWaveQ + kBlock / 64 + 1;  // staging + block_flush
+constexpr uint32_t kExpandLds = 2 * kTailCap + 4 > kWaveLds ? 2 * kTailCap + 4 : kWaveLds;
 
 template <bool MAP, bool WEIGHTS, int MODE>
 MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
@@ -496,7 +536,7 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
   // small vertices: up to 64 per wave step, spread over all waves when few
   expand_rows<MAP, WEIGHTS, MODE>(G, S, sink, epoch, S.q_small[side], n_small, gw, W, slabs, edges,
                                   visited);
-  wq_flush(q, S.q_small[nside], &S.cnt->small[nslot]);
+  block_flush(q, S.q_small[nside], &S.cnt->small[nslot], lds + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
     add_stat(S.cnt, kStatSlabs, slabs);
     add_stat(S.cnt, kStatEdges, edges);
