@@ -630,8 +630,8 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
-  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
-  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  __shared__ uint32_t lq[kWaveLds];
+  WaveQ q{lq + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
   uint32_t slabs = 0, edges = 0;
   for (uint64_t b0 = uint64_t(gw) * 64; b0 < G.total_buckets; b0 += uint64_t(W) * 64) {
     const uint64_t b = b0 + lane;
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(kBlock)
     batched_buckets<MAP, WEIGHTS, 2>(G, S, sink, epoch, ballot(act), uint32_t(b), u, du, slabs,
                                      edges);
   }
-  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
+  block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
     add_stat(S.cnt, kStatSlabs, slabs);
     add_stat(S.cnt, kStatEdges, edges);
@@ -742,8 +742,8 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
-  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
-  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  __shared__ uint32_t lq[kWaveLds];
+  WaveQ q{lq + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
   const uint32_t total = S.cnt->ninv;
   const uint32_t per = per_wave(total, W);
   uint32_t slabs = 0, edges = 0, visited = 0;
@@ -755,18 +755,16 @@ __global__ void __launch_bounds__(kBlock)
       v = S.inv_list[i];
       rb0 = R.bucket_off[v];
       rnb = R.bucket_off[v + 1] - rb0;
-      if (rnb > kSmallBuckets) {
-        const unsigned long long r = atomicAdd(&S.cnt->large[1], (1ull << 32) | rnb);
-        S.q_large[1][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
-        act = false;
-      }
     }
+    const bool hub = act && rnb > kSmallBuckets;
+    push_hubs(S, 1, 1, hub, v, rnb);
+    if (hub) act = false;
     visited += uint32_t(__popcll(ballot(act)));
     for (uint32_t k = 0; ballot(act && rnb > k); ++k)
       pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act && rnb > k), rb0 + k, v, slabs,
                                  edges);
   }
-  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
+  block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
     add_stat(S.cnt, kStatSlabs, slabs);
     add_stat(S.cnt, kStatEdges, edges);
@@ -832,8 +830,8 @@ __global__ void __launch_bounds__(kBlock)
   const unsigned long long lg = S.cnt->large[1];
   const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
   if (n_large == 0) return;
-  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
-  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  __shared__ uint32_t lq[kWaveLds];
+  WaveQ q{lq + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
   uint32_t slabs = 0, edges = 0;
   const uint2 *ql = S.q_large[1];
   for (uint32_t t0 = gw * kLargeChunk; t0 < items; t0 += W * kLargeChunk) {
@@ -854,7 +852,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges);
   }
-  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
+  block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
     add_stat(S.cnt, kStatSlabs, slabs);
     add_stat(S.cnt, kStatEdges, edges);
@@ -974,8 +972,8 @@ constexpr uint32_t kRepairBuckets = 8;
 // k_invalidate_batch does.  One 8-lane group per orphan (each listed once).
 template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, TravView S) {
-  __shared__ uint32_t lq[kBlock / 64][kWaveQ];
-  WaveQ q{lq[threadIdx.x >> 6], 0u, 0u};
+  __shared__ uint32_t lq[kWaveLds];
+  WaveQ q{lq + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
   const uint32_t gl = group_lane();
   const uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / kGroupLanes;
   const uint32_t ng = (gridDim.x * blockDim.x) / kGroupLanes;
@@ -1036,7 +1034,7 @@ __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, Tra
     }
     enqueue(G, S, q, 0, 0, push, v);
   }
-  wq_flush(q, S.q_small[0], &S.cnt->small[0]);
+  block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
 }
 
 
