@@ -51,6 +51,9 @@ constexpr uint32_t kBatch = 4;       // slab loads in flight per wave
 constexpr uint32_t kWaveQ = 512;     // LDS staging entries per wave
 constexpr uint32_t kStatSlots = 128;  // statistics spread over this many 128-byte lines
 enum { kStatSlabs = 0, kStatEdges = 1, kStatVisited = 2, kStatPushed = 3 };
+// statistics sets, one per kind of kernel: frontier steps relaxing (MODE 0),
+// invalidating (MODE 1), pull/bucket re-seeds (MODE 2), batch seeds
+enum { kSetRelax = 0, kSetInvalidate = 1, kSetReseed = 2, kSetSeed = 3, kSets = 4 };
 
 // Device counters.  Frontier slots rotate per step; the statistics
 // accumulate over one compute call, each block adding to slot
@@ -60,23 +63,24 @@ struct Counts {
   uint32_t steps;                 // steps that found a non-empty frontier
   uint32_t ninv;                  // entries of the invalidated-vertex list
   uint32_t abort;                 // set when a grid-barrier wait timed out
-  uint32_t loop_end;              // step index at which k_expand_loop stopped
-  uint32_t pad;
+  uint32_t loop_end[2];           // step index at which k_expand_loop<MODE> stopped
+  uint32_t bar_done;              // grid barriers completed (k_expand_loop)
   unsigned long long large[3];    // (entries << 32) | bucket items
-  unsigned long long stats[kStatSlots][16];  // [slot][which], one line per slot
+  unsigned long long stats[kStatSlots][16];  // [slot][set * 4 + which], one line per slot
 };
 
 // Host-side totals of a Counts snapshot.
 struct Totals {
   uint32_t small[3];
   uint32_t steps;
-  uint32_t abort, loop_end;
+  uint32_t abort, loop_end[2];
   unsigned long long large[3];
-  unsigned long long slabs, edges, visited, pushed;
+  unsigned long long slabs, edges, visited, pushed;  // over every set
+  unsigned long long set[kSets][4];                 // [set][which]
 };
 
-MK_DEV void add_stat(Counts *c, uint32_t which, uint32_t v) {
-  if (v) atomicAdd(&c->stats[blockIdx.x % kStatSlots][which], (unsigned long long)v);
+MK_DEV void add_stat(Counts *c, uint32_t set, uint32_t which, uint32_t v) {
+  if (v) atomicAdd(&c->stats[blockIdx.x % kStatSlots][set * 4 + which], (unsigned long long)v);
 }
 
 // Per-wave frontier staging in LDS: pushes are appended to the wave's LDS
@@ -294,7 +298,7 @@ MK_DEV void block_enqueue(const GraphView &G, const TravView &S, uint32_t side, 
   if (small) qstore(S.q_small[side] + sh[NW] + sh[w] + lanes_below(sm), v);
   push_hubs(S, side, slot, push && !small, v, nb);
   const uint64_t pm = ballot(push);
-  if (lane == 0) add_stat(S.cnt, kStatPushed, uint32_t(__popcll(pm)));
+  if (lane == 0) add_stat(S.cnt, kSetSeed, kStatPushed, uint32_t(__popcll(pm)));
 }
 
 MK_DEV bool inv_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> (v & 31u)) & 1u; }
@@ -467,10 +471,10 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
     nin = nb;
   }
   if (lane == 0) {
-    add_stat(S.cnt, kStatSlabs, slabs);
-    add_stat(S.cnt, kStatEdges, edges);
-    add_stat(S.cnt, kStatVisited, visited);
-    add_stat(S.cnt, kStatPushed, pushed);
+    add_stat(S.cnt, MODE, kStatSlabs, slabs);
+    add_stat(S.cnt, MODE, kStatEdges, edges);
+    add_stat(S.cnt, MODE, kStatVisited, visited);
+    add_stat(S.cnt, MODE, kStatPushed, pushed);
   }
 }
 
@@ -538,10 +542,10 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
                                   visited);
   block_flush(q, S.q_small[nside], &S.cnt->small[nslot], lds + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
-    add_stat(S.cnt, kStatSlabs, slabs);
-    add_stat(S.cnt, kStatEdges, edges);
-    add_stat(S.cnt, kStatVisited, visited);
-    add_stat(S.cnt, kStatPushed, q.pushed);
+    add_stat(S.cnt, MODE, kStatSlabs, slabs);
+    add_stat(S.cnt, MODE, kStatEdges, edges);
+    add_stat(S.cnt, MODE, kStatVisited, visited);
+    add_stat(S.cnt, MODE, kStatPushed, q.pushed);
   }
 }
 
@@ -597,16 +601,21 @@ MK_DEV bool grid_barrier(BarLines *bl, Counts *c, uint32_t k, uint64_t timeout, 
 // Frontier steps j0, j0 + 1, ... < j_end in one launch (all blocks
 // resident), a grid barrier between steps: step j reads queue side j & 1,
 // counter slot j % 3, as k_expand does.  Stops before the first step whose
-// frontier is empty and records its index in cnt->loop_end (j_end when the
-// range ran out first).  MODE 0 step j uses epochs epoch0 + (j - j0) *
-// kEpochStride onwards.  `bdone` grid barriers have completed since the
-// barrier counters were zeroed.
+// frontier is empty and records its index in cnt->loop_end[MODE] (j_end
+// when the range ran out first).  MODE 0 step j uses epochs epoch0 +
+// (j - j0) * kEpochStride onwards.  cnt->bar_done counts the grid barriers
+// completed since the barrier counters were zeroed (read at the start,
+// advanced at the end), so launches queued back to back need no host.
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
-                  uint32_t bdone, uint64_t timeout) {
+                  uint64_t timeout) {
   __shared__ uint32_t lds[kExpandLds];
   __shared__ uint32_t flag;
+  // written by an earlier launch on this stream; every block reads it
+  // before its first barrier arrival, so block 0's update at the end
+  // cannot race with it
+  const uint32_t bdone = coherent_load(&S.cnt->bar_done);
   uint32_t j = j0;
   for (; j < j_end; ++j) {
     const uint32_t slot = j % 3;
@@ -618,7 +627,10 @@ __global__ void __launch_bounds__(kBlock)
                                     MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
     if (!grid_barrier(S.bar, S.cnt, bdone + (j - j0) + 1, timeout, &flag)) return;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) S.cnt->loop_end = j;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.cnt->loop_end[MODE] = j;
+    S.cnt->bar_done = bdone + (j - j0);
+  }
 }
 
 // Re-seed invalidated vertices from valid in-neighbours: waves stream over
@@ -627,6 +639,7 @@ __global__ void __launch_bounds__(kBlock)
 template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock)
     k_reseed(GraphView G, TravView S, uint32_t epoch) {
+  if (S.cnt->steps == 0) return;  // no invalidation step ran: nothing to re-seed
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
@@ -651,9 +664,9 @@ __global__ void __launch_bounds__(kBlock)
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
-    add_stat(S.cnt, kStatSlabs, slabs);
-    add_stat(S.cnt, kStatEdges, edges);
-    add_stat(S.cnt, kStatPushed, q.pushed);
+    add_stat(S.cnt, kSetReseed, kStatSlabs, slabs);
+    add_stat(S.cnt, kSetReseed, kStatEdges, edges);
+    add_stat(S.cnt, kSetReseed, kStatPushed, q.pushed);
   }
 }
 
@@ -766,10 +779,10 @@ __global__ void __launch_bounds__(kBlock)
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
-    add_stat(S.cnt, kStatSlabs, slabs);
-    add_stat(S.cnt, kStatEdges, edges);
-    add_stat(S.cnt, kStatVisited, visited);
-    add_stat(S.cnt, kStatPushed, q.pushed);
+    add_stat(S.cnt, kSetReseed, kStatSlabs, slabs);
+    add_stat(S.cnt, kSetReseed, kStatEdges, edges);
+    add_stat(S.cnt, kSetReseed, kStatVisited, visited);
+    add_stat(S.cnt, kSetReseed, kStatPushed, q.pushed);
   }
 }
 
@@ -780,6 +793,7 @@ __global__ void __launch_bounds__(kBlock)
 constexpr uint32_t kInvWords = 8;
 
 __global__ void __launch_bounds__(kBlock) k_inv_compact(TravView S, uint32_t nwords) {
+  if (S.cnt->steps == 0) return;  // no invalidation step ran: the bitmap is clear
   __shared__ uint32_t wsum[kBlock / 64 + 1];
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   const uint64_t first = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * kInvWords;
@@ -854,9 +868,9 @@ __global__ void __launch_bounds__(kBlock)
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
-    add_stat(S.cnt, kStatSlabs, slabs);
-    add_stat(S.cnt, kStatEdges, edges);
-    add_stat(S.cnt, kStatPushed, q.pushed);
+    add_stat(S.cnt, kSetReseed, kStatSlabs, slabs);
+    add_stat(S.cnt, kSetReseed, kStatEdges, edges);
+    add_stat(S.cnt, kSetReseed, kStatPushed, q.pushed);
   }
 }
 
@@ -1076,7 +1090,6 @@ struct Traversal {
   uint2 *ql[2] = {nullptr, nullptr};
   Counts *cnt = nullptr;  // device; BarLines follow at bar_offset
   BarLines *bar = nullptr;
-  uint32_t bar_done = 0;  // grid barriers completed since the counters were zeroed
   HostBuf host;           // pinned Counts
   HostBuf snap[2];        // pinned Counts snapshots of successive step groups
   hipEvent_t snap_ev[2] = {nullptr, nullptr};
@@ -1218,8 +1231,7 @@ struct Traversal {
     GraphView G = g->view();
     TravView S = view();
     uint64_t to = barrier_timeout;
-    uint32_t bd = bar_done;
-    void *args[] = {&G, &S, &j0, &j1, &ep, &bd, &to};
+    void *args[] = {&G, &S, &j0, &j1, &ep, &to};
     ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
     MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
                                       dim3(kBlock), args, 0u, stream));
@@ -1229,7 +1241,6 @@ struct Traversal {
   // Zero every counter (statistics and grid-barrier lines included).
   void clear_counts() {
     fill_u32(reinterpret_cast<uint32_t *>(cnt), 0u, (bar_offset() + sizeof(BarLines)) / 4, stream);
-    bar_done = 0;
   }
   static constexpr size_t bar_offset() { return (sizeof(Counts) + 127) / 128 * 128; }
   // Zero the frontier slots only.
@@ -1252,13 +1263,20 @@ struct Traversal {
     }
     h.steps = c.steps;
     h.abort = c.abort;
-    h.loop_end = c.loop_end;
+    h.loop_end[0] = c.loop_end[0];
+    h.loop_end[1] = c.loop_end[1];
+    for (uint32_t s = 0; s < kSets; ++s)
+      for (uint32_t w = 0; w < 4; ++w) {
+        unsigned long long x = 0;
+        for (uint32_t k = 0; k < kStatSlots; ++k) x += c.stats[k][s * 4 + w];
+        h.set[s][w] = x;
+      }
     h.slabs = h.edges = h.visited = h.pushed = 0;
-    for (uint32_t k = 0; k < kStatSlots; ++k) {
-      h.slabs += c.stats[k][kStatSlabs];
-      h.edges += c.stats[k][kStatEdges];
-      h.visited += c.stats[k][kStatVisited];
-      h.pushed += c.stats[k][kStatPushed];
+    for (uint32_t s = 0; s < kSets; ++s) {
+      h.slabs += h.set[s][kStatSlabs];
+      h.edges += h.set[s][kStatEdges];
+      h.visited += h.set[s][kStatVisited];
+      h.pushed += h.set[s][kStatPushed];
     }
   }
 
@@ -1272,32 +1290,67 @@ struct Traversal {
     prof_launch(name, 0.0, k, dim3(grid), dim3(kBlock), 0u, stream, G, S, side, slot, ep);
   }
 
-  // Run frontier steps from queue side 0 / slot 0 until the frontier is
-  // empty, kLoopSteps steps per k_expand_loop launch (a further launch only
-  // when a launch used up its range); one host wait per launch.
-  // Algorithmic bytes as in iterate.
+  // Persistent frontier loops (k_expand_loop) are queued without waiting
+  // for the device; end() checks them with the call's one host wait.  An
+  // invalidation loop runs until its frontier drains (it uses no epochs); a
+  // relax loop gets kLoopSteps steps' epochs and end() queues a further
+  // launch in the rare case that it used up its range.
   static constexpr uint32_t kLoopSteps = 256;
-  template <int MODE> int iterate_loop(Totals &h) {
-    Totals prev;
-    if (profiling_on()) {
-      if (int r = read_counts(prev)) return r;
-    }
-    const uint64_t limit = uint64_t(g->n) + 8;
-    for (uint32_t j0 = 0;; j0 += kLoopSteps) {
-      if (j0 > limit) return fail(MK_ERR_STATE, "frontier did not drain within n steps");
-      const uint32_t ep = MODE == 0 ? reserve_epochs(kLoopSteps) : 0u;
-      if (int r = launch_loop<MODE>(j0, j0 + kLoopSteps, ep)) return r;
-      MK_HIP(hipGetLastError());
+  uint32_t relax_end = 0;     // range end of the queued relax loop (0: none)
+  bool inval_queued = false;  // an invalidation loop was queued
+  int reseed_kind = 0;        // queued re-seed: 0 none, 1 pull (mirror), 2 bucket scan
+
+  uint32_t inval_limit() const {
+    return uint32_t(std::min<uint64_t>(uint64_t(g->n) + 8, 0xFFFFFFF0ull));
+  }
+  int queue_relax() {
+    relax_end = kLoopSteps;
+    return launch_loop<0>(0, kLoopSteps, reserve_epochs(kLoopSteps));
+  }
+  int queue_invalidate() {
+    inval_queued = true;
+    return launch_loop<1>(0, inval_limit(), 0u);
+  }
+
+  // end()'s part for queued loops: counters `h` were just read.  Algorithmic
+  // bytes (profiling) per phase from its statistics set, as in iterate and
+  // run_decremental.
+  int finish_loops(Totals &h) {
+    if (h.abort) return fail(MK_ERR_STATE, "frontier loop: a grid barrier wait timed out");
+    if (inval_queued && h.loop_end[1] >= inval_limit())
+      return fail(MK_ERR_STATE, "frontier did not drain within n steps");
+    while (relax_end && h.loop_end[0] == relax_end) {
+      if (uint64_t(relax_end) > uint64_t(g->n) + 8)
+        return fail(MK_ERR_STATE, "frontier did not drain within n steps");
+      const uint32_t j0 = relax_end;
+      relax_end += kLoopSteps;
+      if (int r = launch_loop<0>(j0, relax_end, reserve_epochs(kLoopSteps))) return r;
+      MK_HIP(hipEventRecord(ev1, stream));
       if (int r = read_counts(h)) return r;
       if (h.abort) return fail(MK_ERR_STATE, "frontier loop: a grid barrier wait timed out");
-      bar_done += h.loop_end - j0;  // one barrier after each step run
-      if (h.loop_end < j0 + kLoopSteps) break;
     }
     if (profiling_on()) {
-      const double bytes = 256.0 * double(h.slabs - prev.slabs) + 8.0 * double(h.edges - prev.edges) +
-                           24.0 * double(h.visited - prev.visited) +
-                           16.0 * double(h.pushed - prev.pushed);
-      prof_add_bytes(MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", bytes);
+      auto steps_bytes = [&](int set) {
+        return 256.0 * double(h.set[set][kStatSlabs]) + 8.0 * double(h.set[set][kStatEdges]) +
+               24.0 * double(h.set[set][kStatVisited]) + 16.0 * double(h.set[set][kStatPushed]);
+      };
+      if (relax_end) prof_add_bytes("k_expand_relax", steps_bytes(kSetRelax));
+      if (inval_queued) prof_add_bytes("k_expand_invalidate", steps_bytes(kSetInvalidate));
+      const unsigned long long *r = h.set[kSetReseed];
+      if (reseed_kind == 1)
+        // 12 B per listed vertex (entry, in-bucket range), 256 B per
+        // in-slab, 4 + 8 B per in-neighbour (invalid bit, distance), 16 B
+        // per re-seeded vertex
+        prof_add_bytes("k_reseed_pull", 12.0 * double(r[kStatVisited]) +
+                                            256.0 * double(r[kStatSlabs]) +
+                                            12.0 * double(r[kStatEdges]) +
+                                            16.0 * double(r[kStatPushed]));
+      else if (reseed_kind == 2 && h.set[kSetInvalidate][kStatVisited])
+        // 20 B per bucket (owner, degree, distance, invalid bit of the
+        // owner), 256 B per slab read, 4 B per key (invalid bit), 16 B per
+        // re-seeded vertex
+        prof_add_bytes("k_reseed", 20.0 * double(g->total_buckets) + 256.0 * double(r[kStatSlabs]) +
+                                       4.0 * double(r[kStatEdges]) + 16.0 * double(r[kStatPushed]));
     }
     return MK_OK;
   }
@@ -1312,7 +1365,6 @@ struct Traversal {
   // expanded vertex (entry, bucket range, distance), 16 B per push (atomic,
   // stamp, entry).
   template <int MODE> int iterate(Totals &h) {
-    if (persistent) return iterate_loop<MODE>(h);
     uint32_t side = 0, slot = 0;
     uint64_t launched = 0;
     const uint64_t limit = uint64_t(g->n) + 8;
@@ -1358,6 +1410,9 @@ struct Traversal {
 
   int begin() {
     MK_HIP(hipSetDevice(g->device));
+    relax_end = 0;
+    inval_queued = false;
+    reseed_kind = 0;
     clear_counts();
     MK_HIP(hipEventRecord(ev0, stream));
     return MK_OK;
@@ -1368,6 +1423,11 @@ struct Traversal {
     MK_HIP(hipEventRecord(ev1, stream));
     Totals h;
     if (int r = read_counts(h)) return r;
+    const int lr = finish_loops(h);
+    relax_end = 0;
+    inval_queued = false;
+    reseed_kind = 0;
+    if (lr) return lr;
     float t = 0.f;
     MK_HIP(hipEventElapsedTime(&t, ev0, ev1));
     if (ms) *ms = t;
@@ -1381,8 +1441,12 @@ struct Traversal {
     if (int r = begin()) return r;
     fill_u64(dist, kInfInf, g->n, stream);
     k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
-    Totals h;
-    if (int r = iterate<0>(h)) return r;
+    if (persistent) {
+      if (int r = queue_relax()) return r;
+    } else {
+      Totals h;
+      if (int r = iterate<0>(h)) return r;
+    }
     return end(ms);
   }
 
@@ -1405,9 +1469,53 @@ struct Traversal {
                     view(), src, dst, weights ? w : nullptr, weights ? st : nullptr, n, ep);
       }
       MK_HIP(hipGetLastError());
-      Totals h;
-      if (int r = iterate<0>(h)) return r;
+      if (persistent) {
+        if (int r = queue_relax()) return r;
+      } else {
+        Totals h;
+        if (int r = iterate<0>(h)) return r;
+      }
     }
+    return end(ms);
+  }
+
+  // Decremental after the batch seeds, persistent loops: the invalidation
+  // loop, the re-seed and the relax loop are all queued, then end() waits
+  // once.  The re-seed kernels return at once when nothing was invalidated
+  // (no invalidation step ran), as the host-driven path skips them.
+  int queue_decremental(float *ms) {
+    const GraphView G = g->view();
+    if (int r = queue_invalidate()) return r;
+    const uint32_t ep = next_epoch();
+    clear_slots();
+    if (g->rev) {
+      const GraphView R = g->rev->view();
+      const uint32_t nwords = uint32_t((uint64_t(g->n) + 31) / 32);
+      fill_u32(&cnt->ninv, 0u, 1, stream);
+      k_inv_compact<<<blocks_for_threads((uint64_t(nwords) + kInvWords - 1) / kInvWords), kBlock,
+                      0, stream>>>(view(), nwords);
+      const unsigned grid = blocks_for_waves(uint64_t(max_waves));
+      auto kp = g->weighted ? (weights ? k_reseed_pull<true, true> : k_reseed_pull<true, false>)
+                            : k_reseed_pull<false, false>;
+      auto kh = g->weighted ? (weights ? k_pull_hubs<true, true> : k_pull_hubs<true, false>)
+                            : k_pull_hubs<false, false>;
+      prof_launch("k_reseed_pull", 0.0, kp, dim3(grid), dim3(kBlock), 0u, stream, G, R, view(), ep);
+      prof_launch("k_pull_hubs", 0.0, kh, dim3(grid), dim3(kBlock), 0u, stream, G, R, view(), ep);
+      // the hub list used slot 1, which the first relax step appends to
+      fill_u32(cnt->small + 1, 0u, 1, stream);
+      fill_u64(reinterpret_cast<uint64_t *>(cnt->large + 1), 0ull, 1, stream);
+      reseed_kind = 1;
+    } else {
+      const unsigned grid = blocks_for_waves(std::min<uint64_t>(
+          (uint64_t(g->total_buckets) + 63) / 64, uint64_t(max_waves) * 2));
+      auto k = g->weighted ? (weights ? k_reseed<true, true> : k_reseed<true, false>)
+                           : k_reseed<false, false>;
+      prof_launch("k_reseed", 0.0, k, dim3(grid), dim3(kBlock), 0u, stream, G, view(), ep);
+      reseed_kind = 2;
+    }
+    MK_HIP(hipGetLastError());
+    fill_u32(invbits, 0u, (uint64_t(g->n) + 31) / 32, stream);
+    if (int r = queue_relax()) return r;
     return end(ms);
   }
 
@@ -1430,6 +1538,7 @@ struct Traversal {
         }
       }
       MK_HIP(hipGetLastError());
+      if (persistent) return queue_decremental(ms);
       // propagate invalidation down the tree; any step ran = something was
       // invalidated
       Totals h;
