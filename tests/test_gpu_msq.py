@@ -89,7 +89,10 @@ def test_autotune_picks_a_setting_and_keeps_parity(gpu):
         r = autotune.tune(g, waves=(8, 16), unrolls=(4, 8), repeats=1)
         assert r["waves_per_cu"] in (8, 16) and r["ms_unroll"] in (4, 8)
         assert set(r["timings"]["waves_per_cu"]) == {8, 16}
-        assert autotune.knobs() == {"waves_per_cu": r["waves_per_cu"], "ms_unroll": r["ms_unroll"]}
+        k = autotune.knobs()
+        assert (k["waves_per_cu"], k["ms_unroll"]) == (r["waves_per_cu"], r["ms_unroll"])
+        assert {x: k[x] for x in k if x not in ("waves_per_cu", "ms_unroll")} == {
+            x: before[x] for x in before if x not in ("waves_per_cu", "ms_unroll")}
         srcs = np.array([0, 5, 77], np.uint32)
         q = mk.MultiSource(g, sssp=True)
         q.run(srcs)
