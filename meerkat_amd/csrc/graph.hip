@@ -126,14 +126,43 @@ __global__ void k_bucket_owner(const uint32_t *off, uint32_t n, uint32_t total, 
   owner[b] = lo;
 }
 
-// Range check of a batch: counters[0] += edges with an endpoint >= n.
+// Mutation counters are versioned by the call's epoch instead of being
+// cleared before each call (no clearing launch): counters[0] holds the
+// epoch of the last batch that failed the range check; counters[2..3] (one
+// 64-bit word) hold (epoch << 32) | slab-pool failures of that epoch.
+
+// Range check of a batch: counters[0] = epoch when an endpoint is >= nv.
 __global__ void k_validate(const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t nv,
-                           uint32_t *counters) {
+                           uint32_t *counters, uint32_t epoch) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool bad = false;
   if (i < n) bad = src[i] >= nv || dst[i] >= nv;
   uint64_t m = ballot(bad);
-  if (m && lane_id() == ffs64(m)) atomicAdd(counters, uint32_t(__popcll(m)));
+  if (m && lane_id() == ffs64(m)) counters[0] = epoch;
+}
+
+// Edges of a batch with an endpoint >= nv, added to *count (error path).
+__global__ void k_count_bad(const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t nv,
+                            uint32_t *count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < n) bad = src[i] >= nv || dst[i] >= nv;
+  uint64_t m = ballot(bad);
+  if (m && lane_id() == ffs64(m)) atomicAdd(count, uint32_t(__popcll(m)));
+}
+
+// Add `add` to the (epoch << 32) | count word at p, restarting the count
+// when the word holds an older epoch.
+MK_DEV void add_epoch_count(unsigned long long *p, uint32_t epoch, uint32_t add) {
+  unsigned long long old = coherent_load64(reinterpret_cast<const uint64_t *>(p));
+  for (;;) {
+    const unsigned long long nv = uint32_t(old >> 32) == epoch
+                                      ? old + add
+                                      : ((unsigned long long)epoch << 32) | add;
+    const unsigned long long seen = atomicCAS(p, old, nv);
+    if (seen == old) return;
+    old = seen;
+  }
 }
 
 // One side of a batch mutation: the graph it applies to, that graph's
@@ -177,7 +206,7 @@ template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_insert(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b, PoolView P,
              const uint32_t *src, const uint32_t *dst, const uint32_t *wt, uint32_t n,
-             uint32_t epw, const uint32_t *bad) {
+             uint32_t epw, const uint32_t *bad, uint32_t epoch) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -189,7 +218,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
-  if (*bad) {  // the batch failed the range check: leave the graph as it is
+  if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
     if (lane < cnt) M.status[base + lane] = MK_EDGE_UNCHANGED;
     return;
   }
@@ -228,8 +257,9 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   if (lane < cnt) M.status[base + lane] = st;
-  // pool failures (rare): one atomic per group that had any
-  if (gl == 0 && pool_fail) atomicAdd(M.counters + 1, pool_fail);
+  // pool failures (rare): one update per group that had any
+  if (gl == 0 && pool_fail)
+    add_epoch_count(reinterpret_cast<unsigned long long *>(M.counters + 2), epoch, pool_fail);
 }
 
 // Delete the batch edges whose status equals `only_status` (all edges when
@@ -239,7 +269,7 @@ template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_delete(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b, const uint32_t *src,
              const uint32_t *dst, uint32_t n, uint32_t epw, uint32_t only_status,
-             const uint32_t *bad) {
+             const uint32_t *bad, uint32_t epoch) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -251,7 +281,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
-  if (*bad) {  // the batch failed the range check: leave the graph as it is
+  if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
     if (lane < cnt && only_status == 0xFFu) M.status[base + lane] = MK_EDGE_UNCHANGED;
     return;
   }
@@ -632,29 +662,46 @@ static int batch_prologue(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   g->status_n = n;
   g->last_op = 0;
   g->last_src = g->last_dst = g->last_w = nullptr;
-  fill_u32(g->counters, 0u, 4, s);
   if (n == 0) return MK_OK;
-  // the range check's count (counters[0]) is read by the mutation kernels,
-  // which leave the graph unchanged when it is non-zero; the host reads it
-  // with the mutation's results (range_error)
-  k_validate<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters);
+  if (++g->mut_epoch == 0) {  // epochs wrapped: clear the versioned counters once
+    fill_u32(g->counters, 0u, 4, s);
+    g->mut_epoch = 1;
+  }
+  // the range check's verdict (counters[0] == epoch) is read by the mutation
+  // kernels, which leave the graph unchanged on a failure; the host reads
+  // it with the mutation's results (range_error)
+  k_validate<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters,
+                                                      g->mut_epoch);
   return MK_OK;
 }
 
 // After a mutation's counters were copied to host_scratch and the stream
-// synchronised: MK_ERR_RANGE when the batch failed the range check.
-static int range_error(mk_graph *g, const char *who) {
-  const uint32_t bad = g->host_scratch.as<uint32_t>()[0];
-  if (!bad) return MK_OK;
+// synchronised: MK_ERR_RANGE when the batch failed the range check (the
+// offending edges are counted for the message).
+static int range_error(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
+                       hipStream_t s, const char *who) {
+  if (g->host_scratch.as<uint32_t>()[0] != g->mut_epoch) return MK_OK;
+  fill_u32(g->counters + 1, 0u, 1, s);
+  k_count_bad<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters + 1);
+  uint32_t bad = 0;
+  MK_HIP(hipMemcpyAsync(&bad, g->counters + 1, 4, hipMemcpyDeviceToHost, s));
+  MK_HIP(hipStreamSynchronize(s));
   return fail(MK_ERR_RANGE, std::string(who) + ": " + std::to_string(bad) +
                                 " edges have a vertex id >= n; graph unchanged");
 }
 
-// The mutation side of graph g (its own view, status and counters).
-// The mirror keeps no degree array up to date: no kernel reads it, and
-// mk_graph_degrees counts its live keys instead.
-static MutSide side_of(mk_graph *g, bool swap) {
-  return MutSide{g->view(), g->status.as<uint8_t>(), g->counters, swap ? 1u : 0u,
+// Slab-pool failures of this call's mutation kernels (host_scratch read).
+static uint32_t pool_failures(mk_graph *g) {
+  const uint64_t w = g->host_scratch.as<uint64_t>()[1];
+  return uint32_t(w >> 32) == g->mut_epoch ? uint32_t(w) : 0u;
+}
+
+// The mutation side of graph g (its own view and status; the counters of
+// the graph the batch was given to, `counters`).  The mirror keeps no
+// degree array up to date: no kernel reads it, and mk_graph_degrees counts
+// its live keys instead.
+static MutSide side_of(mk_graph *g, bool swap, uint32_t *counters) {
+  return MutSide{g->view(), g->status.as<uint8_t>(), counters, swap ? 1u : 0u,
                  g->mirror_of ? 0u : 1u};
 }
 
@@ -665,7 +712,8 @@ static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
                           const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
-  const MutSide a = side_of(g, false), b = g->rev ? side_of(g->rev, true) : a;
+  const MutSide a = side_of(g, false, g->counters),
+                b = g->rev ? side_of(g->rev, true, g->counters) : a;
   const uint64_t total = g->rev ? 2 * waves : waves;
   // per edge: batch entry, one slab probe, key (+value) write, degree atomic,
   // status byte; the mirror side the same without the degree atomic
@@ -674,7 +722,7 @@ static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   prof_launch(name, bytes, g->weighted ? k_insert<true> : k_insert<false>,
               dim3(blocks_for_waves(total)), dim3(kBlock), 0u, s, a, b, uint32_t(waves),
               uint32_t(total - waves), g->pool->view, src, dst, g->weighted ? w : nullptr, n, epw,
-              bad);
+              bad, g->mut_epoch);
 }
 
 // Launch the delete kernel over g (and its mirror); only_status != 0xFF
@@ -684,23 +732,22 @@ static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst,
                           const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
-  const MutSide a = side_of(g, false), b = g->rev ? side_of(g->rev, true) : a;
+  const MutSide a = side_of(g, false, g->counters),
+                b = g->rev ? side_of(g->rev, true, g->counters) : a;
   const uint64_t total = g->rev ? 2 * waves : waves;
   // per edge: batch entry, one slab probe, tombstone write, degree atomic,
   // status byte; the mirror side the same without the degree atomic
   prof_launch(name, double(n) * (g->rev ? 2 * 277 - 8 : 277),
               g->weighted ? k_delete<true> : k_delete<false>, dim3(blocks_for_waves(total)),
               dim3(kBlock), 0u, s, a, b, uint32_t(waves), uint32_t(total - waves), src, dst, n,
-              epw, only_status, bad);
+              epw, only_status, bad, g->mut_epoch);
 }
 
-// Status buffer and counters of the in-edge mirror for a batch of n edges.
-static int mirror_prologue(mk_graph *g, uint32_t n, hipStream_t s) {
+// Status buffer of the in-edge mirror for a batch of n edges.
+static int mirror_prologue(mk_graph *g, uint32_t n) {
   if (!g->rev) return MK_OK;
   if (int r = g->rev->status.ensure(std::max<size_t>(n, 1))) return r;
-  if (int r = g->rev->host_scratch.ensure(64)) return r;
   g->rev->status_n = n;
-  fill_u32(g->rev->counters, 0u, 4, s);
   return MK_OK;
 }
 
@@ -710,20 +757,16 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_insert: mirror is read-only");
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
   if (n == 0) return MK_OK;
-  if (int r = mirror_prologue(g, n, s)) return r;
+  if (int r = mirror_prologue(g, n)) return r;
   launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
-  if (g->rev)
-    MK_HIP(hipMemcpyAsync(g->rev->host_scratch.ptr, g->rev->counters, 16, hipMemcpyDeviceToHost,
-                          s));
   MK_HIP(hipStreamSynchronize(s));
-  if (int r = range_error(g, "mk_graph_insert")) return r;
-  uint32_t pool_fail = g->host_scratch.as<uint32_t>()[1];
-  if (g->rev) pool_fail += g->rev->host_scratch.as<uint32_t>()[1];
+  if (int r = range_error(g, src, dst, n, s, "mk_graph_insert")) return r;
+  const uint32_t pool_fail = pool_failures(g);  // both sides count here
   if (pool_fail) {
     // roll back: delete what this batch inserted, restoring the edge set
-    // (counters[0] is zero here, so the kernels run)
+    // (the range check passed, so the kernels run)
     launch_delete(g, src, dst, n, s, MK_EDGE_APPLIED, "k_insert_rollback", g->counters);
     MK_HIP(hipStreamSynchronize(s));
     return fail(MK_ERR_POOL_FULL, "mk_graph_insert: slab pool exhausted (" +
@@ -743,12 +786,12 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_delete: mirror is read-only");
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
   if (n == 0) return MK_OK;
-  if (int r = mirror_prologue(g, n, s)) return r;
+  if (int r = mirror_prologue(g, n)) return r;
   launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);
   MK_HIP(hipGetLastError());
-  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
+  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 4, hipMemcpyDeviceToHost, s));
   MK_HIP(hipStreamSynchronize(s));
-  if (int r = range_error(g, "mk_graph_delete")) return r;
+  if (int r = range_error(g, src, dst, n, s, "mk_graph_delete")) return r;
   g->last_op = 2;
   g->last_src = src;
   g->last_dst = dst;

@@ -66,6 +66,7 @@ struct Counts {
   uint32_t loop_end[2];           // step index at which k_expand_loop<MODE> stopped
   uint32_t bar_done;              // grid barriers completed (k_expand_loop)
   unsigned long long large[3];    // (entries << 32) | bucket items
+  unsigned long long pull_large;  // k_reseed_pull's hub list (entries << 32) | items
   unsigned long long stats[kStatSlots][16];  // [slot][set * 4 + which], one line per slot
 };
 
@@ -148,7 +149,7 @@ struct TravView {
 // queue side `side`, counter slot `slot`: one 64-bit atomic per call claims
 // the entries and the bucket work items of all of them (an exclusive prefix
 // of nb over the lanes gives each its first item).  Wave-collective.
-MK_DEV void push_hubs(const TravView &S, uint32_t side, uint32_t slot, bool hub, uint32_t v,
+MK_DEV void push_hubs(uint2 *queue, unsigned long long *count, bool hub, uint32_t v,
                       uint32_t nb) {
   const uint64_t hm = ballot(hub);
   if (hm == 0) return;
@@ -163,11 +164,10 @@ MK_DEV void push_hubs(const TravView &S, uint32_t side, uint32_t slot, bool hub,
   const uint32_t leader = ffs64(hm);
   unsigned long long r = 0;
   if (lane == leader)
-    r = atomicAdd(&S.cnt->large[slot], (uint64_t(__popcll(hm)) << 32) | total);
+    r = atomicAdd(count, (uint64_t(__popcll(hm)) << 32) | total);
   const uint32_t first = read_lane(uint32_t(r >> 32), leader);
   const uint32_t item0 = read_lane(uint32_t(r), leader);
-  if (hub)
-    qstore2(S.q_large[side] + first + lanes_below(hm), make_uint2(v, item0 + x - nb));
+  if (hub) qstore2(queue + first + lanes_below(hm), make_uint2(v, item0 + x - nb));
 }
 
 // Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
@@ -185,7 +185,7 @@ MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t si
   if (q.n + c > kWaveQ) wq_flush(q, S.q_small[side], &S.cnt->small[slot]);
   if (small) q.buf[q.n + lanes_below(sm)] = v;
   q.n += c;
-  push_hubs(S, side, slot, push && !small, v, nb);
+  push_hubs(S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
   q.pushed += uint32_t(__popcll(pm));
 }
 
@@ -267,7 +267,7 @@ struct TailSink {
       }
     }
     // hubs go to the next step's bucket work list
-    push_hubs(S, side, slot, push && !small, v, nb);
+    push_hubs(S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
     if (push && !small) ctl[1] = 1u;
     pushed += uint32_t(__popcll(pm));
   }
@@ -296,7 +296,7 @@ MK_DEV void block_enqueue(const GraphView &G, const TravView &S, uint32_t side, 
   }
   __syncthreads();
   if (small) qstore(S.q_small[side] + sh[NW] + sh[w] + lanes_below(sm), v);
-  push_hubs(S, side, slot, push && !small, v, nb);
+  push_hubs(S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
   const uint64_t pm = ballot(push);
   if (lane == 0) add_stat(S.cnt, kSetSeed, kStatPushed, uint32_t(__popcll(pm)));
 }
@@ -606,12 +606,18 @@ MK_DEV bool grid_barrier(BarLines *bl, Counts *c, uint32_t k, uint64_t timeout, 
 // (j - j0) * kEpochStride onwards.  cnt->bar_done counts the grid barriers
 // completed since the barrier counters were zeroed (read at the start,
 // advanced at the end), so launches queued back to back need no host.
+// Housekeeping folded in to save launches: words zero[0..nzero) are
+// cleared at the start (no step reads them), and with `clear_exit` the
+// frontier counters of every slot and cnt->ninv are cleared at the end
+// (after the last step every block reads only the empty slot's counts).
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
-                  uint64_t timeout) {
+                  uint64_t timeout, uint32_t *zero, uint32_t nzero, uint32_t clear_exit) {
   __shared__ uint32_t lds[kExpandLds];
   __shared__ uint32_t flag;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += gridDim.x * blockDim.x)
+    zero[i] = 0u;
   // written by an earlier launch on this stream; every block reads it
   // before its first barrier arrival, so block 0's update at the end
   // cannot race with it
@@ -630,6 +636,13 @@ __global__ void __launch_bounds__(kBlock)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S.cnt->loop_end[MODE] = j;
     S.cnt->bar_done = bdone + (j - j0);
+    if (clear_exit) {
+      for (uint32_t k = 0; k < 3; ++k) {
+        S.cnt->small[k] = 0u;
+        S.cnt->large[k] = 0ull;
+      }
+      S.cnt->ninv = 0u;
+    }
   }
 }
 
@@ -747,7 +760,8 @@ MK_DEV void pull_buckets(const GraphView &G, const GraphView &R, const TravView 
 // bit of invbits) takes the minimum of dist[p] + w over its valid, reached
 // in-neighbours p.  Waves take 64 bitmap words at a time; each set bit
 // becomes one lane's vertex.  Vertices with more than kSmallBuckets
-// in-buckets are listed (queue side 1, slot 1, as bucket work items) for
+// in-buckets are listed (queue side 1 storage, counter cnt->pull_large, as
+// bucket work items) for
 // k_pull_hubs, which spreads their buckets over all waves.
 template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock)
@@ -770,7 +784,7 @@ __global__ void __launch_bounds__(kBlock)
       rnb = R.bucket_off[v + 1] - rb0;
     }
     const bool hub = act && rnb > kSmallBuckets;
-    push_hubs(S, 1, 1, hub, v, rnb);
+    push_hubs(S.q_large[1], &S.cnt->pull_large, hub, v, rnb);
     if (hub) act = false;
     visited += uint32_t(__popcll(ballot(act)));
     for (uint32_t k = 0; ballot(act && rnb > k); ++k)
@@ -841,7 +855,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
-  const unsigned long long lg = S.cnt->large[1];
+  const unsigned long long lg = S.cnt->pull_large;
   const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
   if (n_large == 0) return;
   __shared__ uint32_t lq[kWaveLds];
@@ -1227,11 +1241,13 @@ struct Traversal {
   }
 
   // One cooperative launch of k_expand_loop over steps [j0, j1).
-  template <int MODE> int launch_loop(uint32_t j0, uint32_t j1, uint32_t ep) {
+  template <int MODE>
+  int launch_loop(uint32_t j0, uint32_t j1, uint32_t ep, uint32_t *zero = nullptr,
+                  uint32_t nzero = 0, uint32_t clear_exit = 0) {
     GraphView G = g->view();
     TravView S = view();
     uint64_t to = barrier_timeout;
-    void *args[] = {&G, &S, &j0, &j1, &ep, &to};
+    void *args[] = {&G, &S, &j0, &j1, &ep, &to, &zero, &nzero, &clear_exit};
     ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
     MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
                                       dim3(kBlock), args, 0u, stream));
@@ -1298,18 +1314,22 @@ struct Traversal {
   static constexpr uint32_t kLoopSteps = 256;
   uint32_t relax_end = 0;     // range end of the queued relax loop (0: none)
   bool inval_queued = false;  // an invalidation loop was queued
+  bool dirty = true;          // counters may be non-zero (cleared at the next begin)
   int reseed_kind = 0;        // queued re-seed: 0 none, 1 pull (mirror), 2 bucket scan
 
   uint32_t inval_limit() const {
     return uint32_t(std::min<uint64_t>(uint64_t(g->n) + 8, 0xFFFFFFF0ull));
   }
-  int queue_relax() {
+  // `zero`: words the launch clears first (the invalidation bitmap after a
+  // re-seed).  The invalidation loop leaves the frontier counters and ninv
+  // cleared for the re-seed.
+  int queue_relax(uint32_t *zero = nullptr, uint32_t nzero = 0) {
     relax_end = kLoopSteps;
-    return launch_loop<0>(0, kLoopSteps, reserve_epochs(kLoopSteps));
+    return launch_loop<0>(0, kLoopSteps, reserve_epochs(kLoopSteps), zero, nzero);
   }
   int queue_invalidate() {
     inval_queued = true;
-    return launch_loop<1>(0, inval_limit(), 0u);
+    return launch_loop<1>(0, inval_limit(), 0u, nullptr, 0u, 1u);
   }
 
   // end()'s part for queued loops: counters `h` were just read.  Algorithmic
@@ -1408,12 +1428,16 @@ struct Traversal {
     return MK_OK;
   }
 
+  // Counters are zero between calls: end() queues their clearing after it
+  // read them (off the next call's path); a call that stopped early leaves
+  // them `dirty` for the next begin() to clear.
   int begin() {
     MK_HIP(hipSetDevice(g->device));
     relax_end = 0;
     inval_queued = false;
     reseed_kind = 0;
-    clear_counts();
+    if (dirty) clear_counts();
+    dirty = true;
     MK_HIP(hipEventRecord(ev0, stream));
     return MK_OK;
   }
@@ -1428,6 +1452,8 @@ struct Traversal {
     inval_queued = false;
     reseed_kind = 0;
     if (lr) return lr;
+    clear_counts();
+    dirty = false;
     float t = 0.f;
     MK_HIP(hipEventElapsedTime(&t, ev0, ev1));
     if (ms) *ms = t;
@@ -1487,11 +1513,9 @@ struct Traversal {
     const GraphView G = g->view();
     if (int r = queue_invalidate()) return r;
     const uint32_t ep = next_epoch();
-    clear_slots();
+    const uint32_t nwords = uint32_t((uint64_t(g->n) + 31) / 32);
     if (g->rev) {
       const GraphView R = g->rev->view();
-      const uint32_t nwords = uint32_t((uint64_t(g->n) + 31) / 32);
-      fill_u32(&cnt->ninv, 0u, 1, stream);
       k_inv_compact<<<blocks_for_threads((uint64_t(nwords) + kInvWords - 1) / kInvWords), kBlock,
                       0, stream>>>(view(), nwords);
       const unsigned grid = blocks_for_waves(uint64_t(max_waves));
@@ -1501,9 +1525,6 @@ struct Traversal {
                             : k_pull_hubs<false, false>;
       prof_launch("k_reseed_pull", 0.0, kp, dim3(grid), dim3(kBlock), 0u, stream, G, R, view(), ep);
       prof_launch("k_pull_hubs", 0.0, kh, dim3(grid), dim3(kBlock), 0u, stream, G, R, view(), ep);
-      // the hub list used slot 1, which the first relax step appends to
-      fill_u32(cnt->small + 1, 0u, 1, stream);
-      fill_u64(reinterpret_cast<uint64_t *>(cnt->large + 1), 0ull, 1, stream);
       reseed_kind = 1;
     } else {
       const unsigned grid = blocks_for_waves(std::min<uint64_t>(
@@ -1514,8 +1535,7 @@ struct Traversal {
       reseed_kind = 2;
     }
     MK_HIP(hipGetLastError());
-    fill_u32(invbits, 0u, (uint64_t(g->n) + 31) / 32, stream);
-    if (int r = queue_relax()) return r;
+    if (int r = queue_relax(invbits, nwords)) return r;
     return end(ms);
   }
 
@@ -1567,9 +1587,6 @@ struct Traversal {
           // hub half of the pull
           prof_launch("k_pull_hubs", 0.0, kh, dim3(hgrid), dim3(kBlock), 0u, stream, G, R, view(),
                       ep);
-          // the hub list used slot 1, which the first relax step appends to
-          fill_u32(cnt->small + 1, 0u, 1, stream);
-          fill_u64(reinterpret_cast<uint64_t *>(cnt->large + 1), 0ull, 1, stream);
         } else {
           const unsigned grid = blocks_for_waves(std::min<uint64_t>(
               (uint64_t(g->total_buckets) + 63) / 64, uint64_t(max_waves) * 2));
