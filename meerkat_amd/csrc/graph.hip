@@ -53,6 +53,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"ms_unroll", 8},       // neighbour rows in flight per wave in multi-source SSSP
     {"orphan_repair", 0},   // decremental with the in-edge mirror re-parents orphans first
     {"persistent_steps", 1},  // BFS/SSSP frontier steps in one launch with grid barriers
+    {"tail_enter", 512},    // frontiers up to this size run in one block's LDS (0: never)
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {
@@ -457,6 +458,8 @@ int mk_tune_set(const char *name, int64_t value) {
     return fail(MK_ERR_BAD_ARG, "ms_unroll must be 4 or 8");
   if ((k == "orphan_repair" || k == "persistent_steps") && value != 0 && value != 1)
     return fail(MK_ERR_BAD_ARG, k + " must be 0 or 1");
+  if (k == "tail_enter" && (value < 0 || value > 1024))
+    return fail(MK_ERR_BAD_ARG, "tail_enter must be in [0, 1024]");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(k);
   if (it == g_tune.end()) return fail(MK_ERR_BAD_ARG, "mk_tune_set: unknown knob " + k);

@@ -143,6 +143,7 @@ struct TravView {
   uint2 *q_large[2];
   Counts *cnt;
   struct BarLines *bar;  // grid-barrier counters (k_expand_loop)
+  uint32_t tail_enter, tail_stay;  // tail-mode thresholds (0: no tail mode)
 };
 
 // Append the hub vertices v (lanes with `hub`, nb buckets each) to hub
@@ -226,8 +227,9 @@ struct GlobalSink {
 };
 
 constexpr uint32_t kTailCap = 2048;    // LDS queue entries per tail-mode block
-constexpr uint32_t kTailEnter = 512;   // frontiers up to this size run in tail mode
-constexpr uint32_t kTailStay = 768;    // a tail frontier above this is handed back
+// Frontiers of up to S.tail_enter vertices (default 512, the knob
+// "tail_enter") run in tail mode; a tail frontier above S.tail_stay (1.5x)
+// is handed back.
 constexpr uint32_t kEpochStride = 256; // epochs reserved per launch (tail steps)
 
 struct TailSink {
@@ -423,7 +425,7 @@ MK_DEV void expand_rows(const GraphView &G, const TravView &S, Sink &sink, uint3
 // following ones, frontiers ping-ponging between two LDS queues, while they
 // stay small; step j uses epoch + j.  The frontier is handed back (written
 // to the global queues of the next step, where the next launch reads it)
-// when it grows past kTailStay, overflows the LDS queue, reaches a hub, or
+// when it grows past S.tail_stay, overflows the LDS queue, reaches a hub, or
 // the epoch range reserved for this launch runs out.
 template <bool MAP, bool WEIGHTS, int MODE>
 MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
@@ -448,7 +450,7 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
     __syncthreads();
     const uint32_t nb = c[0], handover = c[1];
     if (nb == 0 && handover == 0) break;  // the traversal is done
-    if (handover || nb > kTailStay || j + 2 >= kEpochStride) {
+    if (handover || nb > S.tail_stay || j + 2 >= kEpochStride) {
       // append the LDS part of the frontier to the next step's global queue
       const uint32_t m = min(nb, kTailCap);
       for (uint32_t i0 = wave * 64; i0 < m; i0 += NW * 64) {
@@ -505,7 +507,7 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
     if (n_small + n_large) atomicAdd(&S.cnt->steps, 1u);
   }
   if (n_small + n_large == 0) return;
-  if (n_large == 0 && n_small <= kTailEnter) {
+  if (n_large == 0 && n_small <= S.tail_enter) {
     if (blockIdx.x == 0) expand_tail<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, n_small, lds);
     return;
   }
@@ -1114,6 +1116,7 @@ struct Traversal {
   int max_waves = 0;
   bool repair = false;  // orphan repair before invalidation (with the in-edge mirror)
   bool persistent = true;        // frontier steps in one launch (k_expand_loop)
+  uint32_t tail_enter = 512;     // tail-mode entry threshold (knob "tail_enter")
   unsigned loop_blocks[2] = {0, 0};  // resident-grid size per MODE
   uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
 
@@ -1129,6 +1132,8 @@ struct Traversal {
     v.q_large[1] = ql[1];
     v.cnt = cnt;
     v.bar = bar;
+    v.tail_enter = tail_enter;
+    v.tail_stay = std::min<uint32_t>(tail_enter + tail_enter / 2, kTailCap);
     return v;
   }
 
@@ -1165,6 +1170,7 @@ struct Traversal {
     max_waves = grid_waves(g->device);
     repair = tune_get("orphan_repair", 0) != 0;
     persistent = tune_get("persistent_steps", 1) != 0;
+    tail_enter = uint32_t(tune_get("tail_enter", 512));
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
         khz <= 0)

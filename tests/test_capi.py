@@ -96,12 +96,13 @@ def test_tune_knobs_validate_without_gpu():
 
     before = autotune.knobs()
     assert before == {"waves_per_cu": 16, "ms_unroll": 8, "orphan_repair": 0,
-                      "persistent_steps": 1}
+                      "persistent_steps": 1, "tail_enter": 512}
     try:
         autotune.set_knob("waves_per_cu", 32)
         assert autotune.get_knob("waves_per_cu") == 32
         for name, bad in [("waves_per_cu", 0), ("waves_per_cu", 65), ("ms_unroll", 6),
-                          ("orphan_repair", 2), ("persistent_steps", 3), ("no_such_knob", 1)]:
+                          ("orphan_repair", 2), ("persistent_steps", 3), ("tail_enter", 1025),
+                          ("no_such_knob", 1)]:
             with pytest.raises(_lib.MeerkatError) as e:
                 autotune.set_knob(name, bad)
             assert e.value.code == _lib.MK_ERR_BAD_ARG
