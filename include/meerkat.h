@@ -277,7 +277,7 @@ int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
  * "persistent_steps" (0 or 1, default 1): BFS/SSSP frontier steps run in
  * one cooperative launch with a grid barrier between steps (0: one launch
  * per step, the host polling the frontier counters).  "tail_enter"
- * (0..1024, default 512): frontiers of at most this many vertices (and no
+ * (0..1024, default 128): frontiers of at most this many vertices (and no
  * hub) are expanded by one block step after step in LDS (0: never).
  * The Python autotuner (meerkat_amd/autotune.py) sets them from
  * measurements. */

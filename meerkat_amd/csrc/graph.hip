@@ -53,7 +53,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"ms_unroll", 8},       // neighbour rows in flight per wave in multi-source SSSP
     {"orphan_repair", 0},   // decremental with the in-edge mirror re-parents orphans first
     {"persistent_steps", 1},  // BFS/SSSP frontier steps in one launch with grid barriers
-    {"tail_enter", 512},    // frontiers up to this size run in one block's LDS (0: never)
+    {"tail_enter", 128},    // frontiers up to this size run in one block's LDS (0: never)
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {

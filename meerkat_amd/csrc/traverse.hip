@@ -227,7 +227,7 @@ struct GlobalSink {
 };
 
 constexpr uint32_t kTailCap = 2048;    // LDS queue entries per tail-mode block
-// Frontiers of up to S.tail_enter vertices (default 512, the knob
+// Frontiers of up to S.tail_enter vertices (default 128, the knob
 // "tail_enter") run in tail mode; a tail frontier above S.tail_stay (1.5x)
 // is handed back.
 constexpr uint32_t kEpochStride = 256; // epochs reserved per launch (tail steps)
@@ -1116,7 +1116,7 @@ struct Traversal {
   int max_waves = 0;
   bool repair = false;  // orphan repair before invalidation (with the in-edge mirror)
   bool persistent = true;        // frontier steps in one launch (k_expand_loop)
-  uint32_t tail_enter = 512;     // tail-mode entry threshold (knob "tail_enter")
+  uint32_t tail_enter = 128;     // tail-mode entry threshold (knob "tail_enter")
   unsigned loop_blocks[2] = {0, 0};  // resident-grid size per MODE
   uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
 
@@ -1170,7 +1170,7 @@ struct Traversal {
     max_waves = grid_waves(g->device);
     repair = tune_get("orphan_repair", 0) != 0;
     persistent = tune_get("persistent_steps", 1) != 0;
-    tail_enter = uint32_t(tune_get("tail_enter", 512));
+    tail_enter = uint32_t(tune_get("tail_enter", 128));
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
         khz <= 0)
