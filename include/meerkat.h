@@ -19,9 +19,10 @@
  *    algorithm handle built on it (checked: destroy returns MK_ERR_STATE).
  *  - Vertex ids are uint32 in [0, n); 0xFFFFFFFF and 0xFFFFFFFE are reserved
  *    (empty / tombstone slab words).
- *  - `stream` arguments are hipStream_t values passed as void* (NULL = the
- *    null stream).  Graph mutations are ordered on that stream and complete
- *    before the call returns.  Algorithm compute calls block until done and
+ *  - `stream` arguments are hipStream_t values passed as void*.  Graph
+ *    mutations are ordered on that stream and complete before the call
+ *    returns; NULL selects the graph's own stream, which (created with
+ *    default flags) still orders after work queued on the null stream.  Algorithm compute calls block until done and
  *    report elapsed milliseconds (HIP events), like the reference's float
  *    returns (include/Algorithms/impl/BFSImpl.cuh:644-661).
  */

@@ -591,6 +591,19 @@ static int graph_alloc(mk_pool *pool, uint32_t n, int weighted, double load_fact
     set_error(std::string("graph create: init: ") + hipGetErrorString(e));
     return MK_ERR_HIP;
   }
+  // mutations the caller gives no stream run on this one (default flags, so
+  // it still orders after the caller's NULL-stream work): launching on the
+  // NULL stream itself makes the runtime order it against every stream
+  e = hipStreamCreate(&g->own_stream);
+  if (e != hipSuccess) {
+    (void)hipFree(g->heads);
+    (void)hipFree(g->bucket_off);
+    (void)hipFree(g->bucket_owner);
+    (void)hipFree(g->vdeg);
+    (void)hipFree(g->counters);
+    delete g;
+    return hip_fail(e, "graph create: stream");
+  }
   g->bucket_off_host.swap(off);
   pool->graphs++;
   *out = g;
@@ -647,6 +660,7 @@ int mk_graph_destroy(mk_graph *g) {
   (void)hipFree(g->bucket_owner);
   (void)hipFree(g->vdeg);
   (void)hipFree(g->counters);
+  if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
   g->status.release();
   g->host_scratch.release();
   g->pool->graphs--;
@@ -756,7 +770,7 @@ static int mirror_prologue(mk_graph *g, uint32_t n) {
 
 int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
                     uint32_t n, void *stream) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : (g ? g->own_stream : nullptr);
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_insert: mirror is read-only");
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
   if (n == 0) return MK_OK;
@@ -785,7 +799,7 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
 
 int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
                     void *stream) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : (g ? g->own_stream : nullptr);
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_delete: mirror is read-only");
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
   if (n == 0) return MK_OK;
@@ -806,7 +820,7 @@ int mk_graph_search(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_search: NULL graph");
   if (n && (!src || !dst || !found)) return fail(MK_ERR_BAD_ARG, "mk_graph_search: NULL buffer");
   if (n == 0) return MK_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : (g ? g->own_stream : nullptr);
   MK_HIP(hipSetDevice(g->device));
   const uint32_t epw = edges_per_wave(n, g->device);
   const uint64_t waves = (uint64_t(n) + epw - 1) / epw;

@@ -222,6 +222,7 @@ struct mk_graph {
   uint32_t *vdeg = nullptr;
   uint32_t *counters = nullptr;  // [0] range-check epoch [2..3] pool failures (u64)
   uint32_t mut_epoch = 0;        // epoch of the last mutation call
+  hipStream_t own_stream = nullptr;  // mutations given no stream run here
   std::vector<uint32_t> bucket_off_host;
   mk::DevBuf status;             // uint8 per edge of the last batch
   uint32_t status_n = 0;
