@@ -296,6 +296,13 @@ int mk_profile_reset(void);
 int mk_profile_count(int *count);
 int mk_profile_get(int i, const char **name, uint64_t *launches, double *total_ms,
                    double *algo_bytes);
+/* Frontier steps of the persistent BFS/SSSP loops while profiling (the
+ * first 64 steps of each loop launch): kernel name, step index, frontier
+ * size (vertices, hub bucket items) and the step's duration in
+ * microseconds from the device wall clock, grid barrier included. */
+int mk_profile_step_count(int *count);
+int mk_profile_step_get(int i, const char **name, uint32_t *step, uint32_t *small,
+                        uint32_t *hub_items, double *us);
 
 /* ------------------------------------------------------ workload generators
  * Host-side, seeded, thread-count independent.  They replace the text edge

@@ -30,6 +30,13 @@ std::vector<Pending> g_pending;
 std::vector<hipEvent_t> g_free;
 std::map<std::string, Total> g_totals;
 std::vector<std::string> g_names;  // stable storage for returned names
+struct Step {
+  const char *name;
+  uint32_t step, small, hub_items;
+  double us;
+};
+std::vector<Step> g_steps;
+constexpr size_t kMaxSteps = 1 << 16;
 
 hipEvent_t take_event() {
   if (!g_free.empty()) {
@@ -106,6 +113,12 @@ void prof_add_bytes(const char *name, double algo_bytes) {
   g_totals[name].bytes += algo_bytes;
 }
 
+void prof_step(const char *name, uint32_t step, uint32_t small, uint32_t hub_items, double us) {
+  if (!profiling_on()) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_steps.size() < kMaxSteps) g_steps.push_back({name, step, small, hub_items, us});
+}
+
 }  // namespace mk
 
 using namespace mk;
@@ -121,6 +134,28 @@ int mk_profile_reset(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   resolve_locked();
   g_totals.clear();
+  g_steps.clear();
+  return MK_OK;
+}
+
+int mk_profile_step_count(int *count) {
+  if (!count) return fail(MK_ERR_BAD_ARG, "mk_profile_step_count: NULL count");
+  std::lock_guard<std::mutex> lk(g_mu);
+  *count = int(g_steps.size());
+  return MK_OK;
+}
+
+int mk_profile_step_get(int i, const char **name, uint32_t *step, uint32_t *small,
+                        uint32_t *hub_items, double *us) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (i < 0 || size_t(i) >= g_steps.size())
+    return fail(MK_ERR_BAD_ARG, "mk_profile_step_get: index out of range");
+  const Step &s = g_steps[size_t(i)];
+  if (name) *name = s.name;
+  if (step) *step = s.step;
+  if (small) *small = s.small;
+  if (hub_items) *hub_items = s.hub_items;
+  if (us) *us = s.us;
   return MK_OK;
 }
 

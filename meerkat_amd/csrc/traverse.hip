@@ -50,6 +50,7 @@ constexpr uint32_t kBatch = 4;       // slab loads in flight per wave
 
 constexpr uint32_t kWaveQ = 512;     // LDS staging entries per wave
 constexpr uint32_t kStatSlots = 128;  // statistics spread over this many 128-byte lines
+constexpr uint32_t kTraceSteps = 64;  // steps of a loop launch recorded while profiling
 enum { kStatSlabs = 0, kStatEdges = 1, kStatVisited = 2, kStatPushed = 3 };
 // statistics sets, one per kind of kernel: frontier steps relaxing (MODE 0),
 // invalidating (MODE 1), pull/bucket re-seeds (MODE 2), batch seeds
@@ -68,6 +69,12 @@ struct Counts {
   unsigned long long large[3];    // (entries << 32) | bucket items
   unsigned long long pull_large;  // k_reseed_pull's hub list (entries << 32) | items
   unsigned long long stats[kStatSlots][16];  // [slot][set * 4 + which], one line per slot
+  // step trace of k_expand_loop<MODE> while profiling: the device clock at
+  // the start of step k (k = steps: the loop's end) and its frontier
+  unsigned long long trace_t[2][kTraceSteps + 1];
+  uint32_t trace_small[2][kTraceSteps];
+  uint32_t trace_items[2][kTraceSteps];
+  uint32_t trace_n[2];
 };
 
 // Host-side totals of a Counts snapshot.
@@ -144,6 +151,7 @@ struct TravView {
   Counts *cnt;
   struct BarLines *bar;  // grid-barrier counters (k_expand_loop)
   uint32_t tail_enter, tail_stay;  // tail-mode thresholds (0: no tail mode)
+  uint32_t trace;                  // record the step trace (profiling)
 };
 
 // Append the hub vertices v (lanes with `hub`, nb buckets each) to hub
@@ -630,12 +638,22 @@ __global__ void __launch_bounds__(kBlock)
     // every block reads the same settled counts: the uniform stop decision
     const uint32_t ns = coherent_load(&S.cnt->small[slot]);
     const uint64_t nl = coherent_load64(reinterpret_cast<const uint64_t *>(&S.cnt->large[slot]));
+    if (S.trace && blockIdx.x == 0 && threadIdx.x == 0 && j - j0 < kTraceSteps) {
+      S.cnt->trace_t[MODE][j - j0] = wall_clock64();
+      S.cnt->trace_small[MODE][j - j0] = ns;
+      S.cnt->trace_items[MODE][j - j0] = uint32_t(nl);
+    }
     if (ns == 0 && (nl >> 32) == 0) break;
     expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot,
                                     MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
     if (!grid_barrier(S.bar, S.cnt, bdone + (j - j0) + 1, timeout, &flag)) return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (S.trace) {
+      const uint32_t k = min(j - j0, kTraceSteps);
+      S.cnt->trace_t[MODE][k] = wall_clock64();
+      S.cnt->trace_n[MODE] = k;
+    }
     S.cnt->loop_end[MODE] = j;
     S.cnt->bar_done = bdone + (j - j0);
     if (clear_exit) {
@@ -1119,6 +1137,7 @@ struct Traversal {
   uint32_t tail_enter = 128;     // tail-mode entry threshold (knob "tail_enter")
   unsigned loop_blocks[2] = {0, 0};  // resident-grid size per MODE
   uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
+  int clock_khz = 100000;        // device wall-clock rate
 
   TravView view() const {
     TravView v;
@@ -1133,6 +1152,7 @@ struct Traversal {
     v.cnt = cnt;
     v.bar = bar;
     v.tail_enter = tail_enter;
+    v.trace = profiling_on() ? 1u : 0u;
     v.tail_stay = std::min<uint32_t>(tail_enter + tail_enter / 2, kTailCap);
     return v;
   }
@@ -1175,6 +1195,7 @@ struct Traversal {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
         khz <= 0)
       khz = 100000;
+    clock_khz = khz;
     barrier_timeout = uint64_t(khz) * 5000ull;  // five seconds
     if (persistent) {
       if (int r = loop_grid<0>(loop_blocks[0])) return r;
@@ -1338,6 +1359,20 @@ struct Traversal {
     return launch_loop<1>(0, inval_limit(), 0u, nullptr, 0u, 1u);
   }
 
+  // Step trace of the loops of this call (the host copy of the counters).
+  void record_steps() {
+    const Counts &c = *host.as<Counts>();
+    const double us_per_tick = 1000.0 / double(clock_khz);
+    for (int mode = 0; mode < 2; ++mode) {
+      if (mode == 0 ? !relax_end : !inval_queued) continue;
+      const uint32_t k = std::min(c.trace_n[mode], kTraceSteps);
+      for (uint32_t i = 0; i < k; ++i)
+        prof_step(mode == 0 ? "k_expand_relax" : "k_expand_invalidate", i, c.trace_small[mode][i],
+                  c.trace_items[mode][i],
+                  double(c.trace_t[mode][i + 1] - c.trace_t[mode][i]) * us_per_tick);
+    }
+  }
+
   // end()'s part for queued loops: counters `h` were just read.  Algorithmic
   // bytes (profiling) per phase from its statistics set, as in iterate and
   // run_decremental.
@@ -1458,6 +1493,7 @@ struct Traversal {
     inval_queued = false;
     reseed_kind = 0;
     if (lr) return lr;
+    if (profiling_on() && persistent) record_steps();
     clear_counts();
     dirty = false;
     float t = 0.f;

@@ -34,6 +34,23 @@ def totals():
     return out
 
 
+def steps():
+    """Frontier steps of the persistent BFS/SSSP loops recorded while
+    profiling: [{"kernel", "step", "frontier", "hub_items", "us"}], in
+    order (step 0 starts each loop launch)."""
+    n = ctypes.c_int()
+    _lib.call("mk_profile_step_count", ctypes.byref(n))
+    out = []
+    for i in range(n.value):
+        name = ctypes.c_char_p()
+        step, small, hubs, us = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        _lib.call("mk_profile_step_get", i, ctypes.byref(name), ctypes.byref(step),
+                  ctypes.byref(small), ctypes.byref(hubs), ctypes.byref(us))
+        out.append({"kernel": name.value.decode(), "step": step.value, "frontier": small.value,
+                    "hub_items": hubs.value, "us": us.value})
+    return out
+
+
 # ------------------------------------------------------------ profile markers
 MARKER_KERNEL = "FillFunctor<short>"
 
