@@ -1489,11 +1489,11 @@ struct Traversal {
     Totals h;
     if (int r = read_counts(h)) return r;
     const int lr = finish_loops(h);
+    if (!lr && profiling_on()) record_steps();
     relax_end = 0;
     inval_queued = false;
     reseed_kind = 0;
     if (lr) return lr;
-    if (profiling_on() && persistent) record_steps();
     clear_counts();
     dirty = false;
     float t = 0.f;
