@@ -182,12 +182,10 @@ MK_DEV void push_hubs(uint2 *queue, unsigned long long *count, bool hub, uint32_
 // Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
 // small vertices through the wave's LDS staging, hub vertices directly.
 // Wave-collective.
-MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
-                    uint32_t slot, bool push, uint32_t v) {
+MK_DEV void enqueue_nb(const TravView &S, WaveQ &q, uint32_t side, uint32_t slot, bool push,
+                       uint32_t v, uint32_t nb) {
   const uint64_t pm = ballot(push);
   if (pm == 0) return;
-  uint32_t nb = 0;
-  if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
   const bool small = push && nb <= kSmallBuckets;
   const uint64_t sm = ballot(small);
   const uint32_t c = uint32_t(__popcll(sm));
@@ -196,6 +194,17 @@ MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t si
   q.n += c;
   push_hubs(S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
   q.pushed += uint32_t(__popcll(pm));
+}
+
+// Bucket count of v for the lanes that push it (0 elsewhere).
+MK_DEV uint32_t push_buckets(const GraphView &G, bool push, uint32_t v) {
+  return push ? G.bucket_off[v + 1] - G.bucket_off[v] : 0u;
+}
+
+MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
+                    uint32_t slot, bool push, uint32_t v) {
+  if (ballot(push) == 0) return;
+  enqueue_nb(S, q, side, slot, push, v, push_buckets(G, push, v));
 }
 
 // wq_flush for every wave of the block at once, one atomic per block;
@@ -232,6 +241,10 @@ struct GlobalSink {
   MK_DEV void operator()(const GraphView &G, bool push, uint32_t v) {
     enqueue(G, S, q, side, slot, push, v);
   }
+  // the pushing lanes' bucket counts already loaded
+  MK_DEV void operator()(const GraphView &, bool push, uint32_t v, uint32_t nb) {
+    enqueue_nb(S, q, side, slot, push, v, nb);
+  }
 };
 
 constexpr uint32_t kTailCap = 2048;    // LDS queue entries per tail-mode block
@@ -247,11 +260,14 @@ struct TailSink {
   uint32_t side, slot;  // global queues of the next step
   uint32_t pushed;      // wave-uniform statistics
   MK_DEV void operator()(const GraphView &G, bool push, uint32_t v) {
+    if (ballot(push) == 0) return;
+    (*this)(G, push, v, push_buckets(G, push, v));
+  }
+  // the pushing lanes' bucket counts already loaded
+  MK_DEV void operator()(const GraphView &, bool push, uint32_t v, uint32_t nb) {
     const uint64_t pm = ballot(push);
     if (pm == 0) return;
     const uint32_t lane = lane_id();
-    uint32_t nb = 0;
-    if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
     const bool small = push && nb <= kSmallBuckets;
     const uint64_t sm = ballot(small);
     const uint32_t c = uint32_t(__popcll(sm));
@@ -371,8 +387,72 @@ MK_DEV void chain_work(const GraphView &G, const TravView &S, Sink &sink, uint32
   }
 }
 
+// slab_work over c <= kBatch slabs at once (slab k of vertex u[k] with
+// distance du[k], this lane's word v[k]), phase by phase, so that the
+// dependent memory operations of the slabs overlap instead of following
+// one another: every distance (invalid bit) load, then every atomic, then
+// every stamp (bitmap) update, then every pushed vertex's bucket count,
+// then the pushes.  Wave-collective; returns live keys seen.
+template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
+MK_DEV uint32_t batch_slab_work(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
+                                uint32_t c, const uint32_t (&u)[kBatch],
+                                const uint32_t (&du)[kBatch], const uint32_t (&v)[kBatch]) {
+  const uint32_t lane = lane_id();
+  const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
+  bool live[kBatch], ok[kBatch];
+  uint32_t D[kBatch];
+  uint64_t cur[kBatch];
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) {
+    live[k] = k < c && key_lane && v[k] < kTomb;
+    const uint32_t wv = MAP ? __shfl_down(v[k], 1, 64) : 1u;
+    D[k] = du[k] + (WEIGHTS ? wv : 1u);
+    ok[k] = live[k] && (MODE == 1 || D[k] >= du[k]);
+    cur[k] = kInfInf;
+    if (MODE == 2) {
+      if (ok[k]) ok[k] = inv_bit(S.invbits, v[k]);
+    } else if (ok[k]) {
+      // MODE 1: values only move to INF during this pass, so a plain load
+      // suffices; MODE 0: a stale value only lets an extra atomic through
+      cur[k] = S.dist[v[k]];
+    }
+  }
+  bool push[kBatch];
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) {
+    push[k] = false;
+    if (MODE == 1) {
+      if (ok[k] && cur[k] != kInfInf && parent_of(cur[k]) == u[k])
+        push[k] = atomicCAS((unsigned long long *)(S.dist + v[k]), (unsigned long long)cur[k],
+                            (unsigned long long)kInfInf) == cur[k];
+    } else if (ok[k] && (MODE == 2 || D[k] < dist_of(cur[k]))) {
+      cur[k] = atomicMin((unsigned long long *)(S.dist + v[k]), (unsigned long long)pack(D[k], u[k]));
+      push[k] = D[k] < dist_of(cur[k]);  // improved
+    }
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) {
+    if (MODE == 1) {
+      if (push[k]) atomicOr(S.invbits + (v[k] >> 5), 1u << (v[k] & 31u));
+    } else if (push[k]) {
+      push[k] = atomicExch(S.stamp + v[k], epoch) != epoch;
+    }
+  }
+  uint32_t nb[kBatch];
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) nb[k] = push_buckets(G, push[k], v[k]);
+  uint32_t edges = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) {
+    if (k < c) sink(G, push[k], v[k], nb[k]);
+    edges += uint32_t(__popcll(ballot(live[k])));
+  }
+  return edges;
+}
+
 // Process the buckets named by lanes of `m`: lane j owns bucket bk (its
-// vertex uj with distance dj).  Slabs are loaded kBatch at a time.
+// vertex uj with distance dj).  Head slabs are loaded and worked kBatch at
+// a time (batch_slab_work); chained slabs follow one by one.
 template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
 MK_DEV void batched_buckets(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
                             uint64_t m, uint32_t bk, uint32_t uj, uint32_t dj, uint32_t &slabs,
@@ -389,15 +469,22 @@ MK_DEV void batched_buckets(const GraphView &G, const TravView &S, Sink &sink, u
         c = k + 1;
       }
     }
-    uint32_t v[kBatch];
+    uint32_t v[kBatch], us[kBatch], ds[kBatch];
 #pragma unroll
-    for (uint32_t k = 0; k < kBatch; ++k)
+    for (uint32_t k = 0; k < kBatch; ++k) {
       v[k] = k < c ? G.heads[size_t(read_lane(bk, js[k])) * kSlabWords + lane] : kEmpty;
+      us[k] = read_lane(uj, js[k]);
+      ds[k] = read_lane(dj, js[k]);
+    }
+    slabs += c;
+    edges += batch_slab_work<MAP, WEIGHTS, MODE>(G, S, sink, epoch, c, us, ds, v);
 #pragma unroll
-    for (uint32_t k = 0; k < kBatch; ++k)
-      if (k < c)
-        chain_work<MAP, WEIGHTS, MODE>(G, S, sink, epoch, read_lane(uj, js[k]),
-                                       read_lane(dj, js[k]), v[k], slabs, edges);
+    for (uint32_t k = 0; k < kBatch; ++k) {
+      const uint32_t nx = k < c ? read_lane(v[k], kNextLane) : kEmpty;
+      if (nx != kEmpty)
+        chain_work<MAP, WEIGHTS, MODE>(G, S, sink, epoch, us[k], ds[k],
+                                       slab_ptr(G.pool, nx)[lane], slabs, edges);
+    }
   }
 }
 
