@@ -153,6 +153,10 @@ def test_pool_exhaustion_leaves_edge_set_unchanged(gpu):
     after = g.export_csr()
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
     assert g.num_edges() == 1000
+    # the next batch that fits is applied: the failure count belongs to the
+    # failed call only
+    g.InsertEdges(np.full(10, 7, np.uint32), np.arange(100, 110, dtype=np.uint32))
+    assert g.num_edges() == 1010
 
 
 def test_out_of_range_batch_rejected(rmat12):
@@ -166,7 +170,18 @@ def test_out_of_range_batch_rejected(rmat12):
     with pytest.raises(mk.MeerkatError) as ei:
         g.DeleteEdges(bad_d, bad_s)
     assert ei.value.code == _lib.MK_ERR_RANGE
+    assert "1 edges" in str(ei.value)
     assert_same_adjacency(g, oracle.csr(n, s, d))
+    # a later valid batch is applied (the failure belongs to its own call),
+    # and a later failure is detected again
+    ok_s, ok_d = np.array([1, 2], np.uint32), np.array([3, 4], np.uint32)
+    g.InsertEdges(ok_s, ok_d)
+    with pytest.raises(mk.MeerkatError) as ei:
+        g.InsertEdges(np.array([n + 5, 1], np.uint32), np.array([0, n], np.uint32))
+    assert ei.value.code == _lib.MK_ERR_RANGE and "2 edges" in str(ei.value)
+    keys = np.unique(np.r_[gen.keys_of(s, d), gen.keys_of(ok_s, ok_d)])
+    s2, d2 = gen.split_keys(keys)
+    assert_same_adjacency(g, oracle.csr(n, s2, d2))
 
 
 def test_empty_batches_and_handle_order(gpu):

@@ -58,8 +58,17 @@ def test_long_path_static_decremental_incremental(gpu, weighted, persistent):
     check(algo, n, s2, d2, w2)
 
 
+@pytest.fixture(params=[128, 0, 1024], ids=["tail128", "no_tail", "tail1024"])
+def tail(request):
+    from meerkat_amd import autotune
+    before = autotune.get_knob("tail_enter")
+    autotune.set_knob("tail_enter", request.param)
+    yield request.param
+    autotune.set_knob("tail_enter", before)
+
+
 @pytest.mark.parametrize("hints", [False, True])
-def test_broom_spill_and_hub_handover(gpu, hints):
+def test_broom_spill_and_hub_handover(gpu, hints, tail):
     # 0 -> 1 -> {2 .. 3001}, each leaf -> one of 50 tail vertices: the
     # frontier {1} (tail mode) pushes 3000 leaves, more than the LDS queue
     # holds (spill); with degree hints vertex 1 is a hub (hand-over)

@@ -57,3 +57,33 @@ def test_pmc_region_means(tmp_path):
                         "Counter_Value": 0})
     got = profiling.pmc_per_kernel(str(p), "FETCH_SIZE", region=0)
     assert got == {"k_a": (2, 20.0)}
+
+
+@pytest.mark.gpu
+def test_step_trace_of_the_frontier_loop(gpu):
+    import meerkat_amd as mk
+    from meerkat_amd import autotune, gen
+
+    before = autotune.get_knob("persistent_steps")
+    autotune.set_knob("persistent_steps", 1)
+    try:
+        n, s, d, w = gen.rmat(12, 8, seed=5, weighted=True)
+        pool = mk.SlabPool(1 << 26)
+        g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
+        g.InsertEdges(s, d, w)
+        sp = mk.SSSP(g, 0)
+        profiling.reset()
+        profiling.enable(True)
+        sp.Static()
+        profiling.enable(False)
+        st = profiling.steps()
+        assert st and all(x["kernel"] == "k_expand_relax" for x in st)
+        assert [x["step"] for x in st] == list(range(len(st)))
+        assert st[0]["frontier"] + st[0]["hub_items"] >= 1  # the source
+        assert all(x["us"] > 0 for x in st)
+        profiling.reset()
+        assert profiling.steps() == []
+        sp.close()
+    finally:
+        profiling.enable(False)
+        autotune.set_knob("persistent_steps", before)
