@@ -11,7 +11,6 @@
 #include <cmath>
 #include <cstring>
 #include <map>
-#include <chrono>
 #include <mutex>
 #include <string>
 
@@ -28,19 +27,6 @@ int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
-hipError_t host_wait(hipStream_t s, hipEvent_t e) {
-  hipError_t r = hipEventRecord(e, s);
-  if (r != hipSuccess) return r;
-  const auto t0 = std::chrono::steady_clock::now();
-  for (uint32_t i = 0;; ++i) {
-    r = hipEventQuery(e);
-    if (r != hipErrorNotReady) return r;
-    if ((i & 63u) == 63u && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2))
-      return hipEventSynchronize(e);
-    __builtin_ia32_pause();
-  }
-}
-
 int hip_fail(hipError_t e, const char *what) {
   g_err = std::string(what) + ": " + hipGetErrorString(e);
   return MK_ERR_HIP;
@@ -609,7 +595,6 @@ static int graph_alloc(mk_pool *pool, uint32_t n, int weighted, double load_fact
   // it still orders after the caller's NULL-stream work): launching on the
   // NULL stream itself makes the runtime order it against every stream
   e = hipStreamCreate(&g->own_stream);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->wait_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     (void)hipFree(g->heads);
     (void)hipFree(g->bucket_off);
@@ -676,7 +661,6 @@ int mk_graph_destroy(mk_graph *g) {
   (void)hipFree(g->vdeg);
   (void)hipFree(g->counters);
   if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
-  if (g->wait_ev) (void)hipEventDestroy(g->wait_ev);
   g->status.release();
   g->host_scratch.release();
   g->pool->graphs--;
@@ -794,7 +778,7 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
-  MK_HIP(host_wait(s, g->wait_ev));
+  MK_HIP(hipStreamSynchronize(s));
   if (int r = range_error(g, src, dst, n, s, "mk_graph_insert")) return r;
   const uint32_t pool_fail = pool_failures(g);  // both sides count here
   if (pool_fail) {
@@ -823,7 +807,7 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);
   MK_HIP(hipGetLastError());
   MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 4, hipMemcpyDeviceToHost, s));
-  MK_HIP(host_wait(s, g->wait_ev));
+  MK_HIP(hipStreamSynchronize(s));
   if (int r = range_error(g, src, dst, n, s, "mk_graph_delete")) return r;
   g->last_op = 2;
   g->last_src = src;

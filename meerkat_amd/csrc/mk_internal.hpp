@@ -171,10 +171,6 @@ namespace mk {
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
-// Host wait for the work queued on `s` so far: records `e` on s and polls it
-// (a completed signal is seen at once, without the operating system's
-// wake-up of a blocking wait); after ~2 ms of polling it blocks instead.
-hipError_t host_wait(hipStream_t s, hipEvent_t e);
 int device_cus(int device);
 int64_t tune_get(const char *name, int64_t dflt);
 // Grid-stride kernels: the number of waves to launch on `device`.
@@ -227,7 +223,6 @@ struct mk_graph {
   uint32_t *counters = nullptr;  // [0] range-check epoch [2..3] pool failures (u64)
   uint32_t mut_epoch = 0;        // epoch of the last mutation call
   hipStream_t own_stream = nullptr;  // mutations given no stream run here
-  hipEvent_t wait_ev = nullptr;      // host_wait of the mutation calls
   std::vector<uint32_t> bucket_off_host;
   mk::DevBuf status;             // uint8 per edge of the last batch
   uint32_t status_n = 0;

@@ -1203,7 +1203,6 @@ struct Traversal {
   bool weights = false;  // SSSP
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t wait_ev = nullptr;  // host_wait
   uint64_t *dist = nullptr;
   uint32_t *stamp = nullptr;
   uint32_t *invbits = nullptr;
@@ -1254,7 +1253,6 @@ struct Traversal {
     MK_HIP(hipStreamCreate(&stream));
     MK_HIP(hipEventCreate(&ev0));
     MK_HIP(hipEventCreate(&ev1));
-    MK_HIP(hipEventCreateWithFlags(&wait_ev, hipEventDisableTiming));
     MK_HIP(hipMalloc(&dist, n * 8));
     MK_HIP(hipMalloc(&stamp, n * 4));
     MK_HIP(hipMalloc(&invbits, ((n + 31) / 32) * 4));
@@ -1312,7 +1310,6 @@ struct Traversal {
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (wait_ev) (void)hipEventDestroy(wait_ev);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1384,7 +1381,7 @@ struct Traversal {
 
   int read_counts(Totals &h) {
     MK_HIP(hipMemcpyAsync(host.ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
-    MK_HIP(host_wait(stream, wait_ev));
+    MK_HIP(hipStreamSynchronize(stream));
     parse(*host.as<Counts>(), h);
     return MK_OK;
   }
