@@ -54,6 +54,7 @@ def test_c1_rmat16_bfs_inserts(gpu):
     g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s))
     g.InsertEdges(s, d)
     b = mk.BFS(g, 0)
+    b.Static()  # the first call also pays one-time kernel loading
     static_ms = b.Static()
     keys = gen.keys_of(s, d)
     assert np.array_equal(b.Distances(), oracle.bfs(oracle.csr(n, s, d), 0))
@@ -82,6 +83,7 @@ def test_c2_rmat20_sssp_mixed(gpu):
     g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
     g.InsertEdges(s, d, w)
     sp = mk.SSSP(g, 0)
+    sp.Static()  # the first call also pays one-time kernel loading
     static_ms = sp.Static()
     keys, wt = gen.keys_of(s, d), w.copy()
     upd, decr, incr = [], [], []
