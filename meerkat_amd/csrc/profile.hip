@@ -113,6 +113,14 @@ void prof_add_bytes(const char *name, double algo_bytes) {
   g_totals[name].bytes += algo_bytes;
 }
 
+void prof_time(const char *name, double ms) {
+  if (!profiling_on()) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  Total &t = g_totals[name];
+  t.launches++;
+  t.ms += ms;
+}
+
 void prof_step(const char *name, uint32_t step, uint32_t small, uint32_t hub_items, double us) {
   if (!profiling_on()) return;
   std::lock_guard<std::mutex> lk(g_mu);

@@ -22,6 +22,8 @@ hipEvent_t prof_begin(hipStream_t s);
 void prof_end(hipEvent_t start, hipStream_t s, const char *name, double algo_bytes);
 // Add algorithmic bytes known only after a launch completed.
 void prof_add_bytes(const char *name, double algo_bytes);
+// Add one launch of `name` whose duration was measured another way (ms).
+void prof_time(const char *name, double ms);
 // Record one frontier step of a persistent loop: its frontier (small
 // entries, hub bucket items) and duration.
 void prof_step(const char *name, uint32_t step, uint32_t small, uint32_t hub_items, double us);

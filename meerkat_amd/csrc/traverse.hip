@@ -75,6 +75,9 @@ struct Counts {
   uint32_t trace_small[2][kTraceSteps];
   uint32_t trace_items[2][kTraceSteps];
   uint32_t trace_n[2];
+  // device clock at the start and the end of the last k_expand_loop<MODE>
+  // launch (block 0) while profiling: the launch's duration
+  unsigned long long loop_t[2][2];
 };
 
 // Host-side totals of a Counts snapshot.
@@ -719,6 +722,7 @@ __global__ void __launch_bounds__(kBlock)
   // before its first barrier arrival, so block 0's update at the end
   // cannot race with it
   const uint32_t bdone = coherent_load(&S.cnt->bar_done);
+  if (S.trace && blockIdx.x == 0 && threadIdx.x == 0) S.cnt->loop_t[MODE][0] = wall_clock64();
   uint32_t j = j0;
   for (; j < j_end; ++j) {
     const uint32_t slot = j % 3;
@@ -738,8 +742,10 @@ __global__ void __launch_bounds__(kBlock)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (S.trace) {
       const uint32_t k = min(j - j0, kTraceSteps);
-      S.cnt->trace_t[MODE][k] = wall_clock64();
+      const unsigned long long now = wall_clock64();
+      S.cnt->trace_t[MODE][k] = now;
       S.cnt->trace_n[MODE] = k;
+      S.cnt->loop_t[MODE][1] = now;
     }
     S.cnt->loop_end[MODE] = j;
     S.cnt->bar_done = bdone + (j - j0);
@@ -1362,7 +1368,9 @@ struct Traversal {
     TravView S = view();
     uint64_t to = barrier_timeout;
     void *args[] = {&G, &S, &j0, &j1, &ep, &to, &zero, &nzero, &clear_exit};
-    ProfScope prof(stream, MODE == 0 ? "k_expand_relax" : "k_expand_invalidate", 0.0);
+    // timed (while profiling) by the kernel's own clock readings: a
+    // cooperative launch takes no dispatch events, and events around it
+    // would include the runtime's extra dispatch work
     MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
                                       dim3(kBlock), args, 0u, stream));
     return MK_OK;
@@ -1447,11 +1455,20 @@ struct Traversal {
   }
 
   // Step trace of the loops of this call (the host copy of the counters).
+  // Duration of the last k_expand_loop<mode> launch (host copy of the
+  // counters), added to the profiler's totals.
+  void record_loop_time(int mode) {
+    const Counts &c = *host.as<Counts>();
+    const double ms = double(c.loop_t[mode][1] - c.loop_t[mode][0]) / double(clock_khz);
+    prof_time(mode == 0 ? "k_expand_relax" : "k_expand_invalidate", ms);
+  }
+
   void record_steps() {
     const Counts &c = *host.as<Counts>();
     const double us_per_tick = 1000.0 / double(clock_khz);
     for (int mode = 0; mode < 2; ++mode) {
       if (mode == 0 ? !relax_end : !inval_queued) continue;
+      record_loop_time(mode);
       const uint32_t k = std::min(c.trace_n[mode], kTraceSteps);
       for (uint32_t i = 0; i < k; ++i)
         prof_step(mode == 0 ? "k_expand_relax" : "k_expand_invalidate", i, c.trace_small[mode][i],
@@ -1470,6 +1487,7 @@ struct Traversal {
     while (relax_end && h.loop_end[0] == relax_end) {
       if (uint64_t(relax_end) > uint64_t(g->n) + 8)
         return fail(MK_ERR_STATE, "frontier did not drain within n steps");
+      if (profiling_on()) record_loop_time(0);
       const uint32_t j0 = relax_end;
       relax_end += kLoopSteps;
       if (int r = launch_loop<0>(j0, relax_end, reserve_epochs(kLoopSteps))) return r;
