@@ -114,6 +114,22 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_de
  * A deleted key becomes a tombstone; later inserts reuse tombstoned slots. */
 int mk_graph_delete(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_dev, uint32_t n,
                     void *stream);
+/* Deferred mutation checks (no reference counterpart; the reference's
+ * mutations are asynchronous and unchecked).  With on != 0, mk_graph_insert
+ * and mk_graph_delete return once the batch is queued on the device; the
+ * batch's range check and slab-pool outcome are settled by the graph's next
+ * call (any mk_graph_* call that reads or changes it, mk_graph_sync, or an
+ * algorithm compute call on it), which waits for the batch and, if it
+ * failed, restores the graph as the synchronous call would have and returns
+ * that batch's error (MK_ERR_RANGE / MK_ERR_POOL_FULL) instead of doing its
+ * own work.  A BFS/SSSP compute call queued after the batch orders its
+ * kernels after it without a host wait and settles it at its end; if the
+ * batch failed, the call returns its error and the handle's results stay
+ * invalid until its next Static.  The batch buffers and the stream must stay
+ * valid until the batch is settled. */
+int mk_graph_set_deferred(mk_graph *g, int on);
+int mk_graph_sync(mk_graph *g);
+
 /* Batched DynamicGraphContext::SearchEdge (include/Graph/warp/SearchEdge.cuh:
  * 5-63, 118-142): found_dev[i] = 1 if (src[i], dst[i]) is present; for a
  * weighted graph w_out_dev[i] (nullable) receives its weight. */

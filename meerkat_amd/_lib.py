@@ -65,6 +65,8 @@ _PROTOS = {
     "mk_graph_destroy": [vp],
     "mk_graph_insert": [vp, vp, vp, vp, u32, vp],
     "mk_graph_delete": [vp, vp, vp, u32, vp],
+    "mk_graph_set_deferred": [vp, i32],
+    "mk_graph_sync": [vp],
     "mk_graph_search": [vp, vp, vp, u32, vp, vp, vp],
     "mk_graph_last_status": [vp, P(vp), P(u32)],
     "mk_graph_num_vertices": [vp, P(u32)],
@@ -121,6 +123,8 @@ _PROTOS = {
     "mk_profile_reset": [],
     "mk_profile_count": [P(i32)],
     "mk_profile_get": [i32, P(ctypes.c_char_p), P(u64), P(f64), P(f64)],
+    "mk_profile_step_count": [P(i32)],
+    "mk_profile_step_get": [i32, P(ctypes.c_char_p), P(u32), P(u32), P(u32), P(f64)],
     "mk_gen_rmat": [u32, u32, f64, f64, f64, u64, i32, P(u64)],
     "mk_gen_er": [u32, u64, u64, P(u64)],
     "mk_gen_fetch": [vp, vp, vp, u64],

@@ -251,12 +251,14 @@ int mk_cc_destroy(mk_cc *c) {
 
 int mk_cc_static(mk_cc *c, float *ms) {
   if (!c) return fail(MK_ERR_BAD_ARG, "mk_cc_static: NULL handle");
+  if (int r = graph_settle(c->c.g)) return r;
   return c->c.run_static(ms);
 }
 
 int mk_cc_incremental(mk_cc *c, const uint32_t *src, const uint32_t *dst, uint32_t n,
                       float *ms) {
   if (!c) return fail(MK_ERR_BAD_ARG, "mk_cc_incremental: NULL handle");
+  if (int r = graph_settle(c->c.g)) return r;
   return c->c.run_incremental(src, dst, n, ms);
 }
 
@@ -266,6 +268,7 @@ int mk_cc_decremental(mk_cc *c, const uint32_t *src, const uint32_t *dst, uint32
   (void)dst;
   (void)n;
   if (!c) return fail(MK_ERR_BAD_ARG, "mk_cc_decremental: NULL handle");
+  if (int r = graph_settle(c->c.g)) return r;
   return c->c.run_static(ms);
 }
 

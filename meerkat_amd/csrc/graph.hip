@@ -595,6 +595,7 @@ static int graph_alloc(mk_pool *pool, uint32_t n, int weighted, double load_fact
   // it still orders after the caller's NULL-stream work): launching on the
   // NULL stream itself makes the runtime order it against every stream
   e = hipStreamCreate(&g->own_stream);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->mut_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     (void)hipFree(g->heads);
     (void)hipFree(g->bucket_off);
@@ -647,6 +648,7 @@ int mk_graph_destroy(mk_graph *g) {
   if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_destroy: NULL graph");
   if (g->users) return fail(MK_ERR_STATE, "mk_graph_destroy: algorithm handles still attached");
   if (g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_destroy: an in-edge mirror is owned by its graph");
+  (void)graph_settle(g);
   (void)hipSetDevice(g->device);
   if (g->rev) {
     g->rev->mirror_of = nullptr;
@@ -661,6 +663,7 @@ int mk_graph_destroy(mk_graph *g) {
   (void)hipFree(g->vdeg);
   (void)hipFree(g->counters);
   if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
+  if (g->mut_ev) (void)hipEventDestroy(g->mut_ev);
   g->status.release();
   g->host_scratch.release();
   g->pool->graphs--;
@@ -768,55 +771,105 @@ static int mirror_prologue(mk_graph *g, uint32_t n) {
   return MK_OK;
 }
 
-int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
-                    uint32_t n, void *stream) {
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : (g ? g->own_stream : nullptr);
-  if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_insert: mirror is read-only");
-  if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
-  if (n == 0) return MK_OK;
-  if (int r = mirror_prologue(g, n)) return r;
-  launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
-  MK_HIP(hipGetLastError());
-  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
-  MK_HIP(hipStreamSynchronize(s));
-  if (int r = range_error(g, src, dst, n, s, "mk_graph_insert")) return r;
-  const uint32_t pool_fail = pool_failures(g);  // both sides count here
+// The outcome of a mutation whose counters are in host_scratch (its stream
+// done): MK_ERR_RANGE (the kernels changed nothing), or MK_ERR_POOL_FULL
+// after rolling the insert back on stream s, or MK_OK.
+static int mutation_outcome(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
+                            uint32_t n, hipStream_t s) {
+  const char *who = op == 1 ? "mk_graph_insert" : "mk_graph_delete";
+  if (int r = range_error(g, src, dst, n, s, who)) return r;
+  const uint32_t pool_fail = op == 1 ? pool_failures(g) : 0u;  // both sides count here
   if (pool_fail) {
     // roll back: delete what this batch inserted, restoring the edge set
     // (the range check passed, so the kernels run)
     launch_delete(g, src, dst, n, s, MK_EDGE_APPLIED, "k_insert_rollback", g->counters);
     MK_HIP(hipStreamSynchronize(s));
-    return fail(MK_ERR_POOL_FULL, "mk_graph_insert: slab pool exhausted (" +
+    return fail(MK_ERR_POOL_FULL, std::string(who) + ": slab pool exhausted (" +
                                       std::to_string(pool_fail) +
                                       " edges); batch rolled back, edge set unchanged");
   }
-  g->last_op = 1;
+  return MK_OK;
+}
+
+}  // extern "C"
+
+namespace mk {
+// Settle the graph's deferred mutation, if one is queued: wait for it and
+// report its outcome once (restoring the graph on a pool failure).
+int graph_settle(mk_graph *g) {
+  if (g && g->mirror_of) g = g->mirror_of;  // a mirror changes with its graph
+  if (!g || !g->pend.active) return MK_OK;
+  g->pend.active = false;
+  MK_HIP(hipEventSynchronize(g->mut_ev));
+  const int r = mutation_outcome(g, g->pend.op, g->pend.src, g->pend.dst, g->pend.n, g->pend.s);
+  if (r) g->last_op = 0;  // the batch's statuses no longer describe the graph
+  return r;
+}
+}  // namespace mk
+
+extern "C" {
+
+// Shared back half of insert / delete: the outcome now, or (deferred mode)
+// queued for the next call that settles the graph.
+static int mutation_finish(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
+                           const uint32_t *w, uint32_t n, hipStream_t s) {
+  MK_HIP(hipGetLastError());
+  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
+  g->last_op = op;
   g->last_src = src;
   g->last_dst = dst;
-  g->last_w = w;
-  return MK_OK;
+  g->last_w = op == 1 ? w : nullptr;
+  if (g->deferred) {
+    MK_HIP(hipEventRecord(g->mut_ev, s));
+    g->pend = {true, op, src, dst, n, s};
+    return MK_OK;
+  }
+  MK_HIP(hipStreamSynchronize(s));
+  const int r = mutation_outcome(g, op, src, dst, n, s);
+  if (r) g->last_op = 0;
+  return r;
+}
+
+int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                    uint32_t n, void *stream) {
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : (g ? g->own_stream : nullptr);
+  if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_insert: mirror is read-only");
+  if (int r = graph_settle(g)) return r;
+  if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
+  if (n == 0) return MK_OK;
+  if (int r = mirror_prologue(g, n)) return r;
+  launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
+  return mutation_finish(g, 1, src, dst, w, n, s);
 }
 
 int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
                     void *stream) {
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : (g ? g->own_stream : nullptr);
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_delete: mirror is read-only");
+  if (int r = graph_settle(g)) return r;
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n)) return r;
   launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);
-  MK_HIP(hipGetLastError());
-  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 4, hipMemcpyDeviceToHost, s));
-  MK_HIP(hipStreamSynchronize(s));
-  if (int r = range_error(g, src, dst, n, s, "mk_graph_delete")) return r;
-  g->last_op = 2;
-  g->last_src = src;
-  g->last_dst = dst;
+  return mutation_finish(g, 2, src, dst, nullptr, n, s);
+}
+
+int mk_graph_set_deferred(mk_graph *g, int on) {
+  if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_set_deferred: NULL graph");
+  if (g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_set_deferred: mirror follows its graph");
+  if (int r = graph_settle(g)) return r;
+  g->deferred = on != 0;
   return MK_OK;
+}
+
+int mk_graph_sync(mk_graph *g) {
+  if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_sync: NULL graph");
+  return graph_settle(g);
 }
 
 int mk_graph_search(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
                     uint8_t *found, uint32_t *wout, void *stream) {
+  if (int r = graph_settle(g)) return r;
   if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_search: NULL graph");
   if (n && (!src || !dst || !found)) return fail(MK_ERR_BAD_ARG, "mk_graph_search: NULL buffer");
   if (n == 0) return MK_OK;
@@ -836,6 +889,7 @@ int mk_graph_search(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
 }
 
 int mk_graph_last_status(mk_graph *g, const uint8_t **status, uint32_t *n) {
+  if (int r = graph_settle(g)) return r;
   if (!g || !status || !n) return fail(MK_ERR_BAD_ARG, "mk_graph_last_status: NULL argument");
   *status = g->status.as<uint8_t>();
   *n = g->status_n;
@@ -855,6 +909,7 @@ int mk_graph_is_weighted(mk_graph *g, int *w) {
 }
 
 int mk_graph_degrees(mk_graph *g, uint32_t *deg) {
+  if (int r = graph_settle(g)) return r;
   if (!g || !deg) return fail(MK_ERR_BAD_ARG, "mk_graph_degrees: NULL argument");
   if (g->mirror_of) return mk_graph_count_slots(g, deg, nullptr);  // counted from the slabs
   MK_HIP(hipSetDevice(g->device));
@@ -863,6 +918,7 @@ int mk_graph_degrees(mk_graph *g, uint32_t *deg) {
 }
 
 int mk_graph_num_edges(mk_graph *g, uint64_t *m) {
+  if (int r = graph_settle(g)) return r;
   if (!g || !m) return fail(MK_ERR_BAD_ARG, "mk_graph_num_edges: NULL argument");
   std::vector<uint32_t> deg(g->n);
   if (int r = mk_graph_degrees(g, deg.data())) return r;
@@ -873,6 +929,7 @@ int mk_graph_num_edges(mk_graph *g, uint64_t *m) {
 }
 
 int mk_graph_bucket_offsets(mk_graph *g, uint32_t *off) {
+  if (int r = graph_settle(g)) return r;
   if (!g || !off) return fail(MK_ERR_BAD_ARG, "mk_graph_bucket_offsets: NULL argument");
   std::memcpy(off, g->bucket_off_host.data(), (size_t(g->n) + 1) * 4);
   return MK_OK;
@@ -893,6 +950,7 @@ static int count_slots(mk_graph *g, uint32_t *live_dev, uint32_t *tomb_dev,
 }
 
 int mk_graph_count_slots(mk_graph *g, uint32_t *live, uint32_t *tomb) {
+  if (int r = graph_settle(g)) return r;
   if (!g || (!live && !tomb)) return fail(MK_ERR_BAD_ARG, "mk_graph_count_slots: NULL argument");
   MK_HIP(hipSetDevice(g->device));
   DevBuf b;
@@ -911,6 +969,7 @@ int mk_graph_count_slots(mk_graph *g, uint32_t *live, uint32_t *tomb) {
 }
 
 int mk_graph_slab_stats(mk_graph *g, uint64_t *heads, uint64_t *overflow, uint32_t *max_chain) {
+  if (int r = graph_settle(g)) return r;
   if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_slab_stats: NULL graph");
   MK_HIP(hipSetDevice(g->device));
   DevBuf b;
@@ -932,6 +991,7 @@ int mk_graph_slab_stats(mk_graph *g, uint64_t *heads, uint64_t *overflow, uint32
 }
 
 int mk_graph_export_csr(mk_graph *g, uint64_t *row_off, uint32_t *col, uint32_t *wcol) {
+  if (int r = graph_settle(g)) return r;
   if (!g || !row_off) return fail(MK_ERR_BAD_ARG, "mk_graph_export_csr: NULL argument");
   MK_HIP(hipSetDevice(g->device));
   std::vector<uint32_t> live(g->n);

@@ -171,6 +171,9 @@ namespace mk {
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
+// Settle a graph's deferred mutation (graph.hip): wait for it and return its
+// outcome once; MK_OK when none is queued.
+int graph_settle(mk_graph *g);
 int device_cus(int device);
 int64_t tune_get(const char *name, int64_t dflt);
 // Grid-stride kernels: the number of waves to launch on `device`.
@@ -223,6 +226,17 @@ struct mk_graph {
   uint32_t *counters = nullptr;  // [0] range-check epoch [2..3] pool failures (u64)
   uint32_t mut_epoch = 0;        // epoch of the last mutation call
   hipStream_t own_stream = nullptr;  // mutations given no stream run here
+  // deferred mode (mk_graph_set_deferred): a mutation call returns once
+  // queued; `pend` describes it until the next call settles it
+  bool deferred = false;
+  hipEvent_t mut_ev = nullptr;       // recorded after the queued mutation
+  struct Pending {
+    bool active;
+    int op;  // 1 insert, 2 delete
+    const uint32_t *src, *dst;
+    uint32_t n;
+    hipStream_t s;
+  } pend = {false, 0, nullptr, nullptr, 0, nullptr};
   std::vector<uint32_t> bucket_off_host;
   mk::DevBuf status;             // uint8 per edge of the last batch
   uint32_t status_n = 0;

@@ -769,6 +769,7 @@ int mk_msq_run(mk_msq *q, const uint32_t *sources_host, uint32_t k, float *ms) {
   if (k == 0 || k > 64) return fail(MK_ERR_BAD_ARG, "mk_msq_run: 1..64 sources");
   for (uint32_t i = 0; i < k; ++i)
     if (sources_host[i] >= q->m.g->n) return fail(MK_ERR_RANGE, "mk_msq_run: source >= n");
+  if (int r = graph_settle(q->m.g)) return r;
   return q->m.run(sources_host, k, ms);
 }
 

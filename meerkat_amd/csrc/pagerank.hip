@@ -557,18 +557,21 @@ int mk_pagerank_destroy(mk_pagerank *p) {
 
 int mk_pagerank_static(mk_pagerank *p, float *ms) {
   if (!p) return fail(MK_ERR_BAD_ARG, "mk_pagerank_static: NULL handle");
+  if (int r = graph_settle(p->p.g)) return r;
   return p->p.compute(ms);
 }
 
 int mk_pagerank_incremental(mk_pagerank *p, const uint32_t *src, const uint32_t *dst, uint32_t n,
                             float *ms) {
   if (!p) return fail(MK_ERR_BAD_ARG, "mk_pagerank_incremental: NULL handle");
+  if (int r = graph_settle(p->p.g)) return r;
   return p->p.batch(src, dst, n, +1, ms);
 }
 
 int mk_pagerank_decremental(mk_pagerank *p, const uint32_t *src, const uint32_t *dst, uint32_t n,
                             float *ms) {
   if (!p) return fail(MK_ERR_BAD_ARG, "mk_pagerank_decremental: NULL handle");
+  if (int r = graph_settle(p->p.g)) return r;
   return p->p.batch(src, dst, n, -1, ms);
 }
 

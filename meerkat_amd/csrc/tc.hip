@@ -694,18 +694,21 @@ int mk_tc_destroy(mk_tc *t) {
 
 int mk_tc_static(mk_tc *t, float *ms) {
   if (!t) return fail(MK_ERR_BAD_ARG, "mk_tc_static: NULL handle");
+  if (int r = graph_settle(t->t.g)) return r;
   return t->t.run_static(ms);
 }
 
 int mk_tc_incremental(mk_tc *t, const uint32_t *src, const uint32_t *dst, uint32_t n,
                       float *ms) {
   if (!t) return fail(MK_ERR_BAD_ARG, "mk_tc_incremental: NULL handle");
+  if (int r = graph_settle(t->t.g)) return r;
   return t->t.run_incremental(src, dst, n, ms);
 }
 
 int mk_tc_decremental(mk_tc *t, const uint32_t *src, const uint32_t *dst, uint32_t n,
                       float *ms) {
   if (!t) return fail(MK_ERR_BAD_ARG, "mk_tc_decremental: NULL handle");
+  if (int r = graph_settle(t->t.g)) return r;
   return t->t.run_decremental(src, dst, n, ms);
 }
 

@@ -1437,6 +1437,7 @@ struct Traversal {
   uint32_t relax_end = 0;     // range end of the queued relax loop (0: none)
   bool inval_queued = false;  // an invalidation loop was queued
   bool dirty = true;          // counters may be non-zero (cleared at the next begin)
+  bool stale = false;         // results computed on a batch that failed (until Static)
   int reseed_kind = 0;        // queued re-seed: 0 none, 1 pull (mirror), 2 bucket scan
 
   uint32_t inval_limit() const {
@@ -1579,6 +1580,9 @@ struct Traversal {
   // them `dirty` for the next begin() to clear.
   int begin() {
     MK_HIP(hipSetDevice(g->device));
+    // a deferred mutation of the graph is ordered before this call's
+    // kernels on the device; its outcome is settled in end()
+    if (g->pend.active) MK_HIP(hipStreamWaitEvent(stream, g->mut_ev, 0));
     relax_end = 0;
     inval_queued = false;
     reseed_kind = 0;
@@ -1593,8 +1597,14 @@ struct Traversal {
     MK_HIP(hipEventRecord(ev1, stream));
     Totals h;
     if (int r = read_counts(h)) return r;
-    const int lr = finish_loops(h);
+    int lr = finish_loops(h);
     if (!lr && profiling_on()) record_steps();
+    if (!lr && g->pend.active) {
+      // the mutation this call ran on failed: the graph is restored, and
+      // these results (computed on the failed batch) are not valid
+      lr = graph_settle(g);
+      if (lr) stale = true;
+    }
     relax_end = 0;
     inval_queued = false;
     reseed_kind = 0;
@@ -1612,6 +1622,7 @@ struct Traversal {
 
   int run_static(float *ms) {
     if (int r = begin()) return r;
+    stale = false;
     fill_u64(dist, kInfInf, g->n, stream);
     k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
     if (persistent) {
@@ -1626,6 +1637,8 @@ struct Traversal {
   int run_incremental(const uint32_t *src, const uint32_t *dst, const uint32_t *w, uint32_t n,
                       float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "incremental: NULL batch");
+    if (stale) return fail(MK_ERR_STATE, "incremental: results are invalid after a failed batch; "
+                                         "run Static() first");
     if (int r = begin()) return r;
     if (n) {
       const uint32_t ep = next_epoch();
@@ -1688,6 +1701,8 @@ struct Traversal {
 
   int run_decremental(const uint32_t *src, const uint32_t *dst, uint32_t n, float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "decremental: NULL batch");
+    if (stale) return fail(MK_ERR_STATE, "decremental: results are invalid after a failed batch; "
+                                         "run Static() first");
     if (int r = begin()) return r;
     if (n) {
       const GraphView G = g->view();

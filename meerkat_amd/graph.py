@@ -127,12 +127,25 @@ class DynamicGraph:
         s, d = device_u32(src, self.device), device_u32(dst, self.device)
         w = device_u32(weights, self.device)
         _lib.call("mk_graph_insert", self._h, dptr(s), dptr(d), dptr(w), len(s), stream)
+        self._batch = (s, d, w)  # kept alive until the next batch settles this one
         return s, d, w
 
     def DeleteEdges(self, src, dst, stream=None):
         s, d = device_u32(src, self.device), device_u32(dst, self.device)
         _lib.call("mk_graph_delete", self._h, dptr(s), dptr(d), len(s), stream)
+        self._batch = (s, d)
         return s, d
+
+    def set_deferred(self, on=True):
+        """Deferred mutation checks: InsertEdges / DeleteEdges return once the
+        batch is queued; its range and slab-pool outcome is raised by the
+        graph's next call (or sync()), or by the BFS/SSSP call that follows
+        it, which then leaves its handle invalid until Static()."""
+        _lib.call("mk_graph_set_deferred", self._h, int(on))
+
+    def sync(self):
+        """Settle a deferred batch: raises its MeerkatError if it failed."""
+        _lib.call("mk_graph_sync", self._h)
 
     insert_edges = InsertEdges
     delete_edges = DeleteEdges

@@ -1,0 +1,92 @@
+"""Deferred mutation checks (mk_graph_set_deferred): a mutation call returns
+once queued; the next call on the graph, or the BFS/SSSP call queued after
+it, settles its outcome.  Results bit-exact against the oracle; failures
+surface once, with the graph restored as the synchronous call leaves it."""
+import numpy as np
+import pytest
+
+import meerkat_amd as mk
+import oracle
+from meerkat_amd import _lib, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def check(algo, n, s, d, w=None, source=0):
+    ref = oracle.csr(n, s, d, w)
+    dist, par = algo.read()
+    exp = oracle.dijkstra(ref, source) if w is not None else oracle.bfs(ref, source)
+    assert np.array_equal(dist, exp), int((dist != exp).sum())
+    assert oracle.check_parents_fast(ref, dist, par, source, weights=w is not None) == 0
+
+
+@pytest.mark.parametrize("mirror", [False, True])
+def test_deferred_batches_keep_sssp_exact(gpu, mirror):
+    n, s, d, w = gen.rmat(14, 8, seed=61, weighted=True)
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=mirror,
+                        in_degree_hints=gen.degree_hints(n, d) if mirror else None)
+    g.InsertEdges(s, d, w)
+    g.set_deferred(True)
+    sp = mk.SSSP(g, 0)
+    sp.Static()
+    keys, wt = gen.keys_of(s, d), w.copy()
+    for i in range(5):
+        ds, dd = gen.delete_batch(keys, 2000, seed=700 + i)
+        g.DeleteEdges(ds, dd)
+        sp.Decremental(ds, dd, None)
+        keep = ~np.isin(keys, gen.keys_of(ds, dd))
+        keys, wt = keys[keep], wt[keep]
+        bs, bd, bw = gen.insert_batch(keys, n, 2000, seed=800 + i, weighted=True, wseed=900 + i)
+        g.InsertEdges(bs, bd, bw)
+        sp.Incremental(bs, bd, bw)
+        keys, wt = np.r_[keys, gen.keys_of(bs, bd)], np.r_[wt, bw]
+        order = np.argsort(keys)
+        keys, wt = keys[order], wt[order]
+        s2, d2 = gen.split_keys(keys)
+        check(sp, n, s2, d2, wt)
+    assert g.num_edges() == len(keys)
+
+
+def test_deferred_range_error_surfaces_in_the_traversal(gpu):
+    n, s, d, _ = gen.rmat(12, 8, seed=62)
+    pool = mk.SlabPool(1 << 26)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s))
+    g.InsertEdges(s, d)
+    g.set_deferred(True)
+    b = mk.BFS(g, 0)
+    b.Static()
+    before = g.export_csr()
+    bad_s, bad_d = np.array([0, 1, n + 3], np.uint32), np.array([5, 6, 7], np.uint32)
+    g.InsertEdges(bad_s, bad_d)  # queued: returns at once
+    with pytest.raises(mk.MeerkatError) as ei:
+        b.Incremental(bad_s, bad_d)
+    assert ei.value.code == _lib.MK_ERR_RANGE
+    with pytest.raises(mk.MeerkatError) as ei:  # results invalid until Static
+        b.Incremental(np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    assert ei.value.code == _lib.MK_ERR_STATE
+    after = g.export_csr()  # nothing was applied
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    b.Static()
+    check(b, n, s, d)
+
+
+def test_deferred_pool_failure_surfaces_once_and_rolls_back(gpu):
+    n = 200000
+    pool = mk.SlabPool(2048 * 256)
+    g = mk.DynamicGraph(n, pool, 0.7, None, reverse=True)
+    g.InsertEdges(np.zeros(500, np.uint32), np.arange(1, 501, dtype=np.uint32))
+    g.set_deferred(True)
+    before_f, before_r = g.export_csr(), g.mirror().export_csr()
+    g.InsertEdges(np.zeros(150000, np.uint32), np.arange(1, 150001, dtype=np.uint32))
+    # the next mutation reports the queued batch's failure and does not run
+    with pytest.raises(mk.MeerkatError) as ei:
+        g.InsertEdges(np.array([9], np.uint32), np.array([10], np.uint32))
+    assert ei.value.code == _lib.MK_ERR_POOL_FULL
+    g.sync()  # settled: nothing pending any more
+    after_f, after_r = g.export_csr(), g.mirror().export_csr()
+    assert np.array_equal(before_f[1], after_f[1]) and np.array_equal(before_r[1], after_r[1])
+    assert g.num_edges() == 500
+    g.InsertEdges(np.array([9], np.uint32), np.array([10], np.uint32))
+    g.sync()
+    assert g.num_edges() == 501
