@@ -157,6 +157,12 @@ def main():
                     help="no in-edge mirror: decremental re-seeds by scanning every edge")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop phase")
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="leave the library's kernel timing off in the timed region "
+                         "(no roofline)")
+    ap.add_argument("--breakdown-steps", type=int, default=5,
+                    help="steps after the timed region with every kernel timed (the "
+                         "per-kernel table; dispatch events cost host time per launch)")
     ap.add_argument("--sync-mutations", action="store_true",
                     help="each insert/delete waits for its own outcome instead of the "
                          "following SSSP call settling it (deferred checks)")
@@ -182,7 +188,8 @@ def main():
 
     # update stream: deletions sampled without replacement from the initial
     # edges (each still present when deleted), insertions new edges
-    steps_total = args.warmup + args.steps
+    steps_total = args.warmup + args.steps + args.breakdown_steps
+    timed_end = args.warmup + args.steps
     half = args.batch // 2
     rng = np.random.default_rng(args.seed + 1)
     del_idx = rng.choice(m, size=steps_total * half, replace=False)
@@ -246,19 +253,28 @@ def main():
     torch.cuda.synchronize()
     rec = {k: [] for k in ("upd_s", "decr_ms", "incr_ms", "decr_edges", "incr_edges", "step_s")}
     profiling.reset()
-    profiling.enable(True)
+    # in the timed region only the persistent frontier loops are timed (by
+    # their own device clock); timing every kernel costs host time per launch
+    profiling.enable(not args.no_kernel_timing, loops_only=True)
     torch.cuda.synchronize()
     # marker pair 0 delimits the timed region in a rocprofv3 trace
     profiling.marker(dev)
     ts = time.perf_counter()
-    for i in range(args.warmup, steps_total):
+    for i in range(args.warmup, timed_end):
         step(i, rec)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - ts
     profiling.marker(dev)
-    profiling.enable(False)
-    g.set_deferred(False)
     prof = profiling.totals()
+    # per-kernel table: a few more steps with every kernel timed
+    profiling.reset()
+    profiling.enable(True)
+    for i in range(timed_end, steps_total):
+        step(i, None)
+    torch.cuda.synchronize()
+    profiling.enable(False)
+    prof_all = profiling.totals()
+    g.set_deferred(False)
     calibration_copy(dev)
 
     # incremental BFS rate on the updated graph (not part of the step)
@@ -283,18 +299,22 @@ def main():
     ms_per_step = elapsed / K * 1e3
     value = args.batch * K / elapsed
     upd = np.array(rec["upd_s"])
-    mut_s = sum(prof[k]["ms"] for k in ("k_insert", "k_delete") if k in prof) * 1e-3 or float("nan")
+    nb = max(args.breakdown_steps, 1)
+    mut_s = (sum(prof_all[k]["ms"] for k in ("k_insert", "k_delete") if k in prof_all) * 1e-3 / nb
+             * K) or float("nan")  # mutation kernel time for K steps
     step_ms = np.array(rec["step_s"]) * 1e3
     incr_gteps = (np.sum(rec["incr_edges"]) / (np.sum(rec["incr_ms"]) * 1e-3) / 1e9
                   if np.sum(rec["incr_ms"]) > 0 else 0.0)
     decr_gteps = (np.sum(rec["decr_edges"]) / (np.sum(rec["decr_ms"]) * 1e-3) / 1e9
                   if np.sum(rec["decr_ms"]) > 0 else 0.0)
 
-    # dominant kernel by time over the timed region
-    top = max(prof.items(), key=lambda kv: kv[1]["ms"]) if prof else (None, None)
+    # dominant kernel by time (per-kernel table); its figures from the timed
+    # region when it was timed there (the frontier loops are)
+    top = max(prof_all.items(), key=lambda kv: kv[1]["ms"]) if prof_all else (None, None)
     roofline = None
     if top[0]:
-        k, t = top
+        k = top[0]
+        t = prof[k] if k in prof else top[1]
         per_launch_bytes = t["algo_bytes"] / max(t["launches"], 1)
         achieved = t["achieved_GBps"]
         roofline = {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1),
@@ -356,8 +376,10 @@ def main():
         "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),
         "sssp_static_ms": round(static_ms, 3),
         "build_edges_per_s": round(m / build_s, 1),
-        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
-                        "GBps": round(v["achieved_GBps"], 1)} for k, v in prof.items()},
+        # per kernel over the breakdown steps (every kernel timed)
+        "kernels": {"steps": args.breakdown_steps,
+                    **{k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                           "GBps": round(v["achieved_GBps"], 1)} for k, v in prof_all.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving_out,

@@ -306,7 +306,9 @@ int mk_tune_get(const char *name, int64_t *value);
  * cudaEvent, BFSImpl.cuh:644-661).  When enabled, every instrumented kernel
  * launch is bracketed by HIP events on its own stream; totals (launches,
  * milliseconds, algorithmic bytes) accumulate per kernel name until reset.
- * mk_profile_count snapshots the names; mk_profile_get reads entry i. */
+ * mk_profile_count snapshots the names; mk_profile_get reads entry i.
+ * on: 0 off, 1 every instrumented kernel, 2 only the persistent BFS/SSSP
+ * loops (timed by their own device clock readings, nearly free). */
 int mk_profile_enable(int on);
 int mk_profile_reset(void);
 int mk_profile_count(int *count);

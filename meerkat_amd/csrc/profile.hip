@@ -24,7 +24,7 @@ struct Total {
   double bytes = 0.0;
 };
 
-std::atomic<bool> g_on{false};
+std::atomic<int> g_on{0};  // 0 off, 1 every kernel, 2 persistent loops only
 std::mutex g_mu;
 std::vector<Pending> g_pending;
 std::vector<hipEvent_t> g_free;
@@ -66,10 +66,11 @@ void resolve_locked() {
 
 }  // namespace
 
-bool profiling_on() { return g_on.load(std::memory_order_relaxed); }
+bool profiling_on() { return g_on.load(std::memory_order_relaxed) != 0; }
+bool profiling_all() { return g_on.load(std::memory_order_relaxed) == 1; }
 
 hipEvent_t prof_begin(hipStream_t s) {
-  if (!profiling_on()) return nullptr;
+  if (!profiling_all()) return nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
   hipEvent_t e = take_event();
   if (e && hipEventRecord(e, s) != hipSuccess) {
@@ -134,7 +135,8 @@ using namespace mk;
 extern "C" {
 
 int mk_profile_enable(int on) {
-  g_on.store(on != 0);
+  if (on < 0 || on > 2) return fail(MK_ERR_BAD_ARG, "mk_profile_enable: 0, 1 or 2");
+  g_on.store(on);
   return MK_OK;
 }
 

@@ -16,7 +16,8 @@
 
 namespace mk {
 
-bool profiling_on();
+bool profiling_on();   // any level
+bool profiling_all();  // every kernel timed (dispatch events)
 // Returns the start event to pass to prof_end, or nullptr when disabled.
 hipEvent_t prof_begin(hipStream_t s);
 void prof_end(hipEvent_t start, hipStream_t s, const char *name, double algo_bytes);
@@ -39,7 +40,7 @@ template <typename... KArgs, typename... Args>
 void prof_launch(const char *name, double algo_bytes, void (*kernel)(KArgs...), dim3 grid,
                  dim3 block, uint32_t shmem, hipStream_t s, Args... args) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (profiling_on() && prof_events(a, b)) {
+  if (profiling_all() && prof_events(a, b)) {
     hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0u, KArgs(args)...);
     prof_commit(a, b, name, algo_bytes);
   } else {

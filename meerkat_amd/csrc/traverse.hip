@@ -1536,7 +1536,7 @@ struct Traversal {
     uint64_t launched = 0;
     const uint64_t limit = uint64_t(g->n) + 8;
     Totals prev;
-    if (profiling_on()) {
+    if (profiling_all()) {
       if (int r = read_counts(prev)) return r;
     }
     uint32_t slot_after[2] = {0, 0};
@@ -1561,7 +1561,7 @@ struct Traversal {
       if (int r = group(k, b ^ 1)) return r;  // queued before waiting
       MK_HIP(hipEventSynchronize(snap_ev[b]));
       parse(*snap[b].as<Counts>(), h);
-      if (profiling_on()) {
+      if (profiling_all()) {
         const double bytes = 256.0 * double(h.slabs - prev.slabs) +
                              8.0 * double(h.edges - prev.edges) +
                              24.0 * double(h.visited - prev.visited) +
@@ -1763,7 +1763,7 @@ struct Traversal {
           }
         }
         MK_HIP(hipGetLastError());
-        if (profiling_on() && g->rev) {
+        if (profiling_all() && g->rev) {
           // 4 B per bitmap word, 8 B per invalidated vertex (in-bucket
           // range), 256 B per in-slab, 4 + 8 B per in-neighbour (invalid bit,
           // distance), 16 B per re-seeded vertex
@@ -1774,7 +1774,7 @@ struct Traversal {
                                               256.0 * double(a.slabs - slabs0) +
                                               12.0 * double(a.edges - edges0) +
                                               16.0 * double(a.pushed - pushed0));
-        } else if (profiling_on()) {
+        } else if (profiling_all()) {
           // 20 B per bucket (owner, degree, distance, invalid bit of the
           // owner), 256 B per slab read, 4 B per key (invalid bit), 16 B per
           // re-seeded vertex

@@ -4,8 +4,11 @@ import ctypes
 from . import _lib
 
 
-def enable(on=True):
-    _lib.call("mk_profile_enable", int(on))
+def enable(on=True, loops_only=False):
+    """Kernel timing on/off; loops_only times only the persistent BFS/SSSP
+    frontier loops (by their own device clock, nearly free), not every
+    kernel (a pair of dispatch events each)."""
+    _lib.call("mk_profile_enable", (2 if loops_only else 1) if on else 0)
 
 
 def reset():

@@ -110,3 +110,17 @@ def test_tune_knobs_validate_without_gpu():
     finally:
         for k, v in before.items():
             autotune.set_knob(k, v)
+
+
+def test_profile_levels_validate_without_gpu():
+    from meerkat_amd import profiling
+
+    for on in (0, 1, 2, 0):
+        _lib.call("mk_profile_enable", on)
+    with pytest.raises(_lib.MeerkatError) as e:
+        _lib.call("mk_profile_enable", 3)
+    assert e.value.code == _lib.MK_ERR_BAD_ARG
+    profiling.enable(True, loops_only=True)
+    profiling.enable(False)
+    profiling.reset()
+    assert profiling.totals() == {} and profiling.steps() == []
