@@ -239,6 +239,8 @@ __global__ void __launch_bounds__(kBlock)
     if (!act) st = MK_EDGE_SELF_LOOP;
   }
   const GraphView &G = M.G;
+  // every lane's bucket at once, off the groups' sequential chains
+  const uint32_t bk = act ? bucket_of(G, s, d) : 0u;
   uint32_t pending = group_bits(ballot(act));
   uint32_t pool_fail = 0;
   const uint32_t seed_base = (second ? 0x9E3779B9u : 0u) + wave * 64u + group_first_lane();
@@ -246,7 +248,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     const uint32_t sj = group_read(s, j), dj = group_read(d, j), wj = group_read(w, j);
-    const int r = slab_insert_group<MAP>(G, P, sj, dj, wj, seed_base + j);
+    const int r = slab_insert_group<MAP>(G, P, group_read(bk, j), dj, wj, seed_base + j);
     if (gl == j) {
       if (r == 1) st = MK_EDGE_APPLIED;
       else if (r < 0) st = MK_EDGE_POOL_FULL;  // rolled back by the host
@@ -305,12 +307,13 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   const GraphView &G = M.G;
+  const uint32_t bk = act ? bucket_of(G, s, d) : 0u;
   uint32_t pending = group_bits(ballot(act));
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     const uint32_t sj = group_read(s, j), dj = group_read(d, j);
-    const int r = slab_delete_group<MAP>(G, sj, dj);
+    const int r = slab_delete_group<MAP>(G, group_read(bk, j), dj);
     if (gl == j && r == 1) st = MK_EDGE_APPLIED;
     if (r == 1 && gl == 0 && M.degrees) atomicSub(G.vdeg + sj, 1u);
   }
@@ -340,12 +343,13 @@ __global__ void __launch_bounds__(kBlock)
   uint8_t f = 0;
   uint32_t wv = 0;
   const uint32_t gl = group_lane();
+  const uint32_t bk = act ? bucket_of(G, s, d) : 0u;
   uint32_t pending = group_bits(ballot(act));
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     uint32_t wj = 0;
-    const int r = slab_search_group<MAP>(G, group_read(s, j), group_read(d, j), &wj);
+    const int r = slab_search_group<MAP>(G, group_read(bk, j), group_read(d, j), &wj);
     if (gl == j) {
       f = uint8_t(r);
       wv = wj;

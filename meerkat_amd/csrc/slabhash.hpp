@@ -67,10 +67,14 @@ MK_DEV void pool_free_lane(const PoolView &P, uint32_t idx) {
 }
 
 // ---------------------------------------------------------------- lookup
-MK_DEV uint32_t *head_of(const GraphView &G, uint32_t src, uint32_t key) {
+// Global bucket index of `key` in src's adjacency.
+MK_DEV uint32_t bucket_of(const GraphView &G, uint32_t src, uint32_t key) {
   const uint32_t off = G.bucket_off[src];
   const uint32_t nb = G.bucket_off[src + 1] - off;
-  return G.heads + size_t(off + bucket_in(key, nb)) * kSlabWords;
+  return off + bucket_in(key, nb);
+}
+MK_DEV uint32_t *head_of(const GraphView &G, uint32_t src, uint32_t key) {
+  return G.heads + size_t(bucket_of(G, src, key)) * kSlabWords;
 }
 
 template <bool MAP> MK_DEV uint64_t key_mask() { return MAP ? kMapKeyMask : kSetKeyMask; }
@@ -283,13 +287,15 @@ MK_DEV uint32_t pool_alloc_group(const PoolView &P, uint32_t seed) {
   return kEmpty;
 }
 
-// slab_insert for one group (arguments group-uniform).  Returns 1 if
+// slab_insert for one group (arguments group-uniform; `bucket` is
+// bucket_of(G, src, dst), computed by the caller for all its edges at once
+// so that this load leaves each operation's dependent chain).  Returns 1 if
 // inserted, 0 if already present, -1 if the pool is exhausted.
 template <bool MAP>
-MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t src, uint32_t dst,
+MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t bucket, uint32_t dst,
                              uint32_t w, uint32_t seed) {
   const uint32_t gl = group_lane();
-  uint32_t *slab = head_of(G, src, dst);
+  uint32_t *slab = G.heads + size_t(bucket) * kSlabWords;
   uint32_t *free_slab = nullptr;
   uint32_t free_word = 0, free_old = 0;
   for (;;) {
@@ -356,9 +362,9 @@ MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t src
 // slab_delete for one group: the key becomes a tombstone.  Returns 1 if
 // deleted.
 template <bool MAP>
-MK_DEV int slab_delete_group(const GraphView &G, uint32_t src, uint32_t dst) {
+MK_DEV int slab_delete_group(const GraphView &G, uint32_t bucket, uint32_t dst) {
   const uint32_t gl = group_lane();
-  uint32_t *slab = head_of(G, src, dst);
+  uint32_t *slab = G.heads + size_t(bucket) * kSlabWords;
   for (;;) {
     uint32_t v[kWordsPerLane];
     load_words_coherent(slab + kWordsPerLane * gl, v);
@@ -384,9 +390,9 @@ MK_DEV int slab_delete_group(const GraphView &G, uint32_t src, uint32_t dst) {
 // slab_search for one group (no concurrent writers).  Returns 1 if present
 // and sets *w (map value, or 0).
 template <bool MAP>
-MK_DEV int slab_search_group(const GraphView &G, uint32_t src, uint32_t dst, uint32_t *w) {
+MK_DEV int slab_search_group(const GraphView &G, uint32_t bucket, uint32_t dst, uint32_t *w) {
   const uint32_t gl = group_lane();
-  const uint32_t *slab = head_of(G, src, dst);
+  const uint32_t *slab = G.heads + size_t(bucket) * kSlabWords;
   for (;;) {
     uint32_t v[kWordsPerLane];
     load_words(slab + kWordsPerLane * gl, v);
