@@ -149,6 +149,7 @@ struct TravView {
   uint32_t *stamp;
   uint32_t *invbits;
   uint32_t *inv_list;  // invalidated vertices, compacted from invbits
+  uint32_t *olddist;   // an invalidated vertex's distance before the call
   uint32_t *q_small[2];
   uint2 *q_large[2];
   Counts *cnt;
@@ -359,6 +360,7 @@ MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, Sink &sink, uin
         if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                       (unsigned long long)kInfInf) == cur) {
           atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
+          S.olddist[v] = dist_of(cur);
           push = true;
         }
       }
@@ -436,7 +438,10 @@ MK_DEV uint32_t batch_slab_work(const GraphView &G, const TravView &S, Sink &sin
 #pragma unroll
   for (uint32_t k = 0; k < kBatch; ++k) {
     if (MODE == 1) {
-      if (push[k]) atomicOr(S.invbits + (v[k] >> 5), 1u << (v[k] & 31u));
+      if (push[k]) {
+        atomicOr(S.invbits + (v[k] >> 5), 1u << (v[k] & 31u));
+        S.olddist[v[k]] = dist_of(cur[k]);
+      }
     } else if (push[k]) {
       push[k] = atomicExch(S.stamp + v[k], epoch) != epoch;
     }
@@ -841,12 +846,20 @@ MK_DEV void pull_chain(const GraphView &G, const GraphView &R, const TravView &S
 }
 
 // Pull the in-buckets named by lanes of `m` (lane j: vertex tj, in-bucket
-// bk), kBatch slab loads in flight.
+// bk), kBatch slab loads in flight.  With `skip_done`, a vertex already
+// back at its distance before the call is skipped: deletions never shorten
+// a path, so a valid in-neighbour giving that distance settles it (a hub's
+// remaining in-buckets then need no reading).
 template <bool MAP, bool WEIGHTS>
 MK_DEV void pull_buckets(const GraphView &G, const GraphView &R, const TravView &S, WaveQ &q,
                          uint32_t epoch, uint64_t m, uint32_t bk, uint32_t tj, uint32_t &slabs,
-                         uint32_t &edges) {
+                         uint32_t &edges, bool skip_done = false) {
   const uint32_t lane = lane_id();
+  if (skip_done) {
+    const bool mine = (m >> lane) & 1ull;
+    const bool done = mine && dist_of(coherent_load64(S.dist + tj)) == S.olddist[tj];
+    m &= ~ballot(done);
+  }
   while (m) {
     uint32_t js[kBatch];
     uint32_t c = 0;
@@ -902,7 +915,7 @@ __global__ void __launch_bounds__(kBlock)
     visited += uint32_t(__popcll(ballot(act)));
     for (uint32_t k = 0; ballot(act && rnb > k); ++k)
       pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act && rnb > k), rb0 + k, v, slabs,
-                                 edges);
+                                 edges, k > 0);
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
@@ -991,7 +1004,7 @@ __global__ void __launch_bounds__(kBlock)
       v = ent.x;
       b = R.bucket_off[v] + (t - ent.y);
     }
-    pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges);
+    pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges, true);
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
@@ -1079,6 +1092,7 @@ __global__ void __launch_bounds__(kItemBlock)
         } else if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                              (unsigned long long)kInfInf) == cur) {
           atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
+          S.olddist[v] = dist_of(cur);
           push = true;
         }
       }
@@ -1166,9 +1180,10 @@ __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, Tra
         if (gl == 0) {
           const uint64_t old = atomicCAS((unsigned long long *)(S.dist + v),
                                          (unsigned long long)cur, (unsigned long long)repl);
-          if (old == cur && repl == kInfInf)
+          if (old == cur && repl == kInfInf) {
+            S.olddist[v] = dv;
             push = true;
-          else
+          } else
             atomicAnd(S.invbits + (v >> 5), ~(1u << (v & 31u)));
         }
       }
@@ -1213,6 +1228,7 @@ struct Traversal {
   uint32_t *stamp = nullptr;
   uint32_t *invbits = nullptr;
   uint32_t *inv_list = nullptr;
+  uint32_t *olddist = nullptr;
   uint32_t *qs[2] = {nullptr, nullptr};
   uint2 *ql[2] = {nullptr, nullptr};
   Counts *cnt = nullptr;  // device; BarLines follow at bar_offset
@@ -1238,6 +1254,7 @@ struct Traversal {
     v.stamp = stamp;
     v.invbits = invbits;
     v.inv_list = inv_list;
+    v.olddist = olddist;
     v.q_small[0] = qs[0];
     v.q_small[1] = qs[1];
     v.q_large[0] = ql[0];
@@ -1263,6 +1280,7 @@ struct Traversal {
     MK_HIP(hipMalloc(&stamp, n * 4));
     MK_HIP(hipMalloc(&invbits, ((n + 31) / 32) * 4));
     MK_HIP(hipMalloc(&inv_list, n * 4));
+    MK_HIP(hipMalloc(&olddist, n * 4));
     for (int i = 0; i < 2; ++i) {
       MK_HIP(hipMalloc(&qs[i], n * 4));
       MK_HIP(hipMalloc(&ql[i], n * 8));
@@ -1304,6 +1322,7 @@ struct Traversal {
     (void)hipFree(stamp);
     (void)hipFree(invbits);
     (void)hipFree(inv_list);
+    (void)hipFree(olddist);
     for (int i = 0; i < 2; ++i) {
       (void)hipFree(qs[i]);
       (void)hipFree(ql[i]);
