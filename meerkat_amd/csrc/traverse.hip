@@ -149,7 +149,6 @@ struct TravView {
   uint32_t *stamp;
   uint32_t *invbits;
   uint32_t *inv_list;  // invalidated vertices, compacted from invbits
-  uint32_t *olddist;   // an invalidated vertex's distance before the call
   uint32_t *q_small[2];
   uint2 *q_large[2];
   Counts *cnt;
@@ -360,7 +359,6 @@ MK_DEV uint32_t slab_work(const GraphView &G, const TravView &S, Sink &sink, uin
         if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                       (unsigned long long)kInfInf) == cur) {
           atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
-          S.olddist[v] = dist_of(cur);
           push = true;
         }
       }
@@ -438,10 +436,7 @@ MK_DEV uint32_t batch_slab_work(const GraphView &G, const TravView &S, Sink &sin
 #pragma unroll
   for (uint32_t k = 0; k < kBatch; ++k) {
     if (MODE == 1) {
-      if (push[k]) {
-        atomicOr(S.invbits + (v[k] >> 5), 1u << (v[k] & 31u));
-        S.olddist[v[k]] = dist_of(cur[k]);
-      }
+      if (push[k]) atomicOr(S.invbits + (v[k] >> 5), 1u << (v[k] & 31u));
     } else if (push[k]) {
       push[k] = atomicExch(S.stamp + v[k], epoch) != epoch;
     }
@@ -846,20 +841,12 @@ MK_DEV void pull_chain(const GraphView &G, const GraphView &R, const TravView &S
 }
 
 // Pull the in-buckets named by lanes of `m` (lane j: vertex tj, in-bucket
-// bk), kBatch slab loads in flight.  With `skip_done`, a vertex already
-// back at its distance before the call is skipped: deletions never shorten
-// a path, so a valid in-neighbour giving that distance settles it (a hub's
-// remaining in-buckets then need no reading).
+// bk), kBatch slab loads in flight.
 template <bool MAP, bool WEIGHTS>
 MK_DEV void pull_buckets(const GraphView &G, const GraphView &R, const TravView &S, WaveQ &q,
                          uint32_t epoch, uint64_t m, uint32_t bk, uint32_t tj, uint32_t &slabs,
-                         uint32_t &edges, bool skip_done = false) {
+                         uint32_t &edges) {
   const uint32_t lane = lane_id();
-  if (skip_done) {
-    const bool mine = (m >> lane) & 1ull;
-    const bool done = mine && dist_of(coherent_load64(S.dist + tj)) == S.olddist[tj];
-    m &= ~ballot(done);
-  }
   while (m) {
     uint32_t js[kBatch];
     uint32_t c = 0;
@@ -915,7 +902,7 @@ __global__ void __launch_bounds__(kBlock)
     visited += uint32_t(__popcll(ballot(act)));
     for (uint32_t k = 0; ballot(act && rnb > k); ++k)
       pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act && rnb > k), rb0 + k, v, slabs,
-                                 edges, k > 0);
+                                 edges);
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
@@ -1004,7 +991,7 @@ __global__ void __launch_bounds__(kBlock)
       v = ent.x;
       b = R.bucket_off[v] + (t - ent.y);
     }
-    pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges, true);
+    pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges);
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
@@ -1092,7 +1079,6 @@ __global__ void __launch_bounds__(kItemBlock)
         } else if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
                              (unsigned long long)kInfInf) == cur) {
           atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
-          S.olddist[v] = dist_of(cur);
           push = true;
         }
       }
@@ -1180,10 +1166,9 @@ __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, Tra
         if (gl == 0) {
           const uint64_t old = atomicCAS((unsigned long long *)(S.dist + v),
                                          (unsigned long long)cur, (unsigned long long)repl);
-          if (old == cur && repl == kInfInf) {
-            S.olddist[v] = dv;
+          if (old == cur && repl == kInfInf)
             push = true;
-          } else
+          else
             atomicAnd(S.invbits + (v >> 5), ~(1u << (v & 31u)));
         }
       }
@@ -1228,7 +1213,6 @@ struct Traversal {
   uint32_t *stamp = nullptr;
   uint32_t *invbits = nullptr;
   uint32_t *inv_list = nullptr;
-  uint32_t *olddist = nullptr;
   uint32_t *qs[2] = {nullptr, nullptr};
   uint2 *ql[2] = {nullptr, nullptr};
   Counts *cnt = nullptr;  // device; BarLines follow at bar_offset
@@ -1254,7 +1238,6 @@ struct Traversal {
     v.stamp = stamp;
     v.invbits = invbits;
     v.inv_list = inv_list;
-    v.olddist = olddist;
     v.q_small[0] = qs[0];
     v.q_small[1] = qs[1];
     v.q_large[0] = ql[0];
@@ -1280,7 +1263,6 @@ struct Traversal {
     MK_HIP(hipMalloc(&stamp, n * 4));
     MK_HIP(hipMalloc(&invbits, ((n + 31) / 32) * 4));
     MK_HIP(hipMalloc(&inv_list, n * 4));
-    MK_HIP(hipMalloc(&olddist, n * 4));
     for (int i = 0; i < 2; ++i) {
       MK_HIP(hipMalloc(&qs[i], n * 4));
       MK_HIP(hipMalloc(&ql[i], n * 8));
@@ -1322,7 +1304,6 @@ struct Traversal {
     (void)hipFree(stamp);
     (void)hipFree(invbits);
     (void)hipFree(inv_list);
-    (void)hipFree(olddist);
     for (int i = 0; i < 2; ++i) {
       (void)hipFree(qs[i]);
       (void)hipFree(ql[i]);
