@@ -45,7 +45,7 @@
 namespace mk {
 
 constexpr uint32_t kSmallBuckets = 2;
-constexpr uint32_t kLargeChunk = 8;  // bucket work items per wave step
+constexpr uint32_t kHubChunk = 8;  // buckets per hub-list entry
 constexpr uint32_t kBatch = 4;       // slab loads in flight per wave
 
 constexpr uint32_t kWaveQ = 512;     // LDS staging entries per wave
@@ -157,36 +157,44 @@ struct TravView {
   uint32_t trace;                  // record the step trace (profiling)
 };
 
-// Append the hub vertices v (lanes with `hub`, nb buckets each) to hub
-// queue side `side`, counter slot `slot`: one 64-bit atomic per call claims
-// the entries and the bucket work items of all of them (an exclusive prefix
-// of nb over the lanes gives each its first item).  Wave-collective.
-MK_DEV void push_hubs(uint2 *queue, unsigned long long *count, bool hub, uint32_t v,
-                      uint32_t nb) {
+// Append the hub vertices v (lanes with `hub`, nb buckets of G each) to a
+// hub list: one entry (v, global index of its first bucket) per kHubChunk
+// buckets, so a consumer maps its work straight to entries.  One 64-bit
+// atomic per call claims the entries of all of them and counts their
+// buckets: *count += (entries << 32) | buckets (an exclusive prefix over
+// the lanes places each vertex's entries).  Wave-collective.
+MK_DEV void push_hubs(const GraphView &G, uint2 *queue, unsigned long long *count, bool hub,
+                      uint32_t v, uint32_t nb) {
   const uint64_t hm = ballot(hub);
   if (hm == 0) return;
   const uint32_t lane = lane_id();
-  uint32_t x = hub ? nb : 0u;  // inclusive prefix of nb over the wave
+  const uint32_t nc = hub ? (nb + kHubChunk - 1) / kHubChunk : 0u;
+  uint32_t xc = nc, xb = hub ? nb : 0u;  // inclusive prefixes over the wave
 #pragma unroll
   for (uint32_t off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
+    const uint32_t yc = __shfl_up(xc, off, 64), yb = __shfl_up(xb, off, 64);
+    if (lane >= off) {
+      xc += yc;
+      xb += yb;
+    }
   }
-  const uint32_t total = read_lane(x, 63);
   const uint32_t leader = ffs64(hm);
   unsigned long long r = 0;
   if (lane == leader)
-    r = atomicAdd(count, (uint64_t(__popcll(hm)) << 32) | total);
+    r = atomicAdd(count, (uint64_t(read_lane(xc, 63)) << 32) | read_lane(xb, 63));
   const uint32_t first = read_lane(uint32_t(r >> 32), leader);
-  const uint32_t item0 = read_lane(uint32_t(r), leader);
-  if (hub) qstore2(queue + first + lanes_below(hm), make_uint2(v, item0 + x - nb));
+  if (hub) {
+    const uint32_t b0 = G.bucket_off[v];
+    uint2 *e = queue + first + xc - nc;
+    for (uint32_t c = 0; c < nc; ++c) qstore2(e + c, make_uint2(v, b0 + c * kHubChunk));
+  }
 }
 
 // Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
 // small vertices through the wave's LDS staging, hub vertices directly.
 // Wave-collective.
-MK_DEV void enqueue_nb(const TravView &S, WaveQ &q, uint32_t side, uint32_t slot, bool push,
-                       uint32_t v, uint32_t nb) {
+MK_DEV void enqueue_nb(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
+                       uint32_t slot, bool push, uint32_t v, uint32_t nb) {
   const uint64_t pm = ballot(push);
   if (pm == 0) return;
   const bool small = push && nb <= kSmallBuckets;
@@ -195,7 +203,7 @@ MK_DEV void enqueue_nb(const TravView &S, WaveQ &q, uint32_t side, uint32_t slot
   if (q.n + c > kWaveQ) wq_flush(q, S.q_small[side], &S.cnt->small[slot]);
   if (small) q.buf[q.n + lanes_below(sm)] = v;
   q.n += c;
-  push_hubs(S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
+  push_hubs(G, S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
   q.pushed += uint32_t(__popcll(pm));
 }
 
@@ -207,7 +215,7 @@ MK_DEV uint32_t push_buckets(const GraphView &G, bool push, uint32_t v) {
 MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t side,
                     uint32_t slot, bool push, uint32_t v) {
   if (ballot(push) == 0) return;
-  enqueue_nb(S, q, side, slot, push, v, push_buckets(G, push, v));
+  enqueue_nb(G, S, q, side, slot, push, v, push_buckets(G, push, v));
 }
 
 // wq_flush for every wave of the block at once, one atomic per block;
@@ -245,8 +253,8 @@ struct GlobalSink {
     enqueue(G, S, q, side, slot, push, v);
   }
   // the pushing lanes' bucket counts already loaded
-  MK_DEV void operator()(const GraphView &, bool push, uint32_t v, uint32_t nb) {
-    enqueue_nb(S, q, side, slot, push, v, nb);
+  MK_DEV void operator()(const GraphView &G, bool push, uint32_t v, uint32_t nb) {
+    enqueue_nb(G, S, q, side, slot, push, v, nb);
   }
 };
 
@@ -267,7 +275,7 @@ struct TailSink {
     (*this)(G, push, v, push_buckets(G, push, v));
   }
   // the pushing lanes' bucket counts already loaded
-  MK_DEV void operator()(const GraphView &, bool push, uint32_t v, uint32_t nb) {
+  MK_DEV void operator()(const GraphView &G, bool push, uint32_t v, uint32_t nb) {
     const uint64_t pm = ballot(push);
     if (pm == 0) return;
     const uint32_t lane = lane_id();
@@ -296,7 +304,7 @@ struct TailSink {
       }
     }
     // hubs go to the next step's bucket work list
-    push_hubs(S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
+    push_hubs(G, S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
     if (push && !small) ctl[1] = 1u;
     pushed += uint32_t(__popcll(pm));
   }
@@ -325,7 +333,7 @@ MK_DEV void block_enqueue(const GraphView &G, const TravView &S, uint32_t side, 
   }
   __syncthreads();
   if (small) qstore(S.q_small[side] + sh[NW] + sh[w] + lanes_below(sm), v);
-  push_hubs(S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
+  push_hubs(G, S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
   const uint64_t pm = ballot(push);
   if (lane == 0) add_stat(S.cnt, kSetSeed, kStatPushed, uint32_t(__popcll(pm)));
 }
@@ -597,7 +605,7 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
   const uint32_t nside = side ^ 1u;
   const uint32_t n_small = coherent_load(&S.cnt->small[slot]);
   const unsigned long long lg = coherent_load64(reinterpret_cast<const uint64_t *>(&S.cnt->large[slot]));
-  const uint32_t n_large = uint32_t(lg >> 32), large_items = uint32_t(lg);
+  const uint32_t n_large = uint32_t(lg >> 32);  // hub-list entries
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     coherent_store(&S.cnt->small[cslot], 0u);
     __hip_atomic_store(reinterpret_cast<uint64_t *>(&S.cnt->large[cslot]), uint64_t(0),
@@ -612,28 +620,23 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
   WaveQ q{lds + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
   GlobalSink sink{S, q, nside, nslot};
   uint32_t slabs = 0, edges = 0, visited = 0;
-  // hub vertices: bucket work items spread over all waves
+  // hub vertices: their list entries (kHubChunk buckets each) spread over
+  // all waves; lane l takes bucket l % kHubChunk of entry l / kHubChunk
   const uint2 *ql = S.q_large[side];
-  for (uint32_t t0 = gw * kLargeChunk; t0 < large_items; t0 += W * kLargeChunk) {
-    uint32_t lo = 0, hi = n_large;  // last entry with start <= t0
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (qload2_y(ql + mid) <= t0) lo = mid;
-      else hi = mid;
-    }
-    // lane k < chunk takes work item t0 + k
-    const uint32_t t = t0 + lane;
-    bool act = lane < kLargeChunk && t < large_items;
-    uint32_t e = lo;
-    if (act)
-      while (e + 1 < n_large && qload2_y(ql + e + 1) <= t) ++e;
+  const uint32_t eper = min(per_wave(n_large, W), 64u / kHubChunk);
+  for (uint32_t e0 = gw * eper; e0 < n_large; e0 += W * eper) {
+    const uint32_t el = lane / kHubChunk, ei = e0 + el;
+    bool act = el < eper && ei < n_large;
     uint32_t u = 0, b = 0, du = 0;
     if (act) {
-      const uint2 ent = qload2(ql + e);
+      const uint2 ent = qload2(ql + ei);
       u = ent.x;
-      b = G.bucket_off[u] + (t - ent.y);
-      du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
-      if (MODE == 0 && du == kInf) act = false;
+      b = ent.y + lane % kHubChunk;
+      act = b < G.bucket_off[u + 1];
+      if (act) {
+        du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
+        if (MODE == 0 && du == kInf) act = false;
+      }
     }
     batched_buckets<MAP, WEIGHTS, MODE>(G, S, sink, epoch, ballot(act), b, u, du, slabs, edges);
   }
@@ -897,7 +900,7 @@ __global__ void __launch_bounds__(kBlock)
       rnb = R.bucket_off[v + 1] - rb0;
     }
     const bool hub = act && rnb > kSmallBuckets;
-    push_hubs(S.q_large[1], &S.cnt->pull_large, hub, v, rnb);
+    push_hubs(R, S.q_large[1], &S.cnt->pull_large, hub, v, rnb);
     if (hub) act = false;
     visited += uint32_t(__popcll(ballot(act)));
     for (uint32_t k = 0; ballot(act && rnb > k); ++k)
@@ -968,28 +971,23 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
-  const unsigned long long lg = S.cnt->pull_large;
-  const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
+  const uint32_t n_large = uint32_t(S.cnt->pull_large >> 32);  // list entries
   if (n_large == 0) return;
   __shared__ uint32_t lq[kWaveLds];
   WaveQ q{lq + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
   uint32_t slabs = 0, edges = 0;
   const uint2 *ql = S.q_large[1];
-  for (uint32_t t0 = gw * kLargeChunk; t0 < items; t0 += W * kLargeChunk) {
-    uint32_t lo = 0, hi = n_large;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ql[mid].y <= t0) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t t = t0 + lane;
-    const bool act = lane < kLargeChunk && t < items;
-    uint32_t e = lo, v = 0, b = 0;
+  // lane l takes in-bucket l % kHubChunk of entry l / kHubChunk
+  const uint32_t eper = min(per_wave(n_large, W), 64u / kHubChunk);
+  for (uint32_t e0 = gw * eper; e0 < n_large; e0 += W * eper) {
+    const uint32_t el = lane / kHubChunk, ei = e0 + el;
+    bool act = el < eper && ei < n_large;
+    uint32_t v = 0, b = 0;
     if (act) {
-      while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
-      const uint2 ent = ql[e];
+      const uint2 ent = ql[ei];
       v = ent.x;
-      b = R.bucket_off[v] + (t - ent.y);
+      b = ent.y + lane % kHubChunk;
+      act = b < R.bucket_off[v + 1];
     }
     pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges);
   }
@@ -1184,13 +1182,14 @@ __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, Tra
 __global__ void k_set_source(GraphView G, uint64_t *dist, uint32_t source, uint32_t *q_small,
                              uint2 *q_large, Counts *cnt) {
   dist[source] = pack(0, source);
-  const uint32_t nb = G.bucket_off[source + 1] - G.bucket_off[source];
+  const uint32_t b0 = G.bucket_off[source], nb = G.bucket_off[source + 1] - b0;
   if (nb <= kSmallBuckets) {
     q_small[0] = source;
     cnt->small[0] = 1;
-  } else {
-    q_large[0] = make_uint2(source, 0u);
-    cnt->large[0] = (1ull << 32) | nb;
+  } else {  // hub-list entries, kHubChunk buckets each
+    const uint32_t nc = (nb + kHubChunk - 1) / kHubChunk;
+    for (uint32_t c = 0; c < nc; ++c) q_large[c] = make_uint2(source, b0 + c * kHubChunk);
+    cnt->large[0] = (uint64_t(nc) << 32) | nb;
   }
 }
 
@@ -1265,7 +1264,10 @@ struct Traversal {
     MK_HIP(hipMalloc(&inv_list, n * 4));
     for (int i = 0; i < 2; ++i) {
       MK_HIP(hipMalloc(&qs[i], n * 4));
-      MK_HIP(hipMalloc(&ql[i], n * 8));
+      // one entry per kHubChunk buckets of every listed hub (a vertex is
+      // listed once per step; the pull re-seed lists in-buckets)
+      const size_t buckets = std::max<size_t>(g->total_buckets, g->rev ? g->rev->total_buckets : 0);
+      MK_HIP(hipMalloc(&ql[i], (n + buckets / kHubChunk + 64) * 8));
     }
     MK_HIP(hipMalloc(&cnt, bar_offset() + sizeof(BarLines)));
     bar = reinterpret_cast<BarLines *>(reinterpret_cast<char *>(cnt) + bar_offset());
