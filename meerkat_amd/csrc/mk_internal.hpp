@@ -141,6 +141,46 @@ MK_DEV void wave_push(bool pred, uint32_t value, uint32_t *queue, uint32_t *coun
   if (pred) queue[base + lanes_below(m)] = value;
 }
 
+// Hub lists of the frontier kernels: an entry (vertex, global index of its
+// first bucket) per kHubChunk buckets of a listed vertex.
+constexpr uint32_t kHubChunk = 8;
+
+// Append the hub vertices v (lanes with `hub`, nb buckets of G each) to a
+// hub list: one entry (v, global index of its first bucket) per kHubChunk
+// buckets, so a consumer maps its work straight to entries.  One 64-bit
+// atomic per call claims the entries of all of them and counts their
+// buckets: *count += (entries << 32) | buckets (an exclusive prefix over
+// the lanes places each vertex's entries).  Wave-collective.
+MK_DEV void push_hubs(const GraphView &G, uint2 *queue, unsigned long long *count, bool hub,
+                      uint32_t v, uint32_t nb) {
+  const uint64_t hm = ballot(hub);
+  if (hm == 0) return;
+  const uint32_t lane = lane_id();
+  const uint32_t nc = hub ? (nb + kHubChunk - 1) / kHubChunk : 0u;
+  uint32_t xc = nc, xb = hub ? nb : 0u;  // inclusive prefixes over the wave
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t yc = __shfl_up(xc, off, 64), yb = __shfl_up(xb, off, 64);
+    if (lane >= off) {
+      xc += yc;
+      xb += yb;
+    }
+  }
+  const uint32_t leader = ffs64(hm);
+  unsigned long long r = 0;
+  if (lane == leader)
+    r = atomicAdd(count, (uint64_t(read_lane(xc, 63)) << 32) | read_lane(xb, 63));
+  const uint32_t first = read_lane(uint32_t(r >> 32), leader);
+  if (hub) {
+    const uint32_t b0 = G.bucket_off[v];
+    uint2 *e = queue + first + xc - nc;
+    for (uint32_t c = 0; c < nc; ++c)  // coherent: read by later steps of a persistent loop
+      __hip_atomic_store(reinterpret_cast<uint64_t *>(e + c),
+                         (uint64_t(b0 + c * kHubChunk) << 32) | v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Sum of a uint32 over the wave (all lanes must be active).
 MK_DEV uint32_t wave_sum_u32(uint32_t v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);

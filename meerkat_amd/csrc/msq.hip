@@ -16,7 +16,8 @@
 //    step i (label-correcting, as the single-source SSSP).
 // Frontier bookkeeping follows the single-source traversal: counters in
 // three rotating slots read on the device, hub vertices (more than
-// kMsSmall buckets) split into bucket work items over all waves, and the
+// kMsSmall buckets) listed one entry per kHubChunk buckets and spread over
+// all waves, and the
 // host reading counters only every few steps.
 #include <hip/hip_runtime.h>
 
@@ -30,7 +31,6 @@
 namespace mk {
 
 constexpr uint32_t kMsSmall = 2;
-constexpr uint32_t kMsChunk = 8;
 
 // BFS rotates three slots (its queue is the append-only log); SSSP rotates
 // four, because a step clears the masks of the previous queue (slot i-1)
@@ -41,7 +41,7 @@ struct MsCounts {
   uint32_t cnt[kMsSlots];          // small entries per slot
   uint32_t steps;                  // non-empty steps
   uint32_t pad0;
-  unsigned long long large[kMsSlots];  // (entries << 32) | items per slot
+  unsigned long long large[kMsSlots];  // (hub-list entries << 32) | buckets per slot
   uint32_t overflow;               // log capacity exceeded
   uint32_t pad;
   unsigned long long edges;        // adjacency entries read
@@ -105,10 +105,7 @@ MK_DEV void ms_push(const GraphView &G, const MsView &S, MsQ &q, bool push, uint
   if (q.n + c > kMsWaveQ) ms_flush(S, q, list, base, cnt, cap);
   if (to_list) q.buf[q.n + lanes_below(lm)] = v;
   q.n += c;
-  if (hub) {
-    const unsigned long long r = atomicAdd(large, (1ull << 32) | nb);
-    ql[uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
-  }
+  push_hubs(G, ql, large, hub, v, nb);
 }
 
 // ------------------------------------------------------------------ BFS
@@ -157,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
   const uint32_t count = S.c->cnt[slot];
   const uint32_t nbase = start + count;
   const unsigned long long lg = S.c->large[slot];
-  const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
+  const uint32_t n_large = uint32_t(lg >> 32);  // hub-list entries
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S.seg[L + 1] = nbase;
     S.c->cnt[pslot] = 0;
@@ -177,24 +174,20 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
   __shared__ uint32_t lqbuf[kBlock / 64][kMsWaveQ];
   MsQ q{lqbuf[threadIdx.x >> 6], 0u};
   uint32_t edges = 0;
-  // hubs: bucket work items
+  // hubs: their list entries
   const uint2 *ql = S.ql[slot];
-  for (uint32_t t0 = gw * kMsChunk; t0 < items; t0 += W * kMsChunk) {
-    uint32_t lo = 0, hi = n_large;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ql[mid].y <= t0) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t t = t0 + lane;
-    const bool act = lane < kMsChunk && t < items;
-    uint32_t e = lo, b = 0;
+  // lane l takes bucket l % kHubChunk of hub-list entry l / kHubChunk
+  const uint32_t eper = min(per_wave(n_large, W), 64u / kHubChunk);
+  for (uint32_t e0 = gw * eper; e0 < n_large; e0 += W * eper) {
+    const uint32_t el = lane / kHubChunk, ei = e0 + el;
+    bool act = el < eper && ei < n_large;
+    uint32_t b = 0;
     uint64_t mu = 0;
     if (act) {
-      while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
-      const uint2 ent = ql[e];
-      b = G.bucket_off[ent.x] + (t - ent.y);
-      mu = Fc[ent.x];
+      const uint2 ent = ql[ei];
+      b = ent.y + lane % kHubChunk;
+      act = b < G.bucket_off[ent.x + 1];
+      if (act) mu = Fc[ent.x];
     }
     uint64_t m = ballot(act);
     while (m) {
@@ -275,16 +268,15 @@ __global__ void k_msbfs_init(MsView S, const uint32_t *sources, uint32_t k) {
   S.c->cnt[0] = cntv;
 }
 
-// Hub vertices among the sources go to the hub list of slot 0.
+// Hub vertices among the sources go to the hub list of slot 0 (one wave).
 __global__ void k_ms_source_hubs(GraphView G, MsView S) {
   const uint32_t i = threadIdx.x;
-  if (i >= S.c->cnt[0]) return;
-  const uint32_t v = S.logv[i];
-  const uint32_t nb = G.bucket_off[v + 1] - G.bucket_off[v];
-  if (nb > kMsSmall) {
-    const unsigned long long r = atomicAdd(&S.c->large[0], (1ull << 32) | nb);
-    S.ql[0][uint32_t(r >> 32)] = make_uint2(v, uint32_t(r));
+  uint32_t v = 0, nb = 0;
+  if (i < S.c->cnt[0]) {
+    v = S.logv[i];
+    nb = G.bucket_off[v + 1] - G.bucket_off[v];
   }
+  push_hubs(G, S.ql[0], &S.c->large[0], nb > kMsSmall, v, nb);
 }
 
 // Distances of slot s from the log: the level whose mask holds bit s.
@@ -419,7 +411,7 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
   const uint32_t zslot = (i + 2) % kMsSlots, pslot = (i + 3) % kMsSlots;
   const uint32_t count = S.c->cnt[slot];
   const unsigned long long lg = S.c->large[slot];
-  const uint32_t n_large = uint32_t(lg >> 32), items = uint32_t(lg);
+  const uint32_t n_large = uint32_t(lg >> 32);  // hub-list entries
   uint32_t *qc = S.logv + size_t(slot) * S.n, *qn = S.logv + size_t(nslot) * S.n;
   const uint32_t *qp = S.logv + size_t(pslot) * S.n;
   uint64_t *Fc = S.F[slot], *Fn = S.F[nslot], *Fp = S.F[pslot];
@@ -438,23 +430,19 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
   MsQ q{lqbuf[threadIdx.x >> 6], 0u};
   uint32_t edges = 0;
   const uint2 *ql = S.ql[slot];
-  for (uint32_t t0 = gw * kMsChunk; t0 < items; t0 += W * kMsChunk) {
-    uint32_t lo = 0, hi = n_large;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ql[mid].y <= t0) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t t = t0 + lane;
-    const bool act = lane < kMsChunk && t < items;
-    uint32_t e = lo, b = 0, u = 0;
+  // lane l takes bucket l % kHubChunk of hub-list entry l / kHubChunk
+  const uint32_t eper = min(per_wave(n_large, W), 64u / kHubChunk);
+  for (uint32_t e0 = gw * eper; e0 < n_large; e0 += W * eper) {
+    const uint32_t el = lane / kHubChunk, ei = e0 + el;
+    bool act = el < eper && ei < n_large;
+    uint32_t b = 0, u = 0;
     uint64_t mu = 0;
     if (act) {
-      while (e + 1 < n_large && ql[e + 1].y <= t) ++e;
-      const uint2 ent = ql[e];
+      const uint2 ent = ql[ei];
       u = ent.x;
-      b = G.bucket_off[u] + (t - ent.y);
-      mu = Fc[u];
+      b = ent.y + lane % kHubChunk;
+      act = b < G.bucket_off[u + 1];
+      if (act) mu = Fc[u];
     }
     uint64_t m = ballot(act);
     while (m) {
@@ -586,7 +574,8 @@ struct MultiSource {
     MK_HIP(hipEventCreate(&ev1));
     for (uint32_t i = 0; i < kMsSlots; ++i) {
       MK_HIP(hipMalloc(&F[i], n * 8));
-      MK_HIP(hipMalloc(&ql[i], n * 8));
+      // one entry per kHubChunk buckets of each listed hub
+      MK_HIP(hipMalloc(&ql[i], (n + size_t(g->total_buckets) / kHubChunk + 64) * 8));
       fill_u64(F[i], 0ull, n, stream);
     }
     MK_HIP(hipMalloc(&c, sizeof(MsCounts)));

@@ -45,7 +45,6 @@
 namespace mk {
 
 constexpr uint32_t kSmallBuckets = 2;
-constexpr uint32_t kHubChunk = 8;  // buckets per hub-list entry
 constexpr uint32_t kBatch = 4;       // slab loads in flight per wave
 
 constexpr uint32_t kWaveQ = 512;     // LDS staging entries per wave
@@ -156,39 +155,6 @@ struct TravView {
   uint32_t tail_enter, tail_stay;  // tail-mode thresholds (0: no tail mode)
   uint32_t trace;                  // record the step trace (profiling)
 };
-
-// Append the hub vertices v (lanes with `hub`, nb buckets of G each) to a
-// hub list: one entry (v, global index of its first bucket) per kHubChunk
-// buckets, so a consumer maps its work straight to entries.  One 64-bit
-// atomic per call claims the entries of all of them and counts their
-// buckets: *count += (entries << 32) | buckets (an exclusive prefix over
-// the lanes places each vertex's entries).  Wave-collective.
-MK_DEV void push_hubs(const GraphView &G, uint2 *queue, unsigned long long *count, bool hub,
-                      uint32_t v, uint32_t nb) {
-  const uint64_t hm = ballot(hub);
-  if (hm == 0) return;
-  const uint32_t lane = lane_id();
-  const uint32_t nc = hub ? (nb + kHubChunk - 1) / kHubChunk : 0u;
-  uint32_t xc = nc, xb = hub ? nb : 0u;  // inclusive prefixes over the wave
-#pragma unroll
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    const uint32_t yc = __shfl_up(xc, off, 64), yb = __shfl_up(xb, off, 64);
-    if (lane >= off) {
-      xc += yc;
-      xb += yb;
-    }
-  }
-  const uint32_t leader = ffs64(hm);
-  unsigned long long r = 0;
-  if (lane == leader)
-    r = atomicAdd(count, (uint64_t(read_lane(xc, 63)) << 32) | read_lane(xb, 63));
-  const uint32_t first = read_lane(uint32_t(r >> 32), leader);
-  if (hub) {
-    const uint32_t b0 = G.bucket_off[v];
-    uint2 *e = queue + first + xc - nc;
-    for (uint32_t c = 0; c < nc; ++c) qstore2(e + c, make_uint2(v, b0 + c * kHubChunk));
-  }
-}
 
 // Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
 // small vertices through the wave's LDS staging, hub vertices directly.
