@@ -1221,7 +1221,9 @@ struct Traversal {
     weights = w;
     const size_t n = g->n;
     MK_HIP(hipSetDevice(g->device));
-    MK_HIP(hipStreamCreate(&stream));
+    // the graph's own stream: the graph's mutations given no stream run
+    // there too, so a call that follows one is ordered after it as is
+    stream = g->own_stream;
     MK_HIP(hipEventCreate(&ev0));
     MK_HIP(hipEventCreate(&ev1));
     MK_HIP(hipMalloc(&dist, n * 8));
@@ -1284,7 +1286,7 @@ struct Traversal {
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;  // the graph's
   }
 
   // Epoch base of the next launch: kEpochStride values are reserved per
@@ -1550,7 +1552,7 @@ struct Traversal {
     MK_HIP(hipSetDevice(g->device));
     // a deferred mutation of the graph is ordered before this call's
     // kernels on the device; its outcome is settled in end()
-    if (g->pend.active) MK_HIP(hipStreamWaitEvent(stream, g->mut_ev, 0));
+    if (g->pend.active && g->pend.s != stream) MK_HIP(hipStreamWaitEvent(stream, g->mut_ev, 0));
     relax_end = 0;
     inval_queued = false;
     reseed_kind = 0;
