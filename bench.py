@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--breakdown-steps", type=int, default=5,
                     help="steps after the timed region with every kernel timed (the "
                          "per-kernel table; dispatch events cost host time per launch)")
+    ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a launch knob (mk_tune_set) before any handle is created")
     ap.add_argument("--sync-mutations", action="store_true",
                     help="each insert/delete waits for its own outcome instead of the "
                          "following SSSP call settling it (deferred checks)")
@@ -180,6 +182,11 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     dev = torch.device("cuda:0")
+    from meerkat_amd import autotune
+
+    for kv in args.knob:
+        name, value = kv.split("=")
+        autotune.set_knob(name, int(value))
 
     t0 = time.perf_counter()
     n, s, d, w = gen.rmat(args.scale, args.edge_factor, seed=args.seed, weighted=True)

@@ -54,6 +54,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"orphan_repair", 0},   // decremental with the in-edge mirror re-parents orphans first
     {"persistent_steps", 1},  // BFS/SSSP frontier steps in one launch with grid barriers
     {"tail_enter", 128},    // frontiers up to this size run in one block's LDS (0: never)
+    {"coop_launch", 1},     // frontier loops launched cooperatively (co-residency checked)
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {
@@ -460,7 +461,8 @@ int mk_tune_set(const char *name, int64_t value) {
     return fail(MK_ERR_BAD_ARG, "waves_per_cu must be in [1, 64]");
   if (k == "ms_unroll" && value != 4 && value != 8)
     return fail(MK_ERR_BAD_ARG, "ms_unroll must be 4 or 8");
-  if ((k == "orphan_repair" || k == "persistent_steps") && value != 0 && value != 1)
+  if ((k == "orphan_repair" || k == "persistent_steps" || k == "coop_launch") && value != 0 &&
+      value != 1)
     return fail(MK_ERR_BAD_ARG, k + " must be 0 or 1");
   if (k == "tail_enter" && (value < 0 || value > 1024))
     return fail(MK_ERR_BAD_ARG, "tail_enter must be in [0, 1024]");

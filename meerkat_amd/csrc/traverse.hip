@@ -1193,6 +1193,7 @@ struct Traversal {
   bool repair = false;  // orphan repair before invalidation (with the in-edge mirror)
   bool persistent = true;        // frontier steps in one launch (k_expand_loop)
   uint32_t tail_enter = 128;     // tail-mode entry threshold (knob "tail_enter")
+  bool coop = true;              // loop kernels launched cooperatively (knob "coop_launch")
   unsigned loop_blocks[2] = {0, 0};  // resident-grid size per MODE
   uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
   int clock_khz = 100000;        // device wall-clock rate
@@ -1254,6 +1255,7 @@ struct Traversal {
     repair = tune_get("orphan_repair", 0) != 0;
     persistent = tune_get("persistent_steps", 1) != 0;
     tail_enter = uint32_t(tune_get("tail_enter", 128));
+    coop = tune_get("coop_launch", 1) != 0;
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
         khz <= 0)
@@ -1341,8 +1343,12 @@ struct Traversal {
     // timed (while profiling) by the kernel's own clock readings: a
     // cooperative launch takes no dispatch events, and events around it
     // would include the runtime's extra dispatch work
-    MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
-                                      dim3(kBlock), args, 0u, stream));
+    if (coop)
+      MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
+                                        dim3(kBlock), args, 0u, stream));
+    else  // the grid is sized to be resident; nothing guarantees it is
+      MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]), dim3(kBlock), args,
+                             0u, stream));
     return MK_OK;
   }
 
