@@ -151,6 +151,14 @@ def lib():
                 f"{LIB_PATH} is not built; run __graft_entry__.build() "
                 "or make -C meerkat_amd/csrc"
             )
+        try:  # torch carries its own copy of the HIP runtime: it must find
+            # the device before the library's runtime is loaded (the other
+            # order leaves one of the two without a device)
+            import torch
+
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, args in _PROTOS.items():
             fn = getattr(L, name, None)

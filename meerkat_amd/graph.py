@@ -50,12 +50,6 @@ class SlabPool:
     def __init__(self, nbytes=0, device=0):
         self._h = _vp()
         self.device = device
-        # torch carries its own copy of the HIP runtime: let it open the
-        # device before the library's runtime does (the other order leaves
-        # torch without a device)
-        torch = _torch()
-        if torch.cuda.is_available():
-            torch.cuda.init()
         _lib.call("mk_pool_create", nbytes, device, ctypes.byref(self._h))
 
     @property
