@@ -296,6 +296,10 @@ int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
  * per step, the host polling the frontier counters).  "tail_enter"
  * (0..1024, default 128): frontiers of at most this many vertices (and no
  * hub) are expanded by one block step after step in LDS (0: never).
+ * "coop_launch" (0 or 1, default 0): 1 launches the frontier loops
+ * cooperatively always; 0 does so only while BFS/SSSP handles exist on more
+ * than one graph of a device (their loops could then overlap), and launches
+ * them as ordinary kernels, with a grid sized to stay resident, otherwise.
  * The Python autotuner (meerkat_amd/autotune.py) sets them from
  * measurements. */
 int mk_tune_set(const char *name, int64_t value);

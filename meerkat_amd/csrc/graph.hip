@@ -54,7 +54,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"orphan_repair", 0},   // decremental with the in-edge mirror re-parents orphans first
     {"persistent_steps", 1},  // BFS/SSSP frontier steps in one launch with grid barriers
     {"tail_enter", 128},    // frontiers up to this size run in one block's LDS (0: never)
-    {"coop_launch", 1},     // frontier loops launched cooperatively (co-residency checked)
+    {"coop_launch", 0},     // 1: frontier loops always launched cooperatively
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {

@@ -37,6 +37,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "profile.hpp"
@@ -1168,6 +1170,26 @@ __global__ void k_unpack(const uint64_t *dist, uint32_t n, uint32_t *d, uint32_t
 }
 
 // ---------------------------------------------------------------- host
+// Streams that run frontier loops, per device.  A loop's grid barrier needs
+// every block resident.  While all loops of the process run on one stream
+// they never overlap, and a grid sized to what the device keeps resident
+// is enough: they are launched like any kernel.  Once loops may run on two
+// streams at once (handles on different graphs), two partly resident grids
+// could wait on each other, so loops are launched cooperatively (the
+// runtime checks co-residency).
+static std::mutex g_loop_mu;
+static std::map<int, std::map<hipStream_t, int>> g_loop_streams;  // device -> stream -> handles
+
+static void loop_stream_add(int device, hipStream_t s, int d) {
+  std::lock_guard<std::mutex> lk(g_loop_mu);
+  auto &m = g_loop_streams[device];
+  if ((m[s] += d) <= 0) m.erase(s);
+}
+static bool loop_stream_alone(int device) {
+  std::lock_guard<std::mutex> lk(g_loop_mu);
+  return g_loop_streams[device].size() <= 1;
+}
+
 struct Traversal {
   mk_graph *g = nullptr;
   uint32_t source = 0;
@@ -1193,7 +1215,7 @@ struct Traversal {
   bool repair = false;  // orphan repair before invalidation (with the in-edge mirror)
   bool persistent = true;        // frontier steps in one launch (k_expand_loop)
   uint32_t tail_enter = 128;     // tail-mode entry threshold (knob "tail_enter")
-  bool coop = true;              // loop kernels launched cooperatively (knob "coop_launch")
+  bool coop = false;             // loops always launched cooperatively (knob "coop_launch")
   unsigned loop_blocks[2] = {0, 0};  // resident-grid size per MODE
   uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
   int clock_khz = 100000;        // device wall-clock rate
@@ -1255,7 +1277,8 @@ struct Traversal {
     repair = tune_get("orphan_repair", 0) != 0;
     persistent = tune_get("persistent_steps", 1) != 0;
     tail_enter = uint32_t(tune_get("tail_enter", 128));
-    coop = tune_get("coop_launch", 1) != 0;
+    coop = tune_get("coop_launch", 0) != 0;
+    loop_stream_add(g->device, stream, 1);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
         khz <= 0)
@@ -1288,6 +1311,7 @@ struct Traversal {
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) loop_stream_add(g->device, stream, -1);
     stream = nullptr;  // the graph's
   }
 
@@ -1343,10 +1367,10 @@ struct Traversal {
     // timed (while profiling) by the kernel's own clock readings: a
     // cooperative launch takes no dispatch events, and events around it
     // would include the runtime's extra dispatch work
-    if (coop)
+    if (coop || !loop_stream_alone(g->device))
       MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
                                         dim3(kBlock), args, 0u, stream));
-    else  // the grid is sized to be resident; nothing guarantees it is
+    else  // the only stream running loops; the grid is sized to be resident
       MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]), dim3(kBlock), args,
                              0u, stream));
     return MK_OK;

@@ -169,3 +169,32 @@ def test_deleted_tree_edge_with_equal_alternative(gpu, weighted, mirror, repair)
     check(algo, n, s[keep], d[keep], ww[keep] if weighted else None)
     assert np.array_equal(algo.Distances(), before)
     assert int(algo.Parents()[3]) == 3 - p3  # the other middle vertex
+
+
+def test_loops_on_two_graphs_and_forced_cooperative_launch(gpu):
+    # handles on two graphs run their loops on two streams (cooperative
+    # launches); coop_launch=1 forces them for a single graph too
+    from meerkat_amd import autotune
+
+    graphs = []
+    for seed in (71, 72):
+        n, s, d, w = gen.rmat(13, 8, seed=seed, weighted=True)
+        pool = mk.SlabPool(1 << 26)
+        g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
+        g.InsertEdges(s, d, w)
+        graphs.append((n, s, d, w, pool, g, mk.SSSP(g, 0)))
+    for n, s, d, w, pool, g, sp in graphs:
+        sp.Static()
+        check(sp, n, s, d, w)
+    for n, s, d, w, pool, g, sp in graphs:
+        sp.close()
+    before = autotune.get_knob("coop_launch")
+    autotune.set_knob("coop_launch", 1)
+    try:
+        n, s, d, w, pool, g, _ = graphs[0]
+        b = mk.BFS(g, 0)
+        b.Static()
+        check(b, n, s, d)
+        b.close()
+    finally:
+        autotune.set_knob("coop_launch", before)
