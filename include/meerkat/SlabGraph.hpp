@@ -173,8 +173,8 @@ class GraphBase {
   // Deferred mutation checks (mk_graph_set_deferred): InsertEdges /
   // DeleteEdges return once queued; a failed batch throws from the graph's
   // next call, or from the BFS/SSSP call that follows it.  Sync() settles.
-  void SetDeferred(bool on) { check(mk_graph_set_deferred(g_, on ? 1 : 0)); }
-  void Sync() { check(mk_graph_sync(g_)); }
+  void SetDeferred(bool on) { check(mk_graph_set_deferred(g_, on ? 1 : 0), "mk_graph_set_deferred"); }
+  void Sync() { check(mk_graph_sync(g_), "mk_graph_sync"); }
 
  protected:
   mk_graph *g_ = nullptr;
