@@ -55,6 +55,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"persistent_steps", 1},  // BFS/SSSP frontier steps in one launch with grid barriers
     {"tail_enter", 128},    // frontiers up to this size run in one block's LDS (0: never)
     {"coop_launch", 0},     // 1: frontier loops always launched cooperatively
+    {"mut_waves_per_cu", 32},  // waves per CU the mutation kernels' batch is spread over
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {
@@ -437,9 +438,9 @@ __global__ void __launch_bounds__(kBlock) k_release(GraphView G, PoolView P) {
 }
 
 static uint32_t edges_per_wave(uint32_t n, int device) {
-  // enough waves to fill every CU ~32 deep (4 slab operations in flight per
-  // wave), at most 64 edges (one per lane)
-  const uint64_t target_waves = uint64_t(device_cus(device)) * 32;
+  // enough waves to fill every CU `mut_waves_per_cu` deep (default 32; eight
+  // slab operations in flight per wave), at most 64 edges (one per lane)
+  const uint64_t target_waves = uint64_t(device_cus(device)) * uint64_t(tune_get("mut_waves_per_cu", 32));
   uint64_t epw = (n + target_waves - 1) / target_waves;
   return uint32_t(std::min<uint64_t>(std::max<uint64_t>(epw, 1), 64));
 }
@@ -464,6 +465,8 @@ int mk_tune_set(const char *name, int64_t value) {
   if ((k == "orphan_repair" || k == "persistent_steps" || k == "coop_launch") && value != 0 &&
       value != 1)
     return fail(MK_ERR_BAD_ARG, k + " must be 0 or 1");
+  if (k == "mut_waves_per_cu" && (value < 1 || value > 256))
+    return fail(MK_ERR_BAD_ARG, "mut_waves_per_cu must be in [1, 256]");
   if (k == "tail_enter" && (value < 0 || value > 1024))
     return fail(MK_ERR_BAD_ARG, "tail_enter must be in [0, 1024]");
   std::lock_guard<std::mutex> lk(g_tune_mu);
