@@ -300,6 +300,10 @@ int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
  * cooperatively always; 0 does so only while BFS/SSSP handles exist on more
  * than one graph of a device (their loops could then overlap), and launches
  * them as ordinary kernels, with a grid sized to stay resident, otherwise.
+ * "mut_waves_per_cu" (1..256, default 32, read at each mutation call): the
+ * waves per compute unit a mutation batch is spread over.
+ * "barrier_timeout_ms" (0..600000, default 5000): the longest a frontier
+ * loop's grid barrier waits before the call gives up with MK_ERR_STATE.
  * The Python autotuner (meerkat_amd/autotune.py) sets them from
  * measurements. */
 int mk_tune_set(const char *name, int64_t value);

@@ -19,7 +19,7 @@ from . import _lib
 
 KNOBS = {"waves_per_cu": (1, 64), "ms_unroll": (4, 8), "orphan_repair": (0, 1),
          "persistent_steps": (0, 1), "tail_enter": (0, 1024), "coop_launch": (0, 1),
-         "mut_waves_per_cu": (1, 256)}
+         "mut_waves_per_cu": (1, 256), "barrier_timeout_ms": (0, 600000)}
 
 
 def set_knob(name, value):

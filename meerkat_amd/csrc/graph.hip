@@ -56,6 +56,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"tail_enter", 128},    // frontiers up to this size run in one block's LDS (0: never)
     {"coop_launch", 0},     // 1: frontier loops always launched cooperatively
     {"mut_waves_per_cu", 32},  // waves per CU the mutation kernels' batch is spread over
+    {"barrier_timeout_ms", 5000},  // longest grid-barrier wait of a frontier loop
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {
@@ -465,6 +466,8 @@ int mk_tune_set(const char *name, int64_t value) {
   if ((k == "orphan_repair" || k == "persistent_steps" || k == "coop_launch") && value != 0 &&
       value != 1)
     return fail(MK_ERR_BAD_ARG, k + " must be 0 or 1");
+  if (k == "barrier_timeout_ms" && (value < 0 || value > 600000))
+    return fail(MK_ERR_BAD_ARG, "barrier_timeout_ms must be in [0, 600000]");
   if (k == "mut_waves_per_cu" && (value < 1 || value > 256))
     return fail(MK_ERR_BAD_ARG, "mut_waves_per_cu must be in [1, 256]");
   if (k == "tail_enter" && (value < 0 || value > 1024))

@@ -1284,7 +1284,8 @@ struct Traversal {
         khz <= 0)
       khz = 100000;
     clock_khz = khz;
-    barrier_timeout = uint64_t(khz) * 5000ull;  // five seconds
+    // knob "barrier_timeout_ms" (default five seconds)
+    barrier_timeout = uint64_t(khz) * uint64_t(tune_get("barrier_timeout_ms", 5000));
     if (persistent) {
       if (int r = loop_grid<0>(loop_blocks[0])) return r;
       if (int r = loop_grid<1>(loop_blocks[1])) return r;
@@ -1598,6 +1599,7 @@ struct Traversal {
     Totals h;
     if (int r = read_counts(h)) return r;
     int lr = finish_loops(h);
+    if (lr) stale = true;  // a loop gave up: the results are incomplete
     if (!lr && profiling_on()) record_steps();
     if (!lr && g->pend.active) {
       // the mutation this call ran on failed: the graph is restored, and

@@ -97,7 +97,7 @@ def test_tune_knobs_validate_without_gpu():
     before = autotune.knobs()
     assert before == {"waves_per_cu": 16, "ms_unroll": 8, "orphan_repair": 0,
                       "persistent_steps": 1, "tail_enter": 128,
-                      "coop_launch": 0, "mut_waves_per_cu": 32}
+                      "coop_launch": 0, "mut_waves_per_cu": 32, "barrier_timeout_ms": 5000}
     try:
         autotune.set_knob("waves_per_cu", 32)
         assert autotune.get_knob("waves_per_cu") == 32

@@ -198,3 +198,28 @@ def test_loops_on_two_graphs_and_forced_cooperative_launch(gpu):
         b.close()
     finally:
         autotune.set_knob("coop_launch", before)
+
+
+def test_barrier_wait_limit_ends_the_call_and_static_recovers(gpu):
+    # a zero wait limit (forced from the host) makes the first grid barrier
+    # give up: every wave still exits, the call returns MK_ERR_STATE, and a
+    # handle created with the normal limit recomputes exactly afterwards
+    from meerkat_amd import _lib, autotune
+
+    n, s, d, w = gen.rmat(14, 8, seed=81, weighted=True)
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
+    g.InsertEdges(s, d, w)
+    before = autotune.get_knob("barrier_timeout_ms")
+    autotune.set_knob("barrier_timeout_ms", 0)
+    try:
+        sp = mk.SSSP(g, 0)
+    finally:
+        autotune.set_knob("barrier_timeout_ms", before)
+    with pytest.raises(mk.MeerkatError) as ei:
+        sp.Static()
+    assert ei.value.code == _lib.MK_ERR_STATE
+    sp.close()
+    sp = mk.SSSP(g, 0)
+    sp.Static()
+    check(sp, n, s, d, w)
