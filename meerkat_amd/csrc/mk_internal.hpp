@@ -171,12 +171,21 @@ MK_DEV void push_hubs(const GraphView &G, uint2 *queue, unsigned long long *coun
   if (lane == leader)
     r = atomicAdd(count, (uint64_t(read_lane(xc, 63)) << 32) | read_lane(xb, 63));
   const uint32_t first = read_lane(uint32_t(r >> 32), leader);
-  if (hub) {
-    const uint32_t b0 = G.bucket_off[v];
-    uint2 *e = queue + first + xc - nc;
-    for (uint32_t c = 0; c < nc; ++c)  // coherent: read by later steps of a persistent loop
-      __hip_atomic_store(reinterpret_cast<uint64_t *>(e + c),
-                         (uint64_t(b0 + c * kHubChunk) << 32) | v, __ATOMIC_RELAXED,
+  const uint32_t total = read_lane(xc, 63);
+  const uint32_t b0 = hub ? G.bucket_off[v] : 0u;
+  // the wave writes the entries together, 64 at a time: entry e belongs to
+  // the first lane whose inclusive prefix exceeds e (found by halving)
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t e = base + lane;
+    uint32_t lo = 0;  // lanes below `lo` have prefixes <= e
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1)
+      if (__shfl(xc, int(lo + step - 1), 64) <= e) lo += step;
+    const uint32_t ov = __shfl(v, int(lo), 64), ob0 = __shfl(b0, int(lo), 64);
+    const uint32_t oc = e - (__shfl(xc, int(lo), 64) - __shfl(nc, int(lo), 64));
+    if (e < total)  // coherent: read by later steps of a persistent loop
+      __hip_atomic_store(reinterpret_cast<uint64_t *>(queue + first + e),
+                         (uint64_t(ob0 + oc * kHubChunk) << 32) | ov, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
   }
 }
