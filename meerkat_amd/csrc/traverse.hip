@@ -811,8 +811,72 @@ MK_DEV void pull_chain(const GraphView &G, const GraphView &R, const TravView &S
   }
 }
 
+// pull_chain over the head slabs of c <= kBatch in-buckets at once (slab k
+// of invalidated vertex tv[k], this lane's word x[k]), phase by phase so
+// that their dependent memory operations overlap: every in-neighbour's
+// invalid bit and distance are loaded together, the c wave minima are
+// taken, then lane k applies slab k's minimum to tv[k] (all c atomics in
+// flight at once) and the improved vertices are pushed.  Chained slabs
+// follow one by one.  Wave-collective.
+template <bool MAP, bool WEIGHTS>
+MK_DEV void batch_pull_work(const GraphView &G, const GraphView &R, const TravView &S, WaveQ &q,
+                            uint32_t epoch, uint32_t c, const uint32_t (&tv)[kBatch],
+                            const uint32_t (&x)[kBatch], uint32_t &slabs, uint32_t &edges) {
+  const uint32_t lane = lane_id();
+  const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
+  bool live[kBatch];
+  uint32_t bits[kBatch], wv[kBatch];
+  uint64_t dp[kBatch];
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) {
+    live[k] = k < c && key_lane && x[k] < kTomb;
+    wv[k] = MAP ? __shfl_down(x[k], 1, 64) : 1u;
+    bits[k] = 0u;
+    dp[k] = kInfInf;
+    if (live[k]) {
+      bits[k] = S.invbits[x[k] >> 5];
+      dp[k] = S.dist[x[k]];
+    }
+  }
+  uint64_t cand[kBatch];
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) {
+    cand[k] = kInfInf;
+    if (live[k] && !((bits[k] >> (x[k] & 31u)) & 1u)) {
+      const uint32_t d = dist_of(dp[k]);
+      const uint32_t D = d + (WEIGHTS ? wv[k] : 1u);
+      if (d != kInf && D >= d) cand[k] = pack(D, x[k]);
+    }
+    edges += uint32_t(__popcll(ballot(live[k])));
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) cand[k] = wave_min_u64(cand[k]);
+  uint64_t mine = kInfInf;
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k)
+    if (lane == k) {
+      mine = cand[k];
+      t = tv[k];
+    }
+  bool push = false;
+  if (mine != kInfInf) {
+    const uint64_t old = atomicMin((unsigned long long *)(S.dist + t), (unsigned long long)mine);
+    push = dist_of(mine) < dist_of(old);
+  }
+  if (push) push = atomicExch(S.stamp + t, epoch) != epoch;
+  enqueue(G, S, q, 0, 0, push, t);
+  slabs += c;
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k) {
+    const uint32_t nx = k < c ? read_lane(x[k], kNextLane) : kEmpty;
+    if (nx != kEmpty)
+      pull_chain<MAP, WEIGHTS>(G, R, S, q, epoch, tv[k], slab_ptr(R.pool, nx)[lane], slabs, edges);
+  }
+}
+
 // Pull the in-buckets named by lanes of `m` (lane j: vertex tj, in-bucket
-// bk), kBatch slab loads in flight.
+// bk), kBatch head slabs at a time.
 template <bool MAP, bool WEIGHTS>
 MK_DEV void pull_buckets(const GraphView &G, const GraphView &R, const TravView &S, WaveQ &q,
                          uint32_t epoch, uint64_t m, uint32_t bk, uint32_t tj, uint32_t &slabs,
@@ -829,14 +893,13 @@ MK_DEV void pull_buckets(const GraphView &G, const GraphView &R, const TravView 
         c = t + 1;
       }
     }
-    uint32_t x[kBatch];
+    uint32_t x[kBatch], tv[kBatch];
 #pragma unroll
-    for (uint32_t t = 0; t < kBatch; ++t)
+    for (uint32_t t = 0; t < kBatch; ++t) {
       x[t] = t < c ? R.heads[size_t(read_lane(bk, js[t])) * kSlabWords + lane] : kEmpty;
-#pragma unroll
-    for (uint32_t t = 0; t < kBatch; ++t)
-      if (t < c)
-        pull_chain<MAP, WEIGHTS>(G, R, S, q, epoch, read_lane(tj, js[t]), x[t], slabs, edges);
+      tv[t] = read_lane(tj, js[t]);
+    }
+    batch_pull_work<MAP, WEIGHTS>(G, R, S, q, epoch, c, tv, x, slabs, edges);
   }
 }
 
