@@ -814,9 +814,10 @@ MK_DEV void pull_chain(const GraphView &G, const GraphView &R, const TravView &S
 // pull_chain over the head slabs of c <= kBatch in-buckets at once (slab k
 // of invalidated vertex tv[k], this lane's word x[k]), phase by phase so
 // that their dependent memory operations overlap: every in-neighbour's
-// invalid bit and distance are loaded together, the c wave minima are
-// taken, then lane k applies slab k's minimum to tv[k] (all c atomics in
-// flight at once) and the improved vertices are pushed.  Chained slabs
+// invalid bit and distance are loaded together (with each target's
+// current value), the c wave minima are taken and merged per target, then
+// lane k applies slab k's minimum to tv[k] where it is lower (all atomics
+// in flight at once) and the improved vertices are pushed.  Chained slabs
 // follow one by one.  Wave-collective.
 template <bool MAP, bool WEIGHTS>
 MK_DEV void batch_pull_work(const GraphView &G, const GraphView &R, const TravView &S, WaveQ &q,
@@ -824,6 +825,13 @@ MK_DEV void batch_pull_work(const GraphView &G, const GraphView &R, const TravVi
                             const uint32_t (&x)[kBatch], uint32_t &slabs, uint32_t &edges) {
   const uint32_t lane = lane_id();
   const bool key_lane = (key_mask<MAP>() >> lane) & 1ull;
+  // lane k < c: target tv[k] and its current value (a stale value only
+  // lets an extra atomic through: values only fall during the pull)
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kBatch; ++k)
+    if (lane == k) t = tv[k];
+  const uint64_t tcur = lane < c ? S.dist[t] : 0ull;
   bool live[kBatch];
   uint32_t bits[kBatch], wv[kBatch];
   uint64_t dp[kBatch];
@@ -851,16 +859,20 @@ MK_DEV void batch_pull_work(const GraphView &G, const GraphView &R, const TravVi
   }
 #pragma unroll
   for (uint32_t k = 0; k < kBatch; ++k) cand[k] = wave_min_u64(cand[k]);
+  // slabs of one vertex (a hub's buckets come in order) give one atomic:
+  // atomics on one address serialise
+#pragma unroll
+  for (uint32_t k = kBatch - 1; k > 0; --k)
+    if (k < c && tv[k - 1] == tv[k]) {
+      cand[k - 1] = cand[k] < cand[k - 1] ? cand[k] : cand[k - 1];
+      cand[k] = kInfInf;
+    }
   uint64_t mine = kInfInf;
-  uint32_t t = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kBatch; ++k)
-    if (lane == k) {
-      mine = cand[k];
-      t = tv[k];
-    }
+    if (lane == k) mine = cand[k];
   bool push = false;
-  if (mine != kInfInf) {
+  if (mine < tcur) {
     const uint64_t old = atomicMin((unsigned long long *)(S.dist + t), (unsigned long long)mine);
     push = dist_of(mine) < dist_of(old);
   }
