@@ -627,8 +627,9 @@ __global__ void __launch_bounds__(kBlock)
   expand_step<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, lds);
 }
 
-// Grid-wide barrier number `k` (1, 2, ... since the counters were zeroed)
-// of a kernel whose blocks are all resident (a cooperative launch).  Every
+// Grid-wide barrier number `k` (1, 2, ... since the counters were zeroed,
+// counted over every launch in between, all of one grid size) of a kernel
+// whose blocks are all resident.  Every
 // wave first waits for its own memory operations to complete (the data that
 // crosses steps is written with coherent stores and atomics, so complete
 // means visible to every XCD); thread 0 of each block then arrives as
@@ -1291,7 +1292,7 @@ struct Traversal {
   bool persistent = true;        // frontier steps in one launch (k_expand_loop)
   uint32_t tail_enter = 128;     // tail-mode entry threshold (knob "tail_enter")
   bool coop = false;             // loops always launched cooperatively (knob "coop_launch")
-  unsigned loop_blocks[2] = {0, 0};  // resident-grid size per MODE
+  unsigned loop_blocks = 0;      // resident-grid size of both loop kernels
   uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
   int clock_khz = 100000;        // device wall-clock rate
 
@@ -1362,8 +1363,13 @@ struct Traversal {
     // knob "barrier_timeout_ms" (default five seconds)
     barrier_timeout = uint64_t(khz) * uint64_t(tune_get("barrier_timeout_ms", 5000));
     if (persistent) {
-      if (int r = loop_grid<0>(loop_blocks[0])) return r;
-      if (int r = loop_grid<1>(loop_blocks[1])) return r;
+      // one grid for both loop kernels: the barrier lines count arrivals
+      // cumulatively across the loops of one call (cnt->bar_done), so every
+      // loop between two clears must arrive with the same number of blocks
+      unsigned b0 = 0, b1 = 0;
+      if (int r = loop_grid<0>(b0)) return r;
+      if (int r = loop_grid<1>(b1)) return r;
+      loop_blocks = std::min(b0, b1);
     }
     return MK_OK;
   }
@@ -1419,11 +1425,15 @@ struct Traversal {
 
   // Blocks of a k_expand_loop grid: the launch shape of k_expand, capped at
   // what the device keeps resident at once (a grid barrier needs every block
-  // running).
+  // running).  One block slot per CU is kept in reserve: the occupancy
+  // figure of 7 blocks per CU given for the unweighted relax loop (67
+  // VGPRs) was measured not to be resident under an ordinary launch (its
+  // first barrier timed out), while 6 per CU ran for every variant.
   template <int MODE> int loop_grid(unsigned &blocks) {
     int per_cu = 0;
     MK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, loop_kernel_for<MODE>(), kBlock,
                                                         0));
+    if (per_cu > 1) --per_cu;
     const uint64_t resident = uint64_t(per_cu > 0 ? per_cu : 0) * uint64_t(device_cus(g->device));
     if (resident == 0) return fail(MK_ERR_HIP, "frontier loop kernel cannot be resident");
     blocks = unsigned(std::min<uint64_t>(
@@ -1444,10 +1454,10 @@ struct Traversal {
     // cooperative launch takes no dispatch events, and events around it
     // would include the runtime's extra dispatch work
     if (coop || !loop_stream_alone(g->device))
-      MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]),
+      MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks),
                                         dim3(kBlock), args, 0u, stream));
     else  // the only stream running loops; the grid is sized to be resident
-      MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks[MODE]), dim3(kBlock), args,
+      MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks), dim3(kBlock), args,
                              0u, stream));
     return MK_OK;
   }

@@ -223,3 +223,45 @@ def test_barrier_wait_limit_ends_the_call_and_static_recovers(gpu):
     sp = mk.SSSP(g, 0)
     sp.Static()
     check(sp, n, s, d, w)
+
+
+@pytest.mark.parametrize("wpc", [32, 64])
+def test_widest_launch_shapes_keep_loop_barriers_consistent(gpu, wpc):
+    # waves_per_cu at and beyond what a CU holds: the loop grid is capped
+    # one block per CU below the occupancy figure, and the relax and
+    # invalidation loops, which reach different occupancy limits but share
+    # one call's barrier counters, take one grid (a grid not all resident,
+    # or two grids, make a barrier wait time out with MK_ERR_STATE).
+    # Weighted SSSP and BFS on the same graph cover both kernel variants.
+    from meerkat_amd import autotune
+
+    n, s, d, w = gen.rmat(13, 8, seed=91, weighted=True)
+    pool = mk.SlabPool(1 << 26)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=True,
+                        in_degree_hints=gen.degree_hints(n, d))
+    g.InsertEdges(s, d, w)
+    before = autotune.get_knob("waves_per_cu")
+    autotune.set_knob("waves_per_cu", wpc)
+    try:
+        sp, bf = mk.SSSP(g, 0), mk.BFS(g, 0)
+    finally:
+        autotune.set_knob("waves_per_cu", before)
+    sp.Static()
+    check(sp, n, s, d, w)
+    bf.Static()
+    keys, wt = gen.keys_of(s, d), w.copy()
+    ds, dd = gen.delete_batch(keys, 2000, seed=92)
+    g.DeleteEdges(ds, dd)
+    sp.Decremental(ds, dd, None)
+    bf.Decremental(ds, dd)
+    keep = ~np.isin(keys, gen.keys_of(ds, dd))
+    keys, wt = keys[keep], wt[keep]
+    bs, bd, bw = gen.insert_batch(keys, n, 2000, seed=93, weighted=True, wseed=94)
+    g.InsertEdges(bs, bd, bw)
+    sp.Incremental(bs, bd, bw)
+    bf.Incremental(bs, bd)
+    keys, wt = np.r_[keys, gen.keys_of(bs, bd)], np.r_[wt, bw]
+    order = np.argsort(keys)
+    s2, d2 = gen.split_keys(keys[order])
+    check(sp, n, s2, d2, wt[order])
+    check(bf, n, s2, d2)
