@@ -369,9 +369,9 @@ MK_DEV void chain_work(const GraphView &G, const TravView &S, Sink &sink, uint32
 // slab_work over c <= kBatch slabs at once (slab k of vertex u[k] with
 // distance du[k], this lane's word v[k]), phase by phase, so that the
 // dependent memory operations of the slabs overlap instead of following
-// one another: every distance (invalid bit) load, then every atomic, then
-// every stamp (bitmap) update, then every pushed vertex's bucket count,
-// then the pushes.  Wave-collective; returns live keys seen.
+// one another: every distance (invalid bit) load, then every atomic (with
+// the bucket count of each candidate vertex), then every stamp (bitmap)
+// update, then the pushes.  Wave-collective; returns live keys seen.
 template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
 MK_DEV uint32_t batch_slab_work(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
                                 uint32_t c, const uint32_t (&u)[kBatch],
@@ -396,15 +396,21 @@ MK_DEV uint32_t batch_slab_work(const GraphView &G, const TravView &S, Sink &sin
       cur[k] = S.dist[v[k]];
     }
   }
+  // the atomics, with the bucket counts of their lanes loaded alongside
+  // (read only where the vertex is then pushed)
   bool push[kBatch];
+  uint32_t nb[kBatch];
 #pragma unroll
   for (uint32_t k = 0; k < kBatch; ++k) {
     push[k] = false;
+    const bool cand = MODE == 1 ? ok[k] && cur[k] != kInfInf && parent_of(cur[k]) == u[k]
+                                : ok[k] && (MODE == 2 || D[k] < dist_of(cur[k]));
+    nb[k] = push_buckets(G, cand, v[k]);
     if (MODE == 1) {
-      if (ok[k] && cur[k] != kInfInf && parent_of(cur[k]) == u[k])
+      if (cand)
         push[k] = atomicCAS((unsigned long long *)(S.dist + v[k]), (unsigned long long)cur[k],
                             (unsigned long long)kInfInf) == cur[k];
-    } else if (ok[k] && (MODE == 2 || D[k] < dist_of(cur[k]))) {
+    } else if (cand) {
       cur[k] = atomicMin((unsigned long long *)(S.dist + v[k]), (unsigned long long)pack(D[k], u[k]));
       push[k] = D[k] < dist_of(cur[k]);  // improved
     }
@@ -417,9 +423,6 @@ MK_DEV uint32_t batch_slab_work(const GraphView &G, const TravView &S, Sink &sin
       push[k] = atomicExch(S.stamp + v[k], epoch) != epoch;
     }
   }
-  uint32_t nb[kBatch];
-#pragma unroll
-  for (uint32_t k = 0; k < kBatch; ++k) nb[k] = push_buckets(G, push[k], v[k]);
   uint32_t edges = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kBatch; ++k) {
