@@ -279,13 +279,13 @@ struct TailSink {
 };
 
 // Block-aggregated append for kernels with one item per thread: one global
-// atomicAdd per block.  Called by every thread of the block.
+// atomicAdd per block.  `nb`: bucket count of v on the pushing lanes
+// (loaded by the caller alongside its atomics).  Called by every thread of
+// the block.
 template <uint32_t NW>
 MK_DEV void block_enqueue(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
-                          bool push, uint32_t v, uint32_t *sh) {
+                          bool push, uint32_t v, uint32_t nb, uint32_t *sh) {
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-  uint32_t nb = 0;
-  if (push) nb = G.bucket_off[v + 1] - G.bucket_off[v];
   const bool small = push && nb <= kSmallBuckets;
   const uint64_t sm = ballot(small);
   if (lane == 0) sh[w] = uint32_t(__popcll(sm));
@@ -1061,13 +1061,24 @@ __global__ void __launch_bounds__(kItemBlock)
   const uint32_t lane = lane_id();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t u = 0, v = 0, wgt = 1;
+  uint64_t xu = kInfInf, xv = kInfInf;
   bool act = false, need = false;
   if (i < n) {
+    // the edge, its status and batch weight, then both distances, each
+    // group of loads issued together
     u = src[i];
     v = dst[i];
-    act = u != v && u < G.n && v < G.n && dist_of(S.dist[u]) != kInf;
+    const bool applied = WEIGHTS && status && wts && status[i] == MK_EDGE_APPLIED;
+    const uint32_t bw = WEIGHTS && wts ? wts[i] : 1u;
+    act = u != v && u < G.n && v < G.n;
+    if (act) {
+      // a stale dist[v] only lets an extra atomic through: values only fall
+      xu = S.dist[u];
+      xv = S.dist[v];
+    }
+    act = act && dist_of(xu) != kInf;
     if (act && WEIGHTS) {
-      if (status && wts && status[i] == MK_EDGE_APPLIED) wgt = wts[i];
+      if (applied) wgt = bw;
       else need = true;
     }
   }
@@ -1085,16 +1096,18 @@ __global__ void __launch_bounds__(kItemBlock)
     }
   }
   bool push = false;
+  uint32_t nb = 0;
   if (act) {
-    const uint32_t du = dist_of(S.dist[u]);
+    const uint32_t du = dist_of(xu);
     const uint32_t D = du + (WEIGHTS ? wgt : 1u);
-    if (D >= du && D < dist_of(S.dist[v])) {
+    if (D >= du && D < dist_of(xv)) {
+      nb = G.bucket_off[v + 1] - G.bucket_off[v];  // alongside the atomic
       const uint64_t old = atomicMin((unsigned long long *)(S.dist + v),
                                      (unsigned long long)pack(D, u));
       if (D < dist_of(old)) push = atomicExch(S.stamp + v, epoch) != epoch;
     }
   }
-  block_enqueue<kItemBlock / 64>(G, S, 0, 0, push, v, sh);
+  block_enqueue<kItemBlock / 64>(G, S, 0, 0, push, v, nb, sh);
 }
 
 // Decremental seed: batch edge (u, v) was deleted; if it was v's tree edge,
@@ -1109,7 +1122,7 @@ __global__ void __launch_bounds__(kItemBlock)
   const uint32_t lane = lane_id();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool push = false, orphan = false;
-  uint32_t v = 0;
+  uint32_t v = 0, nb = 0;
   if (i < n) {
     const uint32_t u = src[i];
     v = dst[i];
@@ -1121,10 +1134,13 @@ __global__ void __launch_bounds__(kItemBlock)
           // lists v once (the list holds at most n vertices)
           const uint32_t bit = 1u << (v & 31u);
           orphan = (atomicOr(S.invbits + (v >> 5), bit) & bit) == 0u;
-        } else if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
-                             (unsigned long long)kInfInf) == cur) {
-          atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
-          push = true;
+        } else {
+          nb = G.bucket_off[v + 1] - G.bucket_off[v];  // alongside the CAS
+          if (atomicCAS((unsigned long long *)(S.dist + v), (unsigned long long)cur,
+                        (unsigned long long)kInfInf) == cur) {
+            atomicOr(S.invbits + (v >> 5), 1u << (v & 31u));
+            push = true;
+          }
         }
       }
     }
@@ -1140,7 +1156,7 @@ __global__ void __launch_bounds__(kItemBlock)
     }
     return;
   }
-  block_enqueue<kItemBlock / 64>(G, S, 0, 0, push, v, sh);
+  block_enqueue<kItemBlock / 64>(G, S, 0, 0, push, v, nb, sh);
 }
 
 // Orphans with at most this many in-buckets are searched for another parent.
