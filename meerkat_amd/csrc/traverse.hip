@@ -187,9 +187,9 @@ MK_DEV void enqueue(const GraphView &G, const TravView &S, WaveQ &q, uint32_t si
 }
 
 // wq_flush for every wave of the block at once, one atomic per block;
-// `sh` holds kBlock / 64 + 1 words of LDS.  Called by every thread.
+// `sh` holds blockDim.x / 64 + 1 words of LDS.  Called by every thread.
 MK_DEV void block_flush(WaveQ &q, uint32_t *queue, uint32_t *count, uint32_t *sh) {
-  constexpr uint32_t NW = kBlock / 64;
+  const uint32_t NW = blockDim.x >> 6;
   const uint32_t w = threadIdx.x >> 6;
   if (lane_id() == 0) sh[w] = q.n;
   __syncthreads();
@@ -561,8 +561,14 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
 // side^1 / slot+1; clear slot+2.  MODE 0 relaxes, MODE 1 invalidates.  A
 // small frontier without hub entries is run in tail mode by block 0.
 // Called by every thread of every block of the grid; `lds` holds
-// kExpandLds words.
+// kExpandLds words (kLoopLds for blocks of kLoopBlock threads).
 constexpr uint32_t kWaveLds = (kBlock / 64) * kWaveQ + kBlock / 64 + 1;  // staging + block_flush
+// Blocks of the persistent loop are four times larger: a quarter of the
+// blocks arrive at each grid barrier and flush their staged frontier with
+// one atomic each on the same counter (such atomics serialise).
+constexpr uint32_t kLoopBlock = 1024;
+constexpr uint32_t kLoopWaveLds = (kLoopBlock / 64) * kWaveQ + kLoopBlock / 64 + 1;
+constexpr uint32_t kLoopLds = 2 * kTailCap + 4 > kLoopWaveLds ? 2 * kTailCap + 4 : kLoopWaveLds;
 constexpr uint32_t kExpandLds = 2 * kTailCap + 4 > kWaveLds ? 2 * kTailCap + 4 : kWaveLds;
 
 template <bool MAP, bool WEIGHTS, int MODE>
@@ -614,7 +620,7 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
   // small vertices: up to 64 per wave step, spread over all waves when few
   expand_rows<MAP, WEIGHTS, MODE>(G, S, sink, epoch, S.q_small[side], n_small, gw, W, slabs, edges,
                                   visited);
-  block_flush(q, S.q_small[nside], &S.cnt->small[nslot], lds + (kBlock / 64) * kWaveQ);
+  block_flush(q, S.q_small[nside], &S.cnt->small[nslot], lds + (blockDim.x / 64) * kWaveQ);
   if (lane == 0) {
     add_stat(S.cnt, MODE, kStatSlabs, slabs);
     add_stat(S.cnt, MODE, kStatEdges, edges);
@@ -686,10 +692,10 @@ MK_DEV bool grid_barrier(BarLines *bl, Counts *c, uint32_t k, uint64_t timeout, 
 // frontier counters of every slot and cnt->ninv are cleared at the end
 // (after the last step every block reads only the empty slot's counts).
 template <bool MAP, bool WEIGHTS, int MODE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kLoopBlock)
     k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
                   uint64_t timeout, uint32_t *zero, uint32_t nzero, uint32_t clear_exit) {
-  __shared__ uint32_t lds[kExpandLds];
+  __shared__ uint32_t lds[kLoopLds];
   __shared__ uint32_t flag;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += gridDim.x * blockDim.x)
     zero[i] = 0u;
@@ -1442,21 +1448,23 @@ struct Traversal {
                        : loop_kernel<false, false, MODE>();
   }
 
-  // Blocks of a k_expand_loop grid: the launch shape of k_expand, capped at
-  // what the device keeps resident at once (a grid barrier needs every block
-  // running).  One block slot per CU is kept in reserve: the occupancy
-  // figure of 7 blocks per CU given for the unweighted relax loop (67
-  // VGPRs) was measured not to be resident under an ordinary launch (its
-  // first barrier timed out), while 6 per CU ran for every variant.
+  // Blocks of a k_expand_loop grid: max_waves in blocks of kLoopBlock
+  // threads, capped at what the device keeps resident at once (a grid
+  // barrier needs every block running).  One block slot per CU is kept in
+  // reserve where there are several: the occupancy figure of 7 blocks of
+  // 256 threads per CU given for the unweighted relax loop (67 VGPRs) was
+  // measured not to be resident under an ordinary launch (its first barrier
+  // timed out), while 6 per CU ran for every variant.
   template <int MODE> int loop_grid(unsigned &blocks) {
     int per_cu = 0;
-    MK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, loop_kernel_for<MODE>(), kBlock,
-                                                        0));
+    MK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, loop_kernel_for<MODE>(),
+                                                        kLoopBlock, 0));
     if (per_cu > 1) --per_cu;
     const uint64_t resident = uint64_t(per_cu > 0 ? per_cu : 0) * uint64_t(device_cus(g->device));
     if (resident == 0) return fail(MK_ERR_HIP, "frontier loop kernel cannot be resident");
     blocks = unsigned(std::min<uint64_t>(
-        std::min<uint64_t>(blocks_for_waves(uint64_t(max_waves)), resident),
+        std::min<uint64_t>((uint64_t(max_waves) + kLoopBlock / 64 - 1) / (kLoopBlock / 64),
+                           resident),
         uint64_t(kMaxBarGroups) * kBarGroup));  // the barrier's group lines
     return MK_OK;
   }
@@ -1474,9 +1482,9 @@ struct Traversal {
     // would include the runtime's extra dispatch work
     if (coop || !loop_stream_alone(g->device))
       MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks),
-                                        dim3(kBlock), args, 0u, stream));
+                                        dim3(kLoopBlock), args, 0u, stream));
     else  // the only stream running loops; the grid is sized to be resident
-      MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks), dim3(kBlock), args,
+      MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks), dim3(kLoopBlock), args,
                              0u, stream));
     return MK_OK;
   }
