@@ -561,14 +561,14 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
 // side^1 / slot+1; clear slot+2.  MODE 0 relaxes, MODE 1 invalidates.  A
 // small frontier without hub entries is run in tail mode by block 0.
 // Called by every thread of every block of the grid; `lds` holds
-// kExpandLds words (kLoopLds for blocks of kLoopBlock threads).
+// kExpandLds words (kLoopLds for blocks of kWideBlock threads).
 constexpr uint32_t kWaveLds = (kBlock / 64) * kWaveQ + kBlock / 64 + 1;  // staging + block_flush
 // Blocks of the persistent loop are four times larger: a quarter of the
 // blocks arrive at each grid barrier and flush their staged frontier with
 // one atomic each on the same counter (such atomics serialise).
-constexpr uint32_t kLoopBlock = 1024;
-constexpr uint32_t kLoopWaveLds = (kLoopBlock / 64) * kWaveQ + kLoopBlock / 64 + 1;
-constexpr uint32_t kLoopLds = 2 * kTailCap + 4 > kLoopWaveLds ? 2 * kTailCap + 4 : kLoopWaveLds;
+constexpr uint32_t kWideBlock = 1024;
+constexpr uint32_t kWideWaveLds = (kWideBlock / 64) * kWaveQ + kWideBlock / 64 + 1;
+constexpr uint32_t kLoopLds = 2 * kTailCap + 4 > kWideWaveLds ? 2 * kTailCap + 4 : kWideWaveLds;
 constexpr uint32_t kExpandLds = 2 * kTailCap + 4 > kWaveLds ? 2 * kTailCap + 4 : kWaveLds;
 
 template <bool MAP, bool WEIGHTS, int MODE>
@@ -692,7 +692,7 @@ MK_DEV bool grid_barrier(BarLines *bl, Counts *c, uint32_t k, uint64_t timeout, 
 // frontier counters of every slot and cnt->ninv are cleared at the end
 // (after the last step every block reads only the empty slot's counts).
 template <bool MAP, bool WEIGHTS, int MODE>
-__global__ void __launch_bounds__(kLoopBlock)
+__global__ void __launch_bounds__(kWideBlock)
     k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
                   uint64_t timeout, uint32_t *zero, uint32_t nzero, uint32_t clear_exit) {
   __shared__ uint32_t lds[kLoopLds];
@@ -1448,7 +1448,7 @@ struct Traversal {
                        : loop_kernel<false, false, MODE>();
   }
 
-  // Blocks of a k_expand_loop grid: max_waves in blocks of kLoopBlock
+  // Blocks of a k_expand_loop grid: max_waves in blocks of kWideBlock
   // threads, capped at what the device keeps resident at once (a grid
   // barrier needs every block running).  One block slot per CU is kept in
   // reserve where there are several: the occupancy figure of 7 blocks of
@@ -1458,12 +1458,12 @@ struct Traversal {
   template <int MODE> int loop_grid(unsigned &blocks) {
     int per_cu = 0;
     MK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, loop_kernel_for<MODE>(),
-                                                        kLoopBlock, 0));
+                                                        kWideBlock, 0));
     if (per_cu > 1) --per_cu;
     const uint64_t resident = uint64_t(per_cu > 0 ? per_cu : 0) * uint64_t(device_cus(g->device));
     if (resident == 0) return fail(MK_ERR_HIP, "frontier loop kernel cannot be resident");
     blocks = unsigned(std::min<uint64_t>(
-        std::min<uint64_t>((uint64_t(max_waves) + kLoopBlock / 64 - 1) / (kLoopBlock / 64),
+        std::min<uint64_t>((uint64_t(max_waves) + kWideBlock / 64 - 1) / (kWideBlock / 64),
                            resident),
         uint64_t(kMaxBarGroups) * kBarGroup));  // the barrier's group lines
     return MK_OK;
@@ -1482,9 +1482,9 @@ struct Traversal {
     // would include the runtime's extra dispatch work
     if (coop || !loop_stream_alone(g->device))
       MK_HIP(hipLaunchCooperativeKernel(loop_kernel_for<MODE>(), dim3(loop_blocks),
-                                        dim3(kLoopBlock), args, 0u, stream));
+                                        dim3(kWideBlock), args, 0u, stream));
     else  // the only stream running loops; the grid is sized to be resident
-      MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks), dim3(kLoopBlock), args,
+      MK_HIP(hipLaunchKernel(loop_kernel_for<MODE>(), dim3(loop_blocks), dim3(kWideBlock), args,
                              0u, stream));
     return MK_OK;
   }
