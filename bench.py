@@ -161,13 +161,17 @@ def main():
                     help="leave the library's kernel timing off in the timed region "
                          "(no roofline)")
     ap.add_argument("--breakdown-steps", type=int, default=5,
-                    help="steps after the timed region with every kernel timed (the "
-                         "per-kernel table; dispatch events cost host time per launch)")
+                    help="steps after the timed region, twice: with synchronous SSSP calls "
+                         "(per-call rates), then with every kernel timed (the per-kernel "
+                         "table; dispatch events cost host time per launch)")
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                     help="set a launch knob (mk_tune_set) before any handle is created")
     ap.add_argument("--sync-mutations", action="store_true",
                     help="each insert/delete waits for its own outcome instead of the "
                          "following SSSP call settling it (deferred checks)")
+    ap.add_argument("--sync-calls", action="store_true",
+                    help="each SSSP call waits for its own outcome instead of returning once "
+                         "queued and being settled by the next call (deferred calls)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -195,7 +199,9 @@ def main():
 
     # update stream: deletions sampled without replacement from the initial
     # edges (each still present when deleted), insertions new edges
-    steps_total = args.warmup + args.steps + args.breakdown_steps
+    # after the timed steps: breakdown steps with synchronous calls, first
+    # for the per-call rates, then for the per-kernel table
+    steps_total = args.warmup + args.steps + 2 * args.breakdown_steps
     timed_end = args.warmup + args.steps
     half = args.batch // 2
     rng = np.random.default_rng(args.seed + 1)
@@ -230,35 +236,48 @@ def main():
     st = sp.last_stats()
     log(f"static SSSP {static_ms:.2f} ms, {st}")
 
-    def step(i, rec):
+    def step(i, rec=None, calls=None):
+        """One update step.  `calls` (synchronous calls only) collects each
+        SSSP call's time and edges read."""
         ds, dd = dels[i]
         a = time.perf_counter()
         g.DeleteEdges(ds, dd)
         b_ = time.perf_counter()
         t_decr = sp.Decremental(ds, dd)
-        e_decr = sp.last_stats()["edges_read"]
+        e_decr = sp.last_stats()["edges_read"] if calls is not None else 0
         is_, id_, iw = inss[i]
         c = time.perf_counter()
         g.InsertEdges(is_, id_, iw)
         d_ = time.perf_counter()
         t_incr = sp.Incremental(is_, id_, iw)
-        e_incr = sp.last_stats()["edges_read"]
+        e_incr = sp.last_stats()["edges_read"] if calls is not None else 0
         e = time.perf_counter()
         if rec is not None:
             rec["upd_s"].append((b_ - a) + (d_ - c))
-            rec["decr_ms"].append(t_decr)
-            rec["incr_ms"].append(t_incr)
-            rec["decr_edges"].append(e_decr)
-            rec["incr_edges"].append(e_incr)
             rec["step_s"].append(e - a)
+        if calls is not None:
+            calls["decr_ms"].append(t_decr)
+            calls["incr_ms"].append(t_incr)
+            calls["decr_edges"].append(e_decr)
+            calls["incr_edges"].append(e_incr)
+
+    def settle():
+        """Wait for every queued call (the library's own streams)."""
+        sp.sync()
+        g.sync()
+        torch.cuda.synchronize()
 
     # deferred mutation checks: each batch's range/pool outcome is settled by
-    # the SSSP call that follows it (one host wait per pair of calls)
+    # the SSSP call that follows it; deferred SSSP calls: each is settled by
+    # the next one, so the host queues a step's mutation and call while the
+    # previous call runs (the host waits once per call, never with the device
+    # idle)
     g.set_deferred(not args.sync_mutations)
+    sp.set_deferred(not args.sync_calls)
     for i in range(args.warmup):
-        step(i, None)
-    torch.cuda.synchronize()
-    rec = {k: [] for k in ("upd_s", "decr_ms", "incr_ms", "decr_edges", "incr_edges", "step_s")}
+        step(i)
+    settle()
+    rec = {k: [] for k in ("upd_s", "step_s")}
     profiling.reset()
     # in the timed region only the persistent frontier loops are timed (by
     # their own device clock); timing every kernel costs host time per launch
@@ -269,16 +288,23 @@ def main():
     ts = time.perf_counter()
     for i in range(args.warmup, timed_end):
         step(i, rec)
-    torch.cuda.synchronize()
+    settle()
     elapsed = time.perf_counter() - ts
     profiling.marker(dev)
     prof = profiling.totals()
-    # per-kernel table: a few more steps with every kernel timed
+    # per-call rates: a few more steps with synchronous calls (each returns
+    # its own time and edges read), only the loops timed
+    sp.set_deferred(False)
+    calls = {k: [] for k in ("decr_ms", "incr_ms", "decr_edges", "incr_edges")}
+    rate_end = timed_end + args.breakdown_steps
+    for i in range(timed_end, rate_end):
+        step(i, calls=calls)
+    # per-kernel table: a few more with every kernel timed
     profiling.reset()
     profiling.enable(True)
-    for i in range(timed_end, steps_total):
-        step(i, None)
-    torch.cuda.synchronize()
+    for i in range(rate_end, steps_total):
+        step(i)
+    settle()
     profiling.enable(False)
     prof_all = profiling.totals()
     g.set_deferred(False)
@@ -310,10 +336,11 @@ def main():
     mut_s = (sum(prof_all[k]["ms"] for k in ("k_insert", "k_delete") if k in prof_all) * 1e-3 / nb
              * K) or float("nan")  # mutation kernel time for K steps
     step_ms = np.array(rec["step_s"]) * 1e3
-    incr_gteps = (np.sum(rec["incr_edges"]) / (np.sum(rec["incr_ms"]) * 1e-3) / 1e9
-                  if np.sum(rec["incr_ms"]) > 0 else 0.0)
-    decr_gteps = (np.sum(rec["decr_edges"]) / (np.sum(rec["decr_ms"]) * 1e-3) / 1e9
-                  if np.sum(rec["decr_ms"]) > 0 else 0.0)
+    # per-call rates from the synchronous breakdown steps
+    incr_gteps = (np.sum(calls["incr_edges"]) / (np.sum(calls["incr_ms"]) * 1e-3) / 1e9
+                  if np.sum(calls["incr_ms"]) > 0 else 0.0)
+    decr_gteps = (np.sum(calls["decr_edges"]) / (np.sum(calls["decr_ms"]) * 1e-3) / 1e9
+                  if np.sum(calls["decr_ms"]) > 0 else 0.0)
 
     # dominant kernel by time (per-kernel table); its figures from the timed
     # region when it was timed there (the frontier loops are)

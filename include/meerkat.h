@@ -183,6 +183,21 @@ int mk_bfs_set_source(mk_bfs *b, uint32_t source);
  * last compute call; edges/ms gives its traversed-edges rate. */
 int mk_bfs_last_stats(mk_bfs *b, uint32_t *iterations, uint64_t *slabs_read,
                       uint64_t *edges_read);
+/* Deferred calls (no reference counterpart; the reference's compute calls
+ * end with a device synchronisation).  With on != 0, mk_bfs_incremental and
+ * mk_bfs_decremental return once the call is queued on the graph's own
+ * stream (*ms = 0: not measured); the host does not wait for it, so the
+ * caller's next mutation and call are queued while it runs.  Its outcome is
+ * settled by the handle's next call (compute, read, last_stats, set_source,
+ * mk_bfs_sync, or set_deferred(0)), which waits for it and, if it failed,
+ * returns its error (MK_ERR_STATE: a loop gave up; the results stay invalid
+ * until Static) instead of doing its own work.  A call whose relax loop
+ * needs more steps than one launch reserves is recomputed from scratch at
+ * settling (exact for the graph as it is then).  The batch buffers must stay
+ * valid until the call is settled; a mutation of the graph queued on another
+ * stream waits for it on the device.  Static calls are never deferred. */
+int mk_bfs_set_deferred(mk_bfs *b, int on);
+int mk_bfs_sync(mk_bfs *b);
 
 /* ------------------------------------------------------ A11 SSSP
  * SSSP(G, Src, N) (include/Algorithms/SSSP.cuh:34-69; impl/SSSPImpl.cuh
@@ -202,6 +217,9 @@ int mk_sssp_read(mk_sssp *s, uint32_t *dist_host, uint32_t *parent_host);
 int mk_sssp_set_source(mk_sssp *s, uint32_t source);
 int mk_sssp_last_stats(mk_sssp *s, uint32_t *iterations, uint64_t *slabs_read,
                        uint64_t *edges_read);
+/* As mk_bfs_set_deferred / mk_bfs_sync. */
+int mk_sssp_set_deferred(mk_sssp *s, int on);
+int mk_sssp_sync(mk_sssp *s);
 
 /* ------------------------------------------------------ A12 PageRank
  * PageRank(G, DampingFactor, ErrorMargin, N, VertexOutDegree, MaxIter)

@@ -85,6 +85,8 @@ _PROTOS = {
     "mk_bfs_read": [vp, vp, vp],
     "mk_bfs_set_source": [vp, u32],
     "mk_bfs_last_stats": [vp, P(u32), P(u64), P(u64)],
+    "mk_bfs_set_deferred": [vp, i32],
+    "mk_bfs_sync": [vp],
     "mk_sssp_create": [vp, u32, P(vp)],
     "mk_sssp_destroy": [vp],
     "mk_sssp_static": [vp, P(f32)],
@@ -93,6 +95,8 @@ _PROTOS = {
     "mk_sssp_read": [vp, vp, vp],
     "mk_sssp_set_source": [vp, u32],
     "mk_sssp_last_stats": [vp, P(u32), P(u64), P(u64)],
+    "mk_sssp_set_deferred": [vp, i32],
+    "mk_sssp_sync": [vp],
     "mk_pagerank_create": [vp, f32, f32, u32, vp, P(vp)],
     "mk_pagerank_destroy": [vp],
     "mk_pagerank_static": [vp, P(f32)],

@@ -4,7 +4,8 @@ Each class keeps the reference's method names and protocol: mutate the graph
 first (``InsertEdges`` / ``DeleteEdges``), then call ``Incremental`` /
 ``Decremental`` with the same batch; every compute call blocks and returns
 elapsed milliseconds (include/Algorithms/BFS.cuh:34-68, SSSP.cuh:34-69,
-PageRank.cuh:32-67, TC.cuh:28-40).
+PageRank.cuh:32-67, TC.cuh:28-40), except BFS/SSSP Incremental /
+Decremental on a handle set to deferred mode (``set_deferred``).
 """
 import ctypes
 
@@ -49,6 +50,17 @@ class _Traversal:
     def Parents(self):
         return self.read()[1]
 
+    def set_deferred(self, on=True):
+        """Deferred calls: Incremental / Decremental return once queued on the
+        graph's stream (returning 0.0: not measured) and the host does not
+        wait for them; the handle's next call, read, last_stats or sync()
+        settles the call, raising its MeerkatError if it failed."""
+        _lib.call(f"{self._prefix}_set_deferred", self._h, int(on))
+
+    def sync(self):
+        """Settle a deferred call: raises its MeerkatError if it failed."""
+        _lib.call(f"{self._prefix}_sync", self._h)
+
     def last_stats(self):
         it, slabs, edges = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
         _lib.call(f"{self._prefix}_last_stats", self._h, ctypes.byref(it), ctypes.byref(slabs),
@@ -77,12 +89,14 @@ class BFS(_Traversal):
         s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
         ms = ctypes.c_float()
         _lib.call("mk_bfs_incremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        self._batch = (s, d)  # a deferred call reads it until it is settled
         return ms.value
 
     def Decremental(self, src, dst):
         s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
         ms = ctypes.c_float()
         _lib.call("mk_bfs_decremental", self._h, dptr(s), dptr(d), len(s), ctypes.byref(ms))
+        self._batch = (s, d)
         return ms.value
 
 
@@ -97,6 +111,7 @@ class SSSP(_Traversal):
         ms = ctypes.c_float()
         _lib.call("mk_sssp_incremental", self._h, dptr(s), dptr(d), dptr(w), len(s),
                   ctypes.byref(ms))
+        self._batch = (s, d, w)  # a deferred call reads it until it is settled
         return ms.value
 
     def Decremental(self, src, dst, weights=None):
@@ -105,6 +120,7 @@ class SSSP(_Traversal):
         ms = ctypes.c_float()
         _lib.call("mk_sssp_decremental", self._h, dptr(s), dptr(d), dptr(w), len(s),
                   ctypes.byref(ms))
+        self._batch = (s, d, w)  # a deferred call reads it until it is settled
         return ms.value
 
 

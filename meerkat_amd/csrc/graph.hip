@@ -608,6 +608,7 @@ static int graph_alloc(mk_pool *pool, uint32_t n, int weighted, double load_fact
   // NULL stream itself makes the runtime order it against every stream
   e = hipStreamCreate(&g->own_stream);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&g->mut_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->trav_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     (void)hipFree(g->heads);
     (void)hipFree(g->bucket_off);
@@ -676,6 +677,7 @@ int mk_graph_destroy(mk_graph *g) {
   (void)hipFree(g->counters);
   if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
   if (g->mut_ev) (void)hipEventDestroy(g->mut_ev);
+  if (g->trav_ev) (void)hipEventDestroy(g->trav_ev);
   g->status.release();
   g->host_scratch.release();
   g->pool->graphs--;
@@ -848,6 +850,8 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_insert: mirror is read-only");
   if (int r = graph_settle(g)) return r;
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
+  // a deferred BFS/SSSP call still reading the graph on its own stream
+  if (n && g->trav_pending && s != g->own_stream) MK_HIP(hipStreamWaitEvent(s, g->trav_ev, 0));
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n)) return r;
   launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
@@ -860,6 +864,8 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_delete: mirror is read-only");
   if (int r = graph_settle(g)) return r;
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
+  // a deferred BFS/SSSP call still reading the graph on its own stream
+  if (n && g->trav_pending && s != g->own_stream) MK_HIP(hipStreamWaitEvent(s, g->trav_ev, 0));
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n)) return r;
   launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);

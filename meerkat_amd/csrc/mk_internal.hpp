@@ -279,6 +279,10 @@ struct mk_graph {
   // queued; `pend` describes it until the next call settles it
   bool deferred = false;
   hipEvent_t mut_ev = nullptr;       // recorded after the queued mutation
+  // recorded (on own_stream) after each deferred BFS/SSSP call of the
+  // graph's handles; a mutation on another stream waits for it
+  hipEvent_t trav_ev = nullptr;
+  bool trav_pending = false;
   struct Pending {
     bool active;
     int op;  // 1 insert, 2 delete

@@ -1297,6 +1297,8 @@ struct Traversal {
   bool weights = false;  // SSSP
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool defer = false;    // incremental/decremental return once queued (settle() reads them)
+  bool pending = false;  // a deferred call's counters are on their way to `host`
   uint64_t *dist = nullptr;
   uint32_t *stamp = nullptr;
   uint32_t *invbits = nullptr;
@@ -1506,6 +1508,14 @@ struct Traversal {
     parse(*host.as<Counts>(), h);
     return MK_OK;
   }
+  // The counter copy, then the stop event: an event recorded between the
+  // last kernel and the copy was measured to delay the copy's start by
+  // ~11 us.  The caller waits (stream or ev1) before parsing `host`.
+  int copy_counts_then_stop() {
+    MK_HIP(hipMemcpyAsync(host.ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
+    MK_HIP(hipEventRecord(ev1, stream));
+    return MK_OK;
+  }
 
   static void parse(const Counts &c, Totals &h) {
     for (int i = 0; i < 3; ++i) {
@@ -1605,8 +1615,9 @@ struct Traversal {
       const uint32_t j0 = relax_end;
       relax_end += kLoopSteps;
       if (int r = launch_loop<0>(j0, relax_end, reserve_epochs(kLoopSteps))) return r;
-      MK_HIP(hipEventRecord(ev1, stream));
-      if (int r = read_counts(h)) return r;
+      if (int r = copy_counts_then_stop()) return r;
+      MK_HIP(hipStreamSynchronize(stream));
+      parse(*host.as<Counts>(), h);
       if (h.abort) return fail(MK_ERR_STATE, "frontier loop: a grid barrier wait timed out");
     }
     if (profiling_on()) {
@@ -1693,6 +1704,7 @@ struct Traversal {
   // them `dirty` for the next begin() to clear.
   int begin() {
     MK_HIP(hipSetDevice(g->device));
+    if (pending) return fail(MK_ERR_STATE, "internal: a deferred call was not settled");
     // a deferred mutation of the graph is ordered before this call's
     // kernels on the device; its outcome is settled in end()
     if (g->pend.active && g->pend.s != stream) MK_HIP(hipStreamWaitEvent(stream, g->mut_ev, 0));
@@ -1705,11 +1717,30 @@ struct Traversal {
     return MK_OK;
   }
 
-  int end(float *ms) {
-    // the stop event precedes the counter copy; one stream sync covers both
-    MK_HIP(hipEventRecord(ev1, stream));
+  // End of a call: its counters (and outcome) now, or, for a deferred
+  // incremental/decremental call (`may_defer`), once the copy is queued:
+  // settle() reads them at the handle's next call, read or sync.
+  int end(float *ms, bool may_defer = false) {
+    if (int r = copy_counts_then_stop()) return r;
+    if (may_defer && defer) {
+      int lr = MK_OK;
+      if (g->pend.active) {
+        // the mutation this call ran on (queued before it) is settled now:
+        // the host waits for it while the device runs this call
+        lr = graph_settle(g);
+        if (lr) stale = true;
+      }
+      MK_HIP(hipEventRecord(g->trav_ev, stream));
+      g->trav_pending = true;
+      clear_counts();  // after the copy on the stream
+      dirty = false;
+      pending = true;
+      if (ms) *ms = 0.f;  // not measured: the call has not run yet
+      return lr;
+    }
+    MK_HIP(hipStreamSynchronize(stream));
     Totals h;
-    if (int r = read_counts(h)) return r;
+    parse(*host.as<Counts>(), h);
     int lr = finish_loops(h);
     if (lr) stale = true;  // a loop gave up: the results are incomplete
     if (!lr && profiling_on()) record_steps();
@@ -1734,7 +1765,47 @@ struct Traversal {
     return MK_OK;
   }
 
+  // Read a deferred call's counters (waiting for it) and report its
+  // outcome once: a loop that gave up leaves the handle stale; a relax loop
+  // that ran out of its steps cannot be continued in place (later work on
+  // the graph may already have run), so the results are recomputed from
+  // scratch, which is exact for the graph as it is now.
+  int settle() {
+    if (!pending) return MK_OK;
+    pending = false;
+    MK_HIP(hipEventSynchronize(ev1));
+    Totals h;
+    parse(*host.as<Counts>(), h);
+    const bool ran_out = relax_end && h.loop_end[0] == relax_end;
+    const int lr = ran_out ? MK_OK : finish_loops(h);  // no relaunch unless ran_out
+    if (!lr && !ran_out && profiling_on()) record_steps();
+    relax_end = 0;
+    inval_queued = false;
+    reseed_kind = 0;
+    if (lr) {
+      stale = true;
+      return lr;
+    }
+    last_slabs = h.slabs;
+    last_edges = h.edges;
+    last_iters = h.steps;
+    if (ran_out) return run_static(nullptr);
+    return MK_OK;
+  }
+
+  int set_deferred(bool on) {
+    if (!on) {
+      if (int r = settle()) {
+        defer = false;
+        return r;
+      }
+    }
+    defer = on;
+    return MK_OK;
+  }
+
   int run_static(float *ms) {
+    if (int r = settle()) return r;
     if (int r = begin()) return r;
     stale = false;
     fill_u64(dist, kInfInf, g->n, stream);
@@ -1751,6 +1822,7 @@ struct Traversal {
   int run_incremental(const uint32_t *src, const uint32_t *dst, const uint32_t *w, uint32_t n,
                       float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "incremental: NULL batch");
+    if (int r = settle()) return r;
     if (stale) return fail(MK_ERR_STATE, "incremental: results are invalid after a failed batch; "
                                          "run Static() first");
     if (int r = begin()) return r;
@@ -1776,7 +1848,7 @@ struct Traversal {
         if (int r = iterate<0>(h)) return r;
       }
     }
-    return end(ms);
+    return end(ms, true);
   }
 
   // Decremental after the batch seeds, persistent loops: the invalidation
@@ -1810,11 +1882,12 @@ struct Traversal {
     }
     MK_HIP(hipGetLastError());
     if (int r = queue_relax(invbits, nwords)) return r;
-    return end(ms);
+    return end(ms, true);
   }
 
   int run_decremental(const uint32_t *src, const uint32_t *dst, uint32_t n, float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "decremental: NULL batch");
+    if (int r = settle()) return r;
     if (stale) return fail(MK_ERR_STATE, "decremental: results are invalid after a failed batch; "
                                          "run Static() first");
     if (int r = begin()) return r;
@@ -1903,11 +1976,12 @@ struct Traversal {
         if (int r = iterate<0>(h)) return r;
       }
     }
-    return end(ms);
+    return end(ms, true);
   }
 
   int read(uint32_t *d, uint32_t *p) {
     MK_HIP(hipSetDevice(g->device));
+    if (int r = settle()) return r;
     const size_t n = g->n;
     DevBuf b;
     if (int r = b.ensure(n * 8)) return r;
@@ -1982,12 +2056,14 @@ int mk_bfs_decremental(mk_bfs *b, const uint32_t *src, const uint32_t *dst, uint
 int mk_bfs_set_source(mk_bfs *b, uint32_t source) {
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_set_source: NULL handle");
   if (source >= b->t.g->n) return fail(MK_ERR_RANGE, "mk_bfs_set_source: source >= n");
+  if (int r = b->t.settle()) return r;
   b->t.source = source;
   return MK_OK;
 }
 int mk_sssp_set_source(mk_sssp *s, uint32_t source) {
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_set_source: NULL handle");
   if (source >= s->t.g->n) return fail(MK_ERR_RANGE, "mk_sssp_set_source: source >= n");
+  if (int r = s->t.settle()) return r;
   s->t.source = source;
   return MK_OK;
 }
@@ -1997,6 +2073,7 @@ int mk_bfs_read(mk_bfs *b, uint32_t *d, uint32_t *p) {
 }
 int mk_bfs_last_stats(mk_bfs *b, uint32_t *iters, uint64_t *slabs, uint64_t *edges) {
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_last_stats: NULL handle");
+  if (int r = b->t.settle()) return r;
   if (iters) *iters = b->t.last_iters;
   if (slabs) *slabs = b->t.last_slabs;
   if (edges) *edges = b->t.last_edges;
@@ -2028,10 +2105,28 @@ int mk_sssp_read(mk_sssp *s, uint32_t *d, uint32_t *p) {
 }
 int mk_sssp_last_stats(mk_sssp *s, uint32_t *iters, uint64_t *slabs, uint64_t *edges) {
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_last_stats: NULL handle");
+  if (int r = s->t.settle()) return r;
   if (iters) *iters = s->t.last_iters;
   if (slabs) *slabs = s->t.last_slabs;
   if (edges) *edges = s->t.last_edges;
   return MK_OK;
+}
+
+int mk_bfs_set_deferred(mk_bfs *b, int on) {
+  if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_set_deferred: NULL handle");
+  return b->t.set_deferred(on != 0);
+}
+int mk_bfs_sync(mk_bfs *b) {
+  if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_sync: NULL handle");
+  return b->t.settle();
+}
+int mk_sssp_set_deferred(mk_sssp *s, int on) {
+  if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_set_deferred: NULL handle");
+  return s->t.set_deferred(on != 0);
+}
+int mk_sssp_sync(mk_sssp *s) {
+  if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_sync: NULL handle");
+  return s->t.settle();
 }
 
 }  // extern "C"

@@ -90,3 +90,103 @@ def test_deferred_pool_failure_surfaces_once_and_rolls_back(gpu):
     g.InsertEdges(np.array([9], np.uint32), np.array([10], np.uint32))
     g.sync()
     assert g.num_edges() == 501
+
+
+# ---------------------------------------------------------------- deferred
+# BFS/SSSP calls (set_deferred on the handle): Incremental / Decremental
+# return once queued; the handle's next call, read or sync settles them.
+
+@pytest.mark.parametrize("mirror", [False, True])
+def test_deferred_calls_run_ahead_and_stay_exact(gpu, mirror):
+    n, s, d, w = gen.rmat(14, 8, seed=63, weighted=True)
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=mirror,
+                        in_degree_hints=gen.degree_hints(n, d) if mirror else None)
+    g.InsertEdges(s, d, w)
+    g.set_deferred(True)
+    sp, bf = mk.SSSP(g, 0), mk.BFS(g, 0)
+    sp.Static()
+    bf.Static()
+    sp.set_deferred(True)
+    bf.set_deferred(True)
+    keys, wt = gen.keys_of(s, d), w.copy()
+    for i in range(6):
+        ds, dd = gen.delete_batch(keys, 1500, seed=1700 + i)
+        g.DeleteEdges(mk.device_u32(ds), mk.device_u32(dd))
+        assert sp.Decremental(ds, dd, None) == 0.0  # queued, not measured
+        assert bf.Decremental(ds, dd) == 0.0
+        keep = ~np.isin(keys, gen.keys_of(ds, dd))
+        keys, wt = keys[keep], wt[keep]
+        bs, bd, bw = gen.insert_batch(keys, n, 1500, seed=1800 + i, weighted=True,
+                                      wseed=1900 + i)
+        g.InsertEdges(bs, bd, bw)
+        sp.Incremental(bs, bd, bw)
+        bf.Incremental(bs, bd)
+        keys, wt = np.r_[keys, gen.keys_of(bs, bd)], np.r_[wt, bw]
+        order = np.argsort(keys)
+        keys, wt = keys[order], wt[order]
+        if i % 3 == 2:  # a few rounds queued back to back, then a read settles
+            s2, d2 = gen.split_keys(keys)
+            check(sp, n, s2, d2, wt)
+            check(bf, n, s2, d2)
+            assert sp.last_stats()["edges_read"] > 0
+    sp.sync()
+    sp.set_deferred(False)
+    assert sp.Incremental(np.zeros(0, np.uint32), np.zeros(0, np.uint32)) >= 0.0
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_deferred_call_past_one_launch_is_recomputed(gpu, weighted):
+    # a 1000-vertex path: after the reconnecting insert the relax runs ~480
+    # steps, more than one loop launch reserves; a deferred call cannot be
+    # continued in place, so settling it recomputes from scratch
+    n = 1000
+    s = np.arange(n - 1, dtype=np.uint32)
+    d = s + 1
+    w = (np.arange(n - 1, dtype=np.uint32) % 7 + 1) if weighted else None
+    pool = mk.SlabPool(1 << 22)
+    g = mk.DynamicGraph(n, pool, 0.7, None, weighted=weighted)
+    g.InsertEdges(s, d, w)
+    algo = mk.SSSP(g, 0) if weighted else mk.BFS(g, 0)
+    algo.Static()
+    algo.set_deferred(True)
+    cs, cd = np.array([500], np.uint32), np.array([501], np.uint32)
+    g.DeleteEdges(cs, cd)
+    algo.Decremental(cs, cd) if not weighted else algo.Decremental(cs, cd, None)
+    keep = ~((s == 500) & (d == 501))
+    s1, d1 = s[keep], d[keep]
+    w1 = w[keep] if weighted else None
+    bs, bd = np.array([499, 10], np.uint32), np.array([700, 520], np.uint32)
+    bw = np.array([3, 5], np.uint32) if weighted else None
+    g.InsertEdges(bs, bd, bw)
+    if weighted:
+        algo.Incremental(bs, bd, bw)
+    else:
+        algo.Incremental(bs, bd)
+    s2, d2 = np.r_[s1, bs], np.r_[d1, bd]
+    w2 = np.r_[w1, bw] if weighted else None
+    check(algo, n, s2, d2, w2)  # the read settles (and recomputes)
+
+
+def test_deferred_call_on_a_failed_batch_reports_it(gpu):
+    # the mutation a deferred call runs on is settled when the call is
+    # queued: its failure is the call's error, and the handle stays invalid
+    # until Static
+    n, s, d, _ = gen.rmat(12, 8, seed=64)
+    pool = mk.SlabPool(1 << 26)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s))
+    g.InsertEdges(s, d)
+    g.set_deferred(True)
+    b = mk.BFS(g, 0)
+    b.Static()
+    b.set_deferred(True)
+    bad_s, bad_d = np.array([0, 1, n + 3], np.uint32), np.array([5, 6, 7], np.uint32)
+    g.InsertEdges(bad_s, bad_d)
+    with pytest.raises(mk.MeerkatError) as ei:
+        b.Incremental(bad_s, bad_d)
+    assert ei.value.code == _lib.MK_ERR_RANGE
+    with pytest.raises(mk.MeerkatError) as ei:
+        b.Incremental(np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    assert ei.value.code == _lib.MK_ERR_STATE
+    b.Static()
+    check(b, n, s, d)
