@@ -83,16 +83,18 @@ void DevBuf::release() {
 int HostBuf::ensure(size_t need) {
   if (need <= bytes && ptr) return MK_OK;
   if (ptr) (void)hipHostFree(ptr);
-  ptr = nullptr;
+  ptr = dev = nullptr;
   bytes = 0;
   size_t want = std::max<size_t>(need, 256);
   MK_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+  std::memset(ptr, 0, want);
+  MK_HIP(hipHostGetDevicePointer(&dev, ptr, 0));
   bytes = want;
   return MK_OK;
 }
 void HostBuf::release() {
   if (ptr) (void)hipHostFree(ptr);
-  ptr = nullptr;
+  ptr = dev = nullptr;
   bytes = 0;
 }
 
@@ -114,6 +116,17 @@ void fill_u64(uint64_t *p, uint64_t v, uint64_t count, hipStream_t s) {
 }
 void fill_u8(uint8_t *p, uint8_t v, uint64_t count, hipStream_t s) {
   if (count) k_fill<uint8_t><<<fill_blocks(count), kBlock, 0, s>>>(p, v, count);
+}
+
+// Device words copied straight into pinned host memory (`dst`: its device
+// alias), then made visible to the host (system-scope fence) before the
+// kernel ends.
+__global__ void k_publish_words(const uint32_t *src, uint32_t *dst, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  __threadfence_system();
+}
+void publish_words(const uint32_t *src, uint32_t *dst_host_dev, uint32_t n, hipStream_t s) {
+  k_publish_words<<<1, 64, 0, s>>>(src, dst_host_dev, n);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -828,7 +841,7 @@ extern "C" {
 static int mutation_finish(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
                            const uint32_t *w, uint32_t n, hipStream_t s) {
   MK_HIP(hipGetLastError());
-  MK_HIP(hipMemcpyAsync(g->host_scratch.ptr, g->counters, 16, hipMemcpyDeviceToHost, s));
+  publish_words(g->counters, static_cast<uint32_t *>(g->host_scratch.dev), 4, s);
   g->last_op = op;
   g->last_src = src;
   g->last_dst = dst;

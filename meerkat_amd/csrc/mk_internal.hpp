@@ -246,6 +246,7 @@ struct DevBuf {
 // Pinned host scratch for small device->host reads.
 struct HostBuf {
   void *ptr = nullptr;
+  void *dev = nullptr;  // the same memory as kernels address it
   size_t bytes = 0;
   int ensure(size_t need);
   void release();
@@ -331,4 +332,9 @@ inline unsigned blocks_for_waves(uint64_t waves) { return blocks_for_threads(wav
 void fill_u32(uint32_t *p, uint32_t v, uint64_t count, hipStream_t s);
 void fill_u64(uint64_t *p, uint64_t v, uint64_t count, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t v, uint64_t count, hipStream_t s);
+// Copy n device words into pinned host memory by a kernel (`dst_host_dev`:
+// HostBuf::dev), readable on the host once the stream has passed it.  A
+// hipMemcpyAsync to pinned memory runs as a copy kernel that was measured
+// to leave ~10 us of device idle before and after it.
+void publish_words(const uint32_t *src, uint32_t *dst_host_dev, uint32_t n, hipStream_t s);
 }  // namespace mk

@@ -81,6 +81,30 @@ struct Counts {
   unsigned long long loop_t[2][2];
 };
 
+// The counters a call's host side reads, written by the device straight
+// into pinned host memory (`out`: HostBuf::dev of the host copy): the
+// fields before the statistics, the statistics summed over their slots
+// (into slot 0; the host copy's other slots stay zero), and the step trace
+// while tracing; then a system-scope fence.  One block.
+__global__ void __launch_bounds__(256) k_publish_counts(const Counts *c, Counts *out,
+                                                        uint32_t trace) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(c);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(out);
+  constexpr uint32_t head = offsetof(Counts, stats) / 4;
+  for (uint32_t i = t; i < head; i += blockDim.x) dst[i] = src[i];
+  if (t < 16) {
+    unsigned long long x = 0;
+    for (uint32_t k = 0; k < kStatSlots; ++k) x += c->stats[k][t];
+    out->stats[0][t] = x;
+  }
+  if (trace) {
+    constexpr uint32_t t0 = offsetof(Counts, trace_t) / 4, t1 = sizeof(Counts) / 4;
+    for (uint32_t i = t0 + t; i < t1; i += blockDim.x) dst[i] = src[i];
+  }
+  __threadfence_system();
+}
+
 // Host-side totals of a Counts snapshot.
 struct Totals {
   uint32_t small[3];
@@ -1502,17 +1526,22 @@ struct Traversal {
     fill_u64(reinterpret_cast<uint64_t *>(cnt->large), 0ull, 3, stream);
   }
 
+  // Queue the counters' copy into the pinned host copy (k_publish_counts).
+  void publish_counts() {
+    k_publish_counts<<<1, 256, 0, stream>>>(cnt, static_cast<Counts *>(host.dev),
+                                            profiling_on() ? 1u : 0u);
+  }
   int read_counts(Totals &h) {
-    MK_HIP(hipMemcpyAsync(host.ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
+    publish_counts();
     MK_HIP(hipStreamSynchronize(stream));
     parse(*host.as<Counts>(), h);
     return MK_OK;
   }
-  // The counter copy, then the stop event: an event recorded between the
-  // last kernel and the copy was measured to delay the copy's start by
-  // ~11 us.  The caller waits (stream or ev1) before parsing `host`.
+  // The counter copy, then the stop event (an event recorded between the
+  // last kernel and a hipMemcpyAsync was measured to delay the copy by
+  // ~11 us).  The caller waits (stream or ev1) before parsing `host`.
   int copy_counts_then_stop() {
-    MK_HIP(hipMemcpyAsync(host.ptr, cnt, sizeof(Counts), hipMemcpyDeviceToHost, stream));
+    publish_counts();
     MK_HIP(hipEventRecord(ev1, stream));
     return MK_OK;
   }
