@@ -119,14 +119,29 @@ void fill_u8(uint8_t *p, uint8_t v, uint64_t count, hipStream_t s) {
 }
 
 // Device words copied straight into pinned host memory (`dst`: its device
-// alias), then made visible to the host (system-scope fence) before the
-// kernel ends.
-__global__ void k_publish_words(const uint32_t *src, uint32_t *dst, uint32_t n) {
+// alias) and made visible to the host (system-scope fence); then `seq` in
+// dst[n], so a host that reads seq there also reads the words.
+__global__ void k_publish_words(const uint32_t *src, uint32_t *dst, uint32_t n, uint32_t seq) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
   __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    dst[n] = seq;
+    __threadfence_system();
+  }
 }
-void publish_words(const uint32_t *src, uint32_t *dst_host_dev, uint32_t n, hipStream_t s) {
-  k_publish_words<<<1, 64, 0, s>>>(src, dst_host_dev, n);
+void publish_words(const uint32_t *src, uint32_t *dst_host_dev, uint32_t n, uint32_t seq,
+                   hipStream_t s) {
+  k_publish_words<<<1, 64, 0, s>>>(src, dst_host_dev, n, seq);
+}
+int host_wait_word(const volatile uint32_t *word, uint32_t want, hipStream_t s) {
+  for (uint32_t i = 0; i < (1u << 22); ++i) {  // tens of ms
+    if (*word == want) return MK_OK;
+    __builtin_ia32_pause();
+  }
+  MK_HIP(hipStreamSynchronize(s));  // a longer call: block instead
+  if (*word == want) return MK_OK;
+  return fail(MK_ERR_HIP, "a counter readback did not arrive");
 }
 
 // ---------------------------------------------------------------- kernels
@@ -827,7 +842,8 @@ int graph_settle(mk_graph *g) {
   if (g && g->mirror_of) g = g->mirror_of;  // a mirror changes with its graph
   if (!g || !g->pend.active) return MK_OK;
   g->pend.active = false;
-  MK_HIP(hipEventSynchronize(g->mut_ev));
+  if (int r = host_wait_word(g->host_scratch.as<uint32_t>() + 4, g->pend.seq, g->pend.s))
+    return r;
   const int r = mutation_outcome(g, g->pend.op, g->pend.src, g->pend.dst, g->pend.n, g->pend.s);
   if (r) g->last_op = 0;  // the batch's statuses no longer describe the graph
   return r;
@@ -841,14 +857,16 @@ extern "C" {
 static int mutation_finish(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
                            const uint32_t *w, uint32_t n, hipStream_t s) {
   MK_HIP(hipGetLastError());
-  publish_words(g->counters, static_cast<uint32_t *>(g->host_scratch.dev), 4, s);
+  publish_words(g->counters, static_cast<uint32_t *>(g->host_scratch.dev), 4, g->mut_epoch, s);
   g->last_op = op;
   g->last_src = src;
   g->last_dst = dst;
   g->last_w = op == 1 ? w : nullptr;
   if (g->deferred) {
-    MK_HIP(hipEventRecord(g->mut_ev, s));
-    g->pend = {true, op, src, dst, n, s};
+    // BFS/SSSP calls run on own_stream: only a mutation on another stream
+    // needs an event for them to wait on
+    if (s != g->own_stream) MK_HIP(hipEventRecord(g->mut_ev, s));
+    g->pend = {true, op, src, dst, n, s, g->mut_epoch};
     return MK_OK;
   }
   MK_HIP(hipStreamSynchronize(s));
@@ -863,8 +881,10 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_insert: mirror is read-only");
   if (int r = graph_settle(g)) return r;
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_insert")) return r;
-  // a deferred BFS/SSSP call still reading the graph on its own stream
-  if (n && g->trav_pending && s != g->own_stream) MK_HIP(hipStreamWaitEvent(s, g->trav_ev, 0));
+  if (n && s != g->own_stream) {  // after the work queued on own_stream (deferred calls)
+    MK_HIP(hipEventRecord(g->trav_ev, g->own_stream));
+    MK_HIP(hipStreamWaitEvent(s, g->trav_ev, 0));
+  }
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n)) return r;
   launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
@@ -877,8 +897,10 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   if (g && g->mirror_of) return fail(MK_ERR_STATE, "mk_graph_delete: mirror is read-only");
   if (int r = graph_settle(g)) return r;
   if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_delete")) return r;
-  // a deferred BFS/SSSP call still reading the graph on its own stream
-  if (n && g->trav_pending && s != g->own_stream) MK_HIP(hipStreamWaitEvent(s, g->trav_ev, 0));
+  if (n && s != g->own_stream) {  // after the work queued on own_stream (deferred calls)
+    MK_HIP(hipEventRecord(g->trav_ev, g->own_stream));
+    MK_HIP(hipStreamWaitEvent(s, g->trav_ev, 0));
+  }
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n)) return r;
   launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);

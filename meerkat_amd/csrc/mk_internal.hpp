@@ -279,18 +279,21 @@ struct mk_graph {
   // deferred mode (mk_graph_set_deferred): a mutation call returns once
   // queued; `pend` describes it until the next call settles it
   bool deferred = false;
-  hipEvent_t mut_ev = nullptr;       // recorded after the queued mutation
-  // recorded (on own_stream) after each deferred BFS/SSSP call of the
-  // graph's handles; a mutation on another stream waits for it
+  // recorded after a queued mutation given another stream than own_stream
+  // (BFS/SSSP calls run on own_stream and wait for it there)
+  hipEvent_t mut_ev = nullptr;
+  // recorded on own_stream when a mutation is given another stream, which
+  // then waits for it: work queued on own_stream (deferred BFS/SSSP calls
+  // included) comes first
   hipEvent_t trav_ev = nullptr;
-  bool trav_pending = false;
   struct Pending {
     bool active;
     int op;  // 1 insert, 2 delete
     const uint32_t *src, *dst;
     uint32_t n;
     hipStream_t s;
-  } pend = {false, 0, nullptr, nullptr, 0, nullptr};
+    uint32_t seq;  // written after its counters in host_scratch (word 4)
+  } pend = {false, 0, nullptr, nullptr, 0, nullptr, 0};
   std::vector<uint32_t> bucket_off_host;
   mk::DevBuf status;             // uint8 per edge of the last batch
   uint32_t status_n = 0;
@@ -333,8 +336,14 @@ void fill_u32(uint32_t *p, uint32_t v, uint64_t count, hipStream_t s);
 void fill_u64(uint64_t *p, uint64_t v, uint64_t count, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t v, uint64_t count, hipStream_t s);
 // Copy n device words into pinned host memory by a kernel (`dst_host_dev`:
-// HostBuf::dev), readable on the host once the stream has passed it.  A
-// hipMemcpyAsync to pinned memory runs as a copy kernel that was measured
-// to leave ~10 us of device idle before and after it.
-void publish_words(const uint32_t *src, uint32_t *dst_host_dev, uint32_t n, hipStream_t s);
+// HostBuf::dev), then `seq` into word n, after the others are visible.  A
+// hipMemcpyAsync to pinned memory runs as a copy kernel, and an event record
+// between kernels, each measured to leave ~10 us of device idle; the host
+// learns of the kernel's completion from the sequence word instead
+// (host_wait_word).
+void publish_words(const uint32_t *src, uint32_t *dst_host_dev, uint32_t n, uint32_t seq,
+                   hipStream_t s);
+// Wait until the pinned word `*word` equals `want` (written by a publish
+// kernel queued on `s`): spin briefly, then synchronise the stream.
+int host_wait_word(const volatile uint32_t *word, uint32_t want, hipStream_t s);
 }  // namespace mk

@@ -85,9 +85,10 @@ struct Counts {
 // into pinned host memory (`out`: HostBuf::dev of the host copy): the
 // fields before the statistics, the statistics summed over their slots
 // (into slot 0; the host copy's other slots stay zero), and the step trace
-// while tracing; then a system-scope fence.  One block.
+// while tracing; then, once those are visible, `seq` in the word after the
+// copy (the host waits for it: host_wait_word).  One block.
 __global__ void __launch_bounds__(256) k_publish_counts(const Counts *c, Counts *out,
-                                                        uint32_t trace) {
+                                                        uint32_t trace, uint32_t seq) {
   const uint32_t t = threadIdx.x;
   const uint32_t *src = reinterpret_cast<const uint32_t *>(c);
   uint32_t *dst = reinterpret_cast<uint32_t *>(out);
@@ -103,6 +104,11 @@ __global__ void __launch_bounds__(256) k_publish_counts(const Counts *c, Counts 
     for (uint32_t i = t0 + t; i < t1; i += blockDim.x) dst[i] = src[i];
   }
   __threadfence_system();
+  __syncthreads();
+  if (t == 0) {  // the word after the copy: seen last by the host
+    reinterpret_cast<uint32_t *>(out + 1)[0] = seq;
+    __threadfence_system();
+  }
 }
 
 // Host-side totals of a Counts snapshot.
@@ -1323,6 +1329,8 @@ struct Traversal {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool defer = false;    // incremental/decremental return once queued (settle() reads them)
   bool pending = false;  // a deferred call's counters are on their way to `host`
+  uint32_t pub_seq = 0;  // sequence number of the last counter copy
+  uint32_t pend_seq = 0; // the pending call's
   uint64_t *dist = nullptr;
   uint32_t *stamp = nullptr;
   uint32_t *invbits = nullptr;
@@ -1389,7 +1397,7 @@ struct Traversal {
     }
     MK_HIP(hipMalloc(&cnt, bar_offset() + sizeof(BarLines)));
     bar = reinterpret_cast<BarLines *>(reinterpret_cast<char *>(cnt) + bar_offset());
-    if (int r = host.ensure(sizeof(Counts))) return r;
+    if (int r = host.ensure(sizeof(Counts) + 64)) return r;  // + the sequence word
     for (int i = 0; i < 2; ++i) {
       if (int r = snap[i].ensure(sizeof(Counts))) return r;
       MK_HIP(hipEventCreateWithFlags(&snap_ev[i], hipEventDisableTiming));
@@ -1526,10 +1534,16 @@ struct Traversal {
     fill_u64(reinterpret_cast<uint64_t *>(cnt->large), 0ull, 3, stream);
   }
 
-  // Queue the counters' copy into the pinned host copy (k_publish_counts).
-  void publish_counts() {
+  // Queue the counters' copy into the pinned host copy (k_publish_counts);
+  // returns the sequence number it ends with.
+  uint32_t publish_counts() {
+    if (++pub_seq == 0) pub_seq = 1;  // the word starts at 0
     k_publish_counts<<<1, 256, 0, stream>>>(cnt, static_cast<Counts *>(host.dev),
-                                            profiling_on() ? 1u : 0u);
+                                            profiling_on() ? 1u : 0u, pub_seq);
+    return pub_seq;
+  }
+  const volatile uint32_t *pub_word() const {
+    return reinterpret_cast<const volatile uint32_t *>(host.as<Counts>() + 1);
   }
   int read_counts(Totals &h) {
     publish_counts();
@@ -1731,7 +1745,9 @@ struct Traversal {
   // Counters are zero between calls: end() queues their clearing after it
   // read them (off the next call's path); a call that stopped early leaves
   // them `dirty` for the next begin() to clear.
-  int begin() {
+  // `timed`: the call reports its time (ev0 .. ev1); an event record between
+  // kernels was measured to leave ~10 us of device idle.
+  int begin(bool timed = true) {
     MK_HIP(hipSetDevice(g->device));
     if (pending) return fail(MK_ERR_STATE, "internal: a deferred call was not settled");
     // a deferred mutation of the graph is ordered before this call's
@@ -1742,7 +1758,7 @@ struct Traversal {
     reseed_kind = 0;
     if (dirty) clear_counts();
     dirty = true;
-    MK_HIP(hipEventRecord(ev0, stream));
+    if (timed) MK_HIP(hipEventRecord(ev0, stream));
     return MK_OK;
   }
 
@@ -1750,8 +1766,8 @@ struct Traversal {
   // incremental/decremental call (`may_defer`), once the copy is queued:
   // settle() reads them at the handle's next call, read or sync.
   int end(float *ms, bool may_defer = false) {
-    if (int r = copy_counts_then_stop()) return r;
     if (may_defer && defer) {
+      pend_seq = publish_counts();  // no events: the host waits for pend_seq
       int lr = MK_OK;
       if (g->pend.active) {
         // the mutation this call ran on (queued before it) is settled now:
@@ -1759,14 +1775,13 @@ struct Traversal {
         lr = graph_settle(g);
         if (lr) stale = true;
       }
-      MK_HIP(hipEventRecord(g->trav_ev, stream));
-      g->trav_pending = true;
       clear_counts();  // after the copy on the stream
       dirty = false;
       pending = true;
       if (ms) *ms = 0.f;  // not measured: the call has not run yet
       return lr;
     }
+    if (int r = copy_counts_then_stop()) return r;
     MK_HIP(hipStreamSynchronize(stream));
     Totals h;
     parse(*host.as<Counts>(), h);
@@ -1802,7 +1817,10 @@ struct Traversal {
   int settle() {
     if (!pending) return MK_OK;
     pending = false;
-    MK_HIP(hipEventSynchronize(ev1));
+    if (int r = host_wait_word(pub_word(), pend_seq, stream)) {
+      stale = true;
+      return r;
+    }
     Totals h;
     parse(*host.as<Counts>(), h);
     const bool ran_out = relax_end && h.loop_end[0] == relax_end;
@@ -1854,7 +1872,7 @@ struct Traversal {
     if (int r = settle()) return r;
     if (stale) return fail(MK_ERR_STATE, "incremental: results are invalid after a failed batch; "
                                          "run Static() first");
-    if (int r = begin()) return r;
+    if (int r = begin(!defer)) return r;
     if (n) {
       const uint32_t ep = next_epoch();
       const GraphView G = g->view();
@@ -1919,7 +1937,7 @@ struct Traversal {
     if (int r = settle()) return r;
     if (stale) return fail(MK_ERR_STATE, "decremental: results are invalid after a failed batch; "
                                          "run Static() first");
-    if (int r = begin()) return r;
+    if (int r = begin(!defer)) return r;
     if (n) {
       const GraphView G = g->view();
       {
