@@ -54,6 +54,10 @@ struct BFS {
   std::vector<uint32_t> Distances() { return read(true); }
   std::vector<uint32_t> Parents() { return read(false); }
   void SetSource(uint32_t source) { check(mk_bfs_set_source(h_, source), "mk_bfs_set_source"); }
+  // Deferred calls (mk_bfs_set_deferred): Incremental / Decremental return 0
+  // once queued; the next call on the handle, or Sync(), settles them.
+  void SetDeferred(bool on) { check(mk_bfs_set_deferred(h_, on ? 1 : 0), "mk_bfs_set_deferred"); }
+  void Sync() { check(mk_bfs_sync(h_), "mk_bfs_sync"); }
 
  private:
   std::vector<uint32_t> read(bool dist) {
@@ -104,6 +108,11 @@ struct SSSP {
   std::vector<uint32_t> Distances() { return read(true); }
   std::vector<uint32_t> Parents() { return read(false); }
   void SetSource(uint32_t source) { check(mk_sssp_set_source(h_, source), "mk_sssp_set_source"); }
+  // As BFS::SetDeferred / BFS::Sync.
+  void SetDeferred(bool on) {
+    check(mk_sssp_set_deferred(h_, on ? 1 : 0), "mk_sssp_set_deferred");
+  }
+  void Sync() { check(mk_sssp_sync(h_), "mk_sssp_sync"); }
 
  private:
   std::vector<uint32_t> read(bool dist) {
