@@ -1551,9 +1551,8 @@ struct Traversal {
     parse(*host.as<Counts>(), h);
     return MK_OK;
   }
-  // The counter copy, then the stop event (an event recorded between the
-  // last kernel and a hipMemcpyAsync was measured to delay the copy by
-  // ~11 us).  The caller waits (stream or ev1) before parsing `host`.
+  // The counter copy, then the stop event of a timed call.  The caller
+  // synchronises the stream before parsing `host`.
   int copy_counts_then_stop() {
     publish_counts();
     MK_HIP(hipEventRecord(ev1, stream));
