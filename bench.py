@@ -10,14 +10,17 @@ vertex 0, i.e. the batch is applied and single-source shortest paths are kept
 exact.  Inputs are staged in HBM before the timed region.  The graph defers
 each batch's range/pool check to the SSSP call that follows it (the call
 fails, and the graph is restored, if the batch failed); --sync-mutations
-makes every mutation call wait for its own outcome instead.
+makes every mutation call wait for its own outcome instead.  The SSSP calls
+are deferred too: each returns once queued and is settled by the next (the
+timed region ends once the last one is settled); --sync-calls makes each
+wait for its own outcome.
 
 value = edge updates per second over the timed steps, shortest paths
 maintained (B * K / seconds).  Also reported: update-only throughput,
 incremental/decremental SSSP and incremental BFS traversed-edge rates,
 per-step latency percentiles, the roofline of the dominant kernel (timing
-inside the library over the timed region: dispatch events, or the device
-clock for the cooperative frontier loops), a CPU baseline (the
+inside the library over the timed region: the device clock of the persistent
+frontier loops), a CPU baseline (the
 oracle's multithreaded static SSSP recompute on the same graph, all host
 threads) and a
 serving-loop phase (continuous batching vs one request at a time on the same
@@ -25,6 +28,7 @@ request mix; query latency percentiles under a Poisson load).
 
   python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--scale 24]
                   [--batch 1000000] [--no-cpu-baseline] [--sync-mutations]
+                  [--sync-calls]
 """
 import argparse
 import json
