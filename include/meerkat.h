@@ -21,10 +21,12 @@
  *    (empty / tombstone slab words).
  *  - `stream` arguments are hipStream_t values passed as void*.  Graph
  *    mutations are ordered on that stream and complete before the call
- *    returns; NULL selects the graph's own stream, which (created with
- *    default flags) still orders after work queued on the null stream.  Algorithm compute calls block until done and
- *    report elapsed milliseconds (HIP events), like the reference's float
- *    returns (include/Algorithms/impl/BFSImpl.cuh:644-661).
+ *    returns (unless deferred: mk_graph_set_deferred); NULL selects the
+ *    graph's own stream, which (created with default flags) still orders
+ *    after work queued on the null stream.  Algorithm compute calls block
+ *    until done and report elapsed milliseconds (HIP events), like the
+ *    reference's float returns (include/Algorithms/impl/BFSImpl.cuh:644-661),
+ *    except deferred BFS/SSSP calls (mk_bfs_set_deferred).
  */
 #ifndef MEERKAT_H_
 #define MEERKAT_H_
