@@ -84,6 +84,29 @@ def measured_traffic(kernel, scale):
     return None
 
 
+def verify_sssp(n, s, d, w, del_sel, ins_s, ins_d, ins_w, dist, parent, source=0):
+    """The bench's SSSP handle after its last step against the oracle's
+    from-scratch recompute on the final edge set (initial edges less the
+    deleted ones, plus the inserted ones), outside every timed region:
+    distances bit-exact and every parent valid (dist[p] + w(p, v) = dist[v]
+    with (p, v) an edge)."""
+    import oracle
+
+    t0 = time.perf_counter()
+    keep = np.ones(len(s), bool)
+    keep[del_sel] = False
+    ref_g = oracle.csr(n, np.concatenate([s[keep], ins_s]), np.concatenate([d[keep], ins_d]),
+                       np.concatenate([w[keep], ins_w]))
+    ref = oracle.sssp_par(ref_g, source, oracle.num_threads())
+    mismatched = int(np.count_nonzero(dist != ref))
+    bad_parents = oracle.check_parents_fast(ref_g, dist, parent, source, weights=True)
+    out = {"verified": mismatched == 0 and bad_parents == 0, "edges": ref_g.m,
+           "reached": int(np.count_nonzero(ref != 0xFFFFFFFF)),
+           "mismatched_distances": mismatched, "invalid_parents": int(bad_parents),
+           "check_s": round(time.perf_counter() - t0, 1)}
+    return out
+
+
 def run_serving(args, n, g, s, d, del_idx, used, log):
     """Serving-loop phase on the resident graph: the same request mix served
     one request at a time (single-source static BFS / SSSP) and with
@@ -129,6 +152,7 @@ def run_serving(args, n, g, s, d, del_idx, used, log):
     el = time.perf_counter() - t0
     lat = serving.summarize(done, el)
     log(f"serving at {rate:.0f} req/s offered: {lat}")
+    loop.close()
     return {
         "workload": f"RMAT-{args.scale} resident; {R} requests per run: 30% update batches "
                     "(lognormal sizes, median 10,000, max 1,000,000 edges, 50/50 insert/delete), "
@@ -160,6 +184,8 @@ def main():
     ap.add_argument("--no-mirror", action="store_true",
                     help="no in-edge mirror: decremental re-seeds by scanning every edge")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop phase")
+    ap.add_argument("--no-verify", action="store_true",
+                    help="skip the check of the final SSSP distances against the oracle")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="leave the library's kernel timing off in the timed region "
@@ -296,13 +322,20 @@ def main():
     elapsed = time.perf_counter() - ts
     profiling.marker(dev)
     prof = profiling.totals()
-    # per-call rates: a few more steps with synchronous calls (each returns
-    # its own time and edges read), only the loops timed
+    deferred = sp.deferred_stats()
+    # the reference's protocol: a few more steps in which every mutation and
+    # SSSP call waits for its own outcome (each call returns its time and
+    # edges read; only the loops timed)
     sp.set_deferred(False)
+    g.set_deferred(False)
     calls = {k: [] for k in ("decr_ms", "incr_ms", "decr_edges", "incr_edges")}
     rate_end = timed_end + args.breakdown_steps
+    torch.cuda.synchronize()
+    tsync = time.perf_counter()
     for i in range(timed_end, rate_end):
         step(i, calls=calls)
+    settle()
+    sync_elapsed = time.perf_counter() - tsync
     # per-kernel table: a few more with every kernel timed
     profiling.reset()
     profiling.enable(True)
@@ -311,18 +344,26 @@ def main():
     settle()
     profiling.enable(False)
     prof_all = profiling.totals()
-    g.set_deferred(False)
     calibration_copy(dev)
+
+    # correctness of the maintained shortest paths after every step, against
+    # the oracle (host work, outside the timed regions)
+    verify = None
+    if not args.no_verify:
+        dist, parent = sp.read()
+        verify = verify_sssp(n, s, d, w, del_idx[:steps_total * half], ins_s, ins_d, ins_w,
+                             dist, parent)
+        log(f"verification against the oracle: {verify}")
 
     # incremental BFS rate on the updated graph (not part of the step)
     bfs = mk.BFS(g, 0)
     bfs.Static()
     bfs_rates = []
-    bs, bd, _ = gen.insert_batch(None, n, 3 * 100_000, seed=args.seed + 4)
+    bs, bd, bw = gen.insert_batch(None, n, 3 * 100_000, seed=args.seed + 4, weighted=True)
     for i in range(3):
-        bsi, bdi = mk.device_u32(bs[i * 100_000:(i + 1) * 100_000]), mk.device_u32(
-            bd[i * 100_000:(i + 1) * 100_000])
-        g.InsertEdges(bsi, bdi)
+        sl = slice(i * 100_000, (i + 1) * 100_000)
+        bsi, bdi, bwi = mk.device_u32(bs[sl]), mk.device_u32(bd[sl]), mk.device_u32(bw[sl])
+        g.InsertEdges(bsi, bdi, bwi)
         ms = bfs.Incremental(bsi, bdi)
         er = bfs.last_stats()["edges_read"]
         if ms > 0:
@@ -331,6 +372,12 @@ def main():
     serving_out = None
     if not args.no_serving:
         serving_out = run_serving(args, n, g, s, d, del_idx, steps_total * half, log)
+    # every handle closed here, in reverse order of creation: nothing is left
+    # to release at interpreter exit
+    bfs.close()
+    sp.close()
+    g.close()
+    pool.close()
 
     K = args.steps
     ms_per_step = elapsed / K * 1e3
@@ -407,6 +454,16 @@ def main():
         # deferred checks: the mutation kernels' own time; synchronous: the calls'
         "update_only_edges_per_s": round(args.batch * K / (upd.sum() if args.sync_mutations else
                                          mut_s), 1),
+        # the reference's protocol (every mutation and SSSP call waits for its
+        # own outcome), wall clock over the synchronous breakdown steps
+        "sync_protocol_ms_per_step": round(sync_elapsed / max(args.breakdown_steps, 1) * 1e3, 3),
+        "sync_protocol_edge_updates_per_s": round(args.batch * args.breakdown_steps /
+                                                  sync_elapsed, 1) if args.breakdown_steps else None,
+        # deferred SSSP calls of the warm-up and timed steps; a recomputed one
+        # ran out of reserved loop steps and was recomputed from scratch
+        "deferred_calls": deferred,
+        "verification": verify,
+        "knobs": autotune.knobs(),
         "sssp_incremental_gteps": round(incr_gteps, 3),
         "sssp_decremental_gteps": round(decr_gteps, 3),
         "bfs_incremental_gteps": round(float(np.median(bfs_rates)), 3) if bfs_rates else None,

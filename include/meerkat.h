@@ -127,8 +127,11 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_de
  * own work.  A BFS/SSSP compute call queued after the batch orders its
  * kernels after it without a host wait and settles it at its end; if the
  * batch failed, the call returns its error and the handle's results stay
- * invalid until its next Static.  The batch buffers and the stream must stay
- * valid until the batch is settled. */
+ * invalid until its next Static.  Any later algorithm call (on any handle
+ * of the graph) given a batch that failed, synchronous or deferred, returns
+ * that batch's error instead of running on edges the graph never applied,
+ * until the graph's next mutation.  The batch buffers and the stream must
+ * stay valid until the batch is settled. */
 int mk_graph_set_deferred(mk_graph *g, int on);
 int mk_graph_sync(mk_graph *g);
 
@@ -200,6 +203,10 @@ int mk_bfs_last_stats(mk_bfs *b, uint32_t *iterations, uint64_t *slabs_read,
  * stream waits for it on the device.  Static calls are never deferred. */
 int mk_bfs_set_deferred(mk_bfs *b, int on);
 int mk_bfs_sync(mk_bfs *b);
+/* Deferred calls settled since the handle was created, and how many of them
+ * were recomputed from scratch at settling (their relax loop needed more
+ * steps than one launch reserves). */
+int mk_bfs_deferred_stats(mk_bfs *b, uint64_t *settled, uint64_t *recomputed);
 
 /* ------------------------------------------------------ A11 SSSP
  * SSSP(G, Src, N) (include/Algorithms/SSSP.cuh:34-69; impl/SSSPImpl.cuh
@@ -219,9 +226,10 @@ int mk_sssp_read(mk_sssp *s, uint32_t *dist_host, uint32_t *parent_host);
 int mk_sssp_set_source(mk_sssp *s, uint32_t source);
 int mk_sssp_last_stats(mk_sssp *s, uint32_t *iterations, uint64_t *slabs_read,
                        uint64_t *edges_read);
-/* As mk_bfs_set_deferred / mk_bfs_sync. */
+/* As mk_bfs_set_deferred / mk_bfs_sync / mk_bfs_deferred_stats. */
 int mk_sssp_set_deferred(mk_sssp *s, int on);
 int mk_sssp_sync(mk_sssp *s);
+int mk_sssp_deferred_stats(mk_sssp *s, uint64_t *settled, uint64_t *recomputed);
 
 /* ------------------------------------------------------ A12 PageRank
  * PageRank(G, DampingFactor, ErrorMargin, N, VertexOutDegree, MaxIter)

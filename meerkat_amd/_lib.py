@@ -4,9 +4,11 @@ The shared library is built in-tree by ``__graft_entry__.build()`` (or
 ``make -C meerkat_amd/csrc``).  There is no fallback: if the library is
 missing or fails to load, every entry point raises.
 """
+import atexit
 import ctypes
 import os
 import re
+import sys
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmeerkat_hip.so")
@@ -42,6 +44,22 @@ class MeerkatError(RuntimeError):
 
 
 _lib = None
+_exiting = False
+
+
+@atexit.register
+def _at_exit():
+    global _exiting
+    _exiting = True
+
+
+def finalizer_ok():
+    """Whether a ``__del__`` may still call into the library.  Not once the
+    interpreter is exiting: the HIP runtime (and a profiler's preloaded
+    tool) tear down then, and a handle left open is released with the
+    process.  Scripts close their handles explicitly (``close()``)."""
+    return not _exiting and not sys.is_finalizing()
+
 
 vp = ctypes.c_void_p
 u32 = ctypes.c_uint32
@@ -87,6 +105,7 @@ _PROTOS = {
     "mk_bfs_last_stats": [vp, P(u32), P(u64), P(u64)],
     "mk_bfs_set_deferred": [vp, i32],
     "mk_bfs_sync": [vp],
+    "mk_bfs_deferred_stats": [vp, P(u64), P(u64)],
     "mk_sssp_create": [vp, u32, P(vp)],
     "mk_sssp_destroy": [vp],
     "mk_sssp_static": [vp, P(f32)],
@@ -97,6 +116,7 @@ _PROTOS = {
     "mk_sssp_last_stats": [vp, P(u32), P(u64), P(u64)],
     "mk_sssp_set_deferred": [vp, i32],
     "mk_sssp_sync": [vp],
+    "mk_sssp_deferred_stats": [vp, P(u64), P(u64)],
     "mk_pagerank_create": [vp, f32, f32, u32, vp, P(vp)],
     "mk_pagerank_destroy": [vp],
     "mk_pagerank_static": [vp, P(f32)],

@@ -61,6 +61,13 @@ class _Traversal:
         """Settle a deferred call: raises its MeerkatError if it failed."""
         _lib.call(f"{self._prefix}_sync", self._h)
 
+    def deferred_stats(self):
+        """Deferred calls settled since create, and how many were recomputed
+        from scratch at settling (their relax loop ran out of steps)."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.call(f"{self._prefix}_deferred_stats", self._h, ctypes.byref(a), ctypes.byref(b))
+        return {"settled": a.value, "recomputed": b.value}
+
     def last_stats(self):
         it, slabs, edges = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
         _lib.call(f"{self._prefix}_last_stats", self._h, ctypes.byref(it), ctypes.byref(slabs),
@@ -74,7 +81,7 @@ class _Traversal:
 
     def __del__(self):
         try:
-            if self._h:
+            if _lib.finalizer_ok() and self._h:
                 getattr(_lib.lib(), f"{self._prefix}_destroy")(self._h)
         except Exception:
             pass
@@ -177,7 +184,7 @@ class PageRank:
 
     def __del__(self):
         try:
-            if self._h:
+            if _lib.finalizer_ok() and self._h:
                 _lib.lib().mk_pagerank_destroy(self._h)
         except Exception:
             pass
@@ -221,7 +228,7 @@ class CC:
 
     def __del__(self):
         try:
-            if self._h:
+            if _lib.finalizer_ok() and self._h:
                 _lib.lib().mk_cc_destroy(self._h)
         except Exception:
             pass
@@ -265,7 +272,7 @@ class TC:
 
     def __del__(self):
         try:
-            if self._h:
+            if _lib.finalizer_ok() and self._h:
                 _lib.lib().mk_tc_destroy(self._h)
         except Exception:
             pass
@@ -305,7 +312,7 @@ class MultiSource:
 
     def __del__(self):
         try:
-            if self._h:
+            if _lib.finalizer_ok() and self._h:
                 _lib.lib().mk_msq_destroy(self._h)
         except Exception:
             pass

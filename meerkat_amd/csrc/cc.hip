@@ -149,6 +149,7 @@ struct CC {
     g = graph;
     MK_HIP(hipSetDevice(g->device));
     MK_HIP(hipStreamCreate(&stream));
+    loop_stream_add(g->device, stream, 1);
     MK_HIP(hipEventCreate(&ev0));
     MK_HIP(hipEventCreate(&ev1));
     MK_HIP(hipMalloc(&parent, size_t(g->n) * 4));
@@ -163,7 +164,10 @@ struct CC {
     (void)hipFree(parent);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      loop_stream_add(g->device, stream, -1);
+      (void)hipStreamDestroy(stream);
+    }
   }
 
   int finish(float *ms) {
@@ -259,6 +263,7 @@ int mk_cc_incremental(mk_cc *c, const uint32_t *src, const uint32_t *dst, uint32
                       float *ms) {
   if (!c) return fail(MK_ERR_BAD_ARG, "mk_cc_incremental: NULL handle");
   if (int r = graph_settle(c->c.g)) return r;
+  if (int r = graph_failed_batch(c->c.g, src, dst, n)) return r;
   return c->c.run_incremental(src, dst, n, ms);
 }
 

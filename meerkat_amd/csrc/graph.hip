@@ -724,6 +724,7 @@ static int batch_prologue(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   g->status_n = n;
   g->last_op = 0;
   g->last_src = g->last_dst = g->last_w = nullptr;
+  g->failed.code = 0;
   if (n == 0) return MK_OK;
   if (++g->mut_epoch == 0) {  // epochs wrapped: clear the versioned counters once
     fill_u32(g->counters, 0u, 4, s);
@@ -836,6 +837,18 @@ static int mutation_outcome(mk_graph *g, int op, const uint32_t *src, const uint
 }  // extern "C"
 
 namespace mk {
+// A failed batch: its statuses no longer describe the graph, and later
+// algorithm calls given the same batch report its error (graph_failed_batch).
+static void record_failure(mk_graph *g, int code, const uint32_t *src, const uint32_t *dst,
+                           uint32_t n) {
+  g->last_op = 0;
+  g->failed.code = code;
+  g->failed.src = src;
+  g->failed.dst = dst;
+  g->failed.n = n;
+  g->failed.msg = g_err;
+}
+
 // Settle the graph's deferred mutation, if one is queued: wait for it and
 // report its outcome once (restoring the graph on a pool failure).
 int graph_settle(mk_graph *g) {
@@ -845,8 +858,15 @@ int graph_settle(mk_graph *g) {
   if (int r = host_wait_word(g->host_scratch.as<uint32_t>() + 4, g->pend.seq, g->pend.s))
     return r;
   const int r = mutation_outcome(g, g->pend.op, g->pend.src, g->pend.dst, g->pend.n, g->pend.s);
-  if (r) g->last_op = 0;  // the batch's statuses no longer describe the graph
+  if (r) record_failure(g, r, g->pend.src, g->pend.dst, g->pend.n);
   return r;
+}
+
+int graph_failed_batch(const mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n) {
+  if (g && g->mirror_of) g = g->mirror_of;
+  if (!g || !g->failed.code || n == 0) return MK_OK;
+  if (g->failed.src != src || g->failed.dst != dst || g->failed.n != n) return MK_OK;
+  return fail(g->failed.code, "this batch failed in its mutation call (" + g->failed.msg + ")");
 }
 }  // namespace mk
 
@@ -871,7 +891,7 @@ static int mutation_finish(mk_graph *g, int op, const uint32_t *src, const uint3
   }
   MK_HIP(hipStreamSynchronize(s));
   const int r = mutation_outcome(g, op, src, dst, n, s);
-  if (r) g->last_op = 0;
+  if (r) record_failure(g, r, src, dst, n);
   return r;
 }
 

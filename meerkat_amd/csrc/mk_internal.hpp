@@ -223,6 +223,18 @@ int hip_fail(hipError_t e, const char *what);
 // Settle a graph's deferred mutation (graph.hip): wait for it and return its
 // outcome once; MK_OK when none is queued.
 int graph_settle(mk_graph *g);
+// The error of the graph's last mutation batch when it failed and (src, dst,
+// n) names that batch (same device pointers and size), else MK_OK.  An
+// algorithm call given a failed batch returns this instead of running on
+// edges the graph never applied (every handle of the graph, not only the
+// one whose call settled the batch).
+int graph_failed_batch(const mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n);
+// Streams of the library's handles, per device (traverse.hip).  A frontier
+// loop's grid barrier needs every block resident; it is launched as an
+// ordinary kernel only while a single stream of the device is registered
+// (no other handle's kernels can hold compute units meanwhile), and
+// cooperatively otherwise.
+void loop_stream_add(int device, hipStream_t s, int delta);
 int device_cus(int device);
 int64_t tune_get(const char *name, int64_t dflt);
 // Grid-stride kernels: the number of waves to launch on `device`.
@@ -303,6 +315,14 @@ struct mk_graph {
   const void *last_src = nullptr;
   const void *last_dst = nullptr;
   const void *last_w = nullptr;
+  // the last batch, when it failed (graph_failed_batch): its identity and
+  // error, kept until the next mutation
+  struct Failed {
+    int code;
+    const void *src, *dst;
+    uint32_t n;
+    std::string msg;
+  } failed = {0, nullptr, nullptr, 0, std::string()};
   mk::HostBuf host_scratch;
   int users = 0;                 // algorithm handles attached
   mk_graph *rev = nullptr;       // in-edge mirror (MK_GRAPH_REVERSE), or null

@@ -570,6 +570,7 @@ struct MultiSource {
     const size_t n = g->n;
     MK_HIP(hipSetDevice(g->device));
     MK_HIP(hipStreamCreate(&stream));
+    loop_stream_add(g->device, stream, 1);
     MK_HIP(hipEventCreate(&ev0));
     MK_HIP(hipEventCreate(&ev1));
     for (uint32_t i = 0; i < kMsSlots; ++i) {
@@ -613,7 +614,10 @@ struct MultiSource {
     host.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      loop_stream_add(g->device, stream, -1);
+      (void)hipStreamDestroy(stream);
+    }
   }
 
   int read_counts(MsCounts &h) {

@@ -353,6 +353,7 @@ struct PageRank {
     const size_t n = g->n;
     MK_HIP(hipSetDevice(g->device));
     MK_HIP(hipStreamCreate(&stream));
+    loop_stream_add(g->device, stream, 1);
     MK_HIP(hipEventCreate(&ev0));
     MK_HIP(hipEventCreate(&ev1));
     for (int k = 0; k < 2; ++k) {
@@ -426,7 +427,10 @@ struct PageRank {
     host.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      loop_stream_add(g->device, stream, -1);
+      (void)hipStreamDestroy(stream);
+    }
   }
 
   int compute(float *ms) {
@@ -565,6 +569,9 @@ int mk_pagerank_incremental(mk_pagerank *p, const uint32_t *src, const uint32_t 
                             float *ms) {
   if (!p) return fail(MK_ERR_BAD_ARG, "mk_pagerank_incremental: NULL handle");
   if (int r = graph_settle(p->p.g)) return r;
+  // a graph of reversed edges saw the batch as (dst, src)
+  if (int r = graph_failed_batch(p->p.g, src, dst, n)) return r;
+  if (int r = graph_failed_batch(p->p.g, dst, src, n)) return r;
   return p->p.batch(src, dst, n, +1, ms);
 }
 
@@ -572,6 +579,8 @@ int mk_pagerank_decremental(mk_pagerank *p, const uint32_t *src, const uint32_t 
                             float *ms) {
   if (!p) return fail(MK_ERR_BAD_ARG, "mk_pagerank_decremental: NULL handle");
   if (int r = graph_settle(p->p.g)) return r;
+  if (int r = graph_failed_batch(p->p.g, src, dst, n)) return r;
+  if (int r = graph_failed_batch(p->p.g, dst, src, n)) return r;
   return p->p.batch(src, dst, n, -1, ms);
 }
 

@@ -464,6 +464,7 @@ struct TC {
     g = graph;
     MK_HIP(hipSetDevice(g->device));
     MK_HIP(hipStreamCreate(&stream));
+    loop_stream_add(g->device, stream, 1);
     MK_HIP(hipEventCreate(&ev0));
     MK_HIP(hipEventCreate(&ev1));
     MK_HIP(hipMalloc(&slots, (kTcSlots + 2) * 8));
@@ -484,7 +485,10 @@ struct TC {
     host.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      loop_stream_add(g->device, stream, -1);
+      (void)hipStreamDestroy(stream);
+    }
   }
 
   int start() {
@@ -702,6 +706,7 @@ int mk_tc_incremental(mk_tc *t, const uint32_t *src, const uint32_t *dst, uint32
                       float *ms) {
   if (!t) return fail(MK_ERR_BAD_ARG, "mk_tc_incremental: NULL handle");
   if (int r = graph_settle(t->t.g)) return r;
+  if (int r = graph_failed_batch(t->t.g, src, dst, n)) return r;
   return t->t.run_incremental(src, dst, n, ms);
 }
 
@@ -709,6 +714,7 @@ int mk_tc_decremental(mk_tc *t, const uint32_t *src, const uint32_t *dst, uint32
                       float *ms) {
   if (!t) return fail(MK_ERR_BAD_ARG, "mk_tc_decremental: NULL handle");
   if (int r = graph_settle(t->t.g)) return r;
+  if (int r = graph_failed_batch(t->t.g, src, dst, n)) return r;
   return t->t.run_decremental(src, dst, n, ms);
 }
 

@@ -1082,11 +1082,14 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Relax the batch edges (u, v) into side 0 / slot 0.  Unit weights: thread
-// per edge.  Weighted: the weight is the batch's when this batch was the
-// graph's last insert and the edge was applied (`status`, may be NULL), and
-// is searched in the graph otherwise (the graph keeps an existing edge's
-// weight), one wave-serial search per such edge.
+// Relax the batch edges (u, v) into side 0 / slot 0, thread per edge.  When
+// this batch was the graph's last insert (`status` non-NULL), only the edges
+// it applied are relaxed, with the batch's weight: an edge it found present
+// was already relaxed by the exact distances before the call, and one it
+// did not apply is not in the graph.  Without `status`, every edge is taken
+// as present (the reference's protocol, BFSImpl.cuh:270-382), its weight
+// searched in the graph (which keeps an existing edge's weight), one
+// wave-serial search per edge.
 constexpr uint32_t kItemBlock = 1024;  // one item per thread kernels
 
 template <bool MAP, bool WEIGHTS>
@@ -1104,9 +1107,9 @@ __global__ void __launch_bounds__(kItemBlock)
     // group of loads issued together
     u = src[i];
     v = dst[i];
-    const bool applied = WEIGHTS && status && wts && status[i] == MK_EDGE_APPLIED;
-    const uint32_t bw = WEIGHTS && wts ? wts[i] : 1u;
-    act = u != v && u < G.n && v < G.n;
+    const bool applied = !status || status[i] == MK_EDGE_APPLIED;
+    const uint32_t bw = WEIGHTS && wts ? wts[i] : 0u;
+    act = applied && u != v && u < G.n && v < G.n;
     if (act) {
       // a stale dist[v] only lets an extra atomic through: values only fall
       xu = S.dist[u];
@@ -1114,7 +1117,7 @@ __global__ void __launch_bounds__(kItemBlock)
     }
     act = act && dist_of(xu) != kInf;
     if (act && WEIGHTS) {
-      if (applied) wgt = bw;
+      if (status) wgt = bw;  // applied by this batch: its weight (NULL weights: 0)
       else need = true;
     }
   }
@@ -1301,17 +1304,21 @@ __global__ void k_unpack(const uint64_t *dist, uint32_t n, uint32_t *d, uint32_t
 }
 
 // ---------------------------------------------------------------- host
-// Streams that run frontier loops, per device.  A loop's grid barrier needs
-// every block resident.  While all loops of the process run on one stream
-// they never overlap, and a grid sized to what the device keeps resident
-// is enough: they are launched like any kernel.  Once loops may run on two
-// streams at once (handles on different graphs), two partly resident grids
-// could wait on each other, so loops are launched cooperatively (the
+// Streams of the library's handles, per device (BFS/SSSP handles share
+// their graph's stream; multi-source, PageRank, TC and CC handles have their
+// own).  A frontier loop's grid barrier needs every block resident.  While
+// a single stream of the device is registered, nothing else of the
+// library's can run beside a loop and hold compute units, and a grid sized
+// to what the device keeps resident is enough: loops are launched like any
+// kernel.  Once handles of the device run on two streams (another graph's
+// traversals, a multi-source query, PageRank, TC or CC work queued from
+// another thread), a partly resident loop grid could wait on kernels that
+// hold the rest of the device, so loops are launched cooperatively (the
 // runtime checks co-residency).
 static std::mutex g_loop_mu;
 static std::map<int, std::map<hipStream_t, int>> g_loop_streams;  // device -> stream -> handles
 
-static void loop_stream_add(int device, hipStream_t s, int d) {
+void loop_stream_add(int device, hipStream_t s, int d) {
   std::lock_guard<std::mutex> lk(g_loop_mu);
   auto &m = g_loop_streams[device];
   if ((m[s] += d) <= 0) m.erase(s);
@@ -1604,6 +1611,8 @@ struct Traversal {
   bool dirty = true;          // counters may be non-zero (cleared at the next begin)
   bool stale = false;         // results computed on a batch that failed (until Static)
   int reseed_kind = 0;        // queued re-seed: 0 none, 1 pull (mirror), 2 bucket scan
+  uint64_t deferred_settled = 0;    // deferred calls settled since create
+  uint64_t deferred_recomputed = 0; // of those, recomputed from scratch (ran out of steps)
 
   uint32_t inval_limit() const {
     return uint32_t(std::min<uint64_t>(uint64_t(g->n) + 8, 0xFFFFFFF0ull));
@@ -1835,7 +1844,11 @@ struct Traversal {
     last_slabs = h.slabs;
     last_edges = h.edges;
     last_iters = h.steps;
-    if (ran_out) return run_static(nullptr);
+    ++deferred_settled;
+    if (ran_out) {
+      ++deferred_recomputed;
+      return run_static(nullptr);
+    }
     return MK_OK;
   }
 
@@ -1869,6 +1882,10 @@ struct Traversal {
                       float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "incremental: NULL batch");
     if (int r = settle()) return r;
+    if (int r = graph_failed_batch(g, src, dst, n)) {
+      stale = true;
+      return r;
+    }
     if (stale) return fail(MK_ERR_STATE, "incremental: results are invalid after a failed batch; "
                                          "run Static() first");
     if (int r = begin(!defer)) return r;
@@ -1876,15 +1893,16 @@ struct Traversal {
       const uint32_t ep = next_epoch();
       const GraphView G = g->view();
       // the insert's per-edge status applies when this is the batch it saw
+      // (SSSP: with the same weights, which it then takes from the batch)
       const bool same = g->last_op == 1 && g->last_src == src && g->last_dst == dst &&
-                        g->status_n == n && g->last_w == w;
+                        g->status_n == n && (!weights || g->last_w == w);
       const uint8_t *st = same ? g->status.as<uint8_t>() : nullptr;
       {
         const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
         auto k = weights ? k_seed_batch<true, true>
                          : (g->weighted ? k_seed_batch<true, false> : k_seed_batch<false, false>);
         prof_launch("k_seed_batch", 24.0 * n, k, dim3(grid), dim3(kItemBlock), 0u, stream, G,
-                    view(), src, dst, weights ? w : nullptr, weights ? st : nullptr, n, ep);
+                    view(), src, dst, weights ? w : nullptr, st, n, ep);
       }
       MK_HIP(hipGetLastError());
       if (persistent) {
@@ -1934,6 +1952,10 @@ struct Traversal {
   int run_decremental(const uint32_t *src, const uint32_t *dst, uint32_t n, float *ms) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "decremental: NULL batch");
     if (int r = settle()) return r;
+    if (int r = graph_failed_batch(g, src, dst, n)) {
+      stale = true;
+      return r;
+    }
     if (stale) return fail(MK_ERR_STATE, "decremental: results are invalid after a failed batch; "
                                          "run Static() first");
     if (int r = begin(!defer)) return r;
@@ -2165,6 +2187,18 @@ int mk_bfs_set_deferred(mk_bfs *b, int on) {
 int mk_bfs_sync(mk_bfs *b) {
   if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_sync: NULL handle");
   return b->t.settle();
+}
+int mk_bfs_deferred_stats(mk_bfs *b, uint64_t *settled, uint64_t *recomputed) {
+  if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_deferred_stats: NULL handle");
+  if (settled) *settled = b->t.deferred_settled;
+  if (recomputed) *recomputed = b->t.deferred_recomputed;
+  return MK_OK;
+}
+int mk_sssp_deferred_stats(mk_sssp *s, uint64_t *settled, uint64_t *recomputed) {
+  if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_deferred_stats: NULL handle");
+  if (settled) *settled = s->t.deferred_settled;
+  if (recomputed) *recomputed = s->t.deferred_recomputed;
+  return MK_OK;
 }
 int mk_sssp_set_deferred(mk_sssp *s, int on) {
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_set_deferred: NULL handle");

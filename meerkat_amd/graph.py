@@ -68,7 +68,7 @@ class SlabPool:
 
     def __del__(self):
         try:
-            if self._h:
+            if _lib.finalizer_ok() and self._h:
                 _lib.lib().mk_pool_destroy(self._h)
         except Exception:
             pass
@@ -227,7 +227,7 @@ class DynamicGraph:
 
     def __del__(self):
         try:
-            if self._h and self._owned:
+            if _lib.finalizer_ok() and self._h and self._owned:
                 _lib.lib().mk_graph_destroy(self._h)
         except Exception:
             pass

@@ -70,6 +70,9 @@ class ServingLoop:
     MAX_BATCH = MultiSource.MAX_SOURCES
 
     def __init__(self, graph, keep_results=False, slab_margin=1.25):
+        """keep_results: True keeps every query's distances on the host
+        (``Request.result``), a callable keeps those of the requests it
+        accepts (a sample, on graphs where each answer is large)."""
         self.g = graph
         self.keep_results = keep_results
         self.slab_margin = slab_margin
@@ -81,6 +84,16 @@ class ServingLoop:
         self.version = 0
         self.rejected_rounds = 0
         self.phase_s = {"updates": 0.0, "queries": 0.0}  # wall time per phase, last run
+
+    def _keeps(self, r):
+        k = self.keep_results
+        return k(r) if callable(k) else bool(k)
+
+    def close(self):
+        """Release the loop's traversal handles (before the graph)."""
+        for h in (self.bfs, self.sssp, self.bfs1, self.sssp1):
+            if h is not None:
+                h.close()
 
     # ------------------------------------------------------------ updates
     def _slabs_needed(self, n_ins):
@@ -119,7 +132,7 @@ class ServingLoop:
         engine.run(np.array([r.src for r in reqs], np.uint32))
         t = now()
         for slot, r in enumerate(reqs):
-            if self.keep_results:
+            if self._keeps(r):
                 r.result = engine.distances(slot)
             r.done = t
 
@@ -129,7 +142,7 @@ class ServingLoop:
         r.version = self.version
         eng.set_source(r.src)
         eng.Static()
-        if self.keep_results:
+        if self._keeps(r):
             r.result = eng.Distances()
         r.done = now()
 

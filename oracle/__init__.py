@@ -184,13 +184,15 @@ def check_parents_fast(g, dist, parent, source, weights=False):
     if len(v) == 0:
         return 0
     p = parent[v].astype(np.int64)
-    if (p >= g.n).any():
-        return int((p >= g.n).sum())
+    in_range = p < g.n
+    p = np.where(in_range, p, 0)  # ids past n are violations; read vertex 0 instead
     keys = g.keys()
+    if len(keys) == 0:
+        return len(v)
     q = (p.astype(np.uint64) << np.uint64(32)) | v.astype(np.uint64)
     j = np.searchsorted(keys, q)
     j = np.minimum(j, len(keys) - 1)
     present = keys[j] == q
     w = g.w[j].astype(np.int64) if weights else np.ones(len(v), np.int64)
-    ok = present & (dist[p].astype(np.int64) + w == dist[v].astype(np.int64))
+    ok = in_range & present & (dist[p].astype(np.int64) + w == dist[v].astype(np.int64))
     return int((~ok).sum())

@@ -91,19 +91,30 @@ int64_t orc_csr_build(uint32_t n, uint64_t m, const uint32_t *src, const uint32_
     t->w = w ? w[i] : 0;
     t->ord = i;
   }
-  uint64_t out = 0;
-  row_off[0] = 0;
-  for (uint32_t v = 0; v < n; ++v) {
-    uint64_t lo = cnt[v], hi = cnt[v + 1];
+  /* rows are sorted (by column, then input order) and counted in parallel,
+   * then written at their offsets: the same result as one sequential pass */
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t v = 0; v < (int64_t)n; ++v) {
+    const uint64_t lo = cnt[v], hi = cnt[v + 1];
     if (hi - lo > 1) qsort(e + lo, hi - lo, sizeof(orc_ent), ent_cmp);
-    for (uint64_t j = lo; j < hi; ++j) {
-      if (j > lo && e[j].col == e[j - 1].col) continue; /* keep first */
+    uint64_t u = 0;
+    for (uint64_t j = lo; j < hi; ++j)
+      if (j == lo || e[j].col != e[j - 1].col) ++u; /* keep first */
+    pos[v] = u;
+  }
+  row_off[0] = 0;
+  for (uint32_t v = 0; v < n; ++v) row_off[v + 1] = row_off[v] + pos[v];
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t v = 0; v < (int64_t)n; ++v) {
+    uint64_t out = row_off[v];
+    for (uint64_t j = cnt[v]; j < cnt[v + 1]; ++j) {
+      if (j > cnt[v] && e[j].col == e[j - 1].col) continue;
       col[out] = e[j].col;
       if (wcol) wcol[out] = e[j].w;
       ++out;
     }
-    row_off[v + 1] = out;
   }
+  const uint64_t out = row_off[n];
   free(cnt);
   free(e);
   free(pos);

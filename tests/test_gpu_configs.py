@@ -14,9 +14,8 @@ the oracle's from-scratch recompute on the updated graph:
   every batch.
 - C4: undirected RMAT scale 22 (seed 7), 10 batches of 500,000 new
   undirected edges (both directions, seed 8); incremental triangle count and
-  connected components: component labels bit-exact after every batch,
-  triangle count bit-exact at the static count, after the first batch and
-  after the last (the CPU count takes ~15 s per check).
+  connected components: component labels and the triangle count bit-exact
+  at the static count and after every batch.
 
 Per-batch GPU times are written as one JSON line per workload to
 $MK_RESULTS_DIR/configs.jsonl when that variable names a directory."""
@@ -181,8 +180,8 @@ def test_c4_rmat22_tc_cc_inserts(gpu):
         s2, d2 = gen.split_keys(keys)
         ref = oracle.csr(n, s2, d2)
         assert np.array_equal(cc.Labels(), oracle.cc(ref)[0]), i
-        if i in (0, 9):
-            assert tc.GetCount() == oracle.tc(ref), i
+        assert tc.GetCount() == oracle.tc(ref), i
+        print(f"C4 batch {i}: {tc.GetCount()} triangles, labels exact", flush=True)
     record("C4", {"vertices": n, "directed_entries": int(len(s)), "static_tc_ms": tc_static_ms,
                   "static_cc_ms": cc_static_ms, "incremental_tc_ms": tci,
                   "incremental_cc_ms": cci, "triangles": int(tc.GetCount()),

@@ -190,3 +190,61 @@ def test_deferred_call_on_a_failed_batch_reports_it(gpu):
     assert ei.value.code == _lib.MK_ERR_STATE
     b.Static()
     check(b, n, s, d)
+
+
+@pytest.mark.parametrize("failure", ["range", "pool"])
+def test_failed_batch_is_reported_to_every_handle(gpu, failure):
+    # the usual pattern of one graph with an SSSP and a BFS handle, deferred
+    # mutations and deferred calls: the first call settles the failed batch
+    # and reports it; the second handle, given the same batch, must not run
+    # on edges the graph never applied (its seed would relax them unchecked)
+    if failure == "range":
+        n, s, d, w = gen.rmat(12, 8, seed=65, weighted=True)
+        pool = mk.SlabPool(1 << 26)
+        bad_s = np.array([0, 1, n + 3], np.uint32)
+        bad_d = np.array([5, 6, 7], np.uint32)
+        code = _lib.MK_ERR_RANGE
+    else:
+        n = 200000
+        s = np.zeros(500, np.uint32)
+        d = np.arange(1, 501, dtype=np.uint32)
+        w = np.full(500, 3, np.uint32)
+        pool = mk.SlabPool(2048 * 256)
+        bad_s = np.r_[np.zeros(100000, np.uint32), np.array([1], np.uint32)]
+        bad_d = np.r_[np.arange(1000, 101000, dtype=np.uint32), np.array([0], np.uint32)]
+        code = _lib.MK_ERR_POOL_FULL
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
+    g.InsertEdges(s, d, w)
+    g.set_deferred(True)
+    sp, bf = mk.SSSP(g, 0), mk.BFS(g, 0)
+    sp.Static()
+    bf.Static()
+    sp.set_deferred(True)
+    bf.set_deferred(True)
+    bs, bd = mk.device_u32(bad_s), mk.device_u32(bad_d)
+    bw = mk.device_u32(np.ones(len(bad_s), np.uint32))
+    g.InsertEdges(bs, bd, bw)
+    with pytest.raises(mk.MeerkatError) as ei:
+        sp.Incremental(bs, bd, bw)
+    assert ei.value.code == code
+    with pytest.raises(mk.MeerkatError) as ei:
+        bf.Incremental(bs, bd)
+    assert ei.value.code == code
+    for h in (sp, bf):  # both invalid until Static
+        with pytest.raises(mk.MeerkatError) as ei:
+            h.Incremental(np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+        assert ei.value.code == _lib.MK_ERR_STATE
+    sp.Static()
+    bf.Static()
+    check(sp, n, s, d, w)
+    check(bf, n, s, d)
+    # the next mutation clears the record: a good batch runs everywhere
+    gs, gd = mk.device_u32(np.array([2], np.uint32)), mk.device_u32(np.array([3], np.uint32))
+    gw = mk.device_u32(np.array([4], np.uint32))
+    g.InsertEdges(gs, gd, gw)
+    sp.Incremental(gs, gd, gw)
+    bf.Incremental(gs, gd)
+    s2, d2, w2 = np.r_[s, [2]].astype(np.uint32), np.r_[d, [3]].astype(np.uint32), \
+        np.r_[w, [4]].astype(np.uint32)
+    check(sp, n, s2, d2, w2)
+    check(bf, n, s2, d2)

@@ -263,3 +263,37 @@ def test_status_of_every_edge_when_the_grid_has_spare_waves(gpu, mirror, count):
     st = g.last_status()
     assert (st == _lib.EDGE_APPLIED).all(), np.bincount(st)
     assert g.num_edges() == 0
+
+
+@pytest.mark.parametrize("weighted,lf,hints,expected", [
+    # set slabs hold 63 keys: buckets = max(1, ceil(hint / (63 * lf)))
+    (False, 1.0, [0, 1, 63, 64, 126, 127, 1000], [1, 1, 1, 2, 2, 3, 16]),
+    (False, 0.5, [0, 31, 32, 63, 64], [1, 1, 2, 2, 3]),
+    # map slabs hold 31 (key, weight) pairs
+    (True, 1.0, [0, 31, 32, 62, 63], [1, 1, 2, 2, 3]),
+    (True, 0.7, [21, 22, 43, 44], [1, 2, 2, 3]),
+])
+def test_bucket_count_rule(gpu, weighted, lf, hints, expected):
+    # the reference's rule (include/Graph/SlabGraph.cuh:404-406) with the
+    # keys per 256-byte slab of this layout
+    n = len(hints)
+    pool = mk.SlabPool(1 << 24)
+    g = mk.DynamicGraph(n, pool, lf, np.array(hints, np.uint32), weighted=weighted)
+    assert g.GetBucketsPerVertex().tolist() == expected
+    off = g.GetVertexBucketsOffsets()
+    assert off[0] == 0 and np.array_equal(np.diff(off), expected)
+    g.close()
+    pool.close()
+
+
+def test_bucket_count_rule_random_hints(gpu):
+    rng = np.random.default_rng(8)
+    hints = rng.integers(0, 5000, 20000).astype(np.uint32)
+    pool = mk.SlabPool(1 << 26)
+    for weighted, keys in ((False, 63), (True, 31)):
+        for lf in (0.3, 0.7, 1.0, 2.0):
+            g = mk.DynamicGraph(len(hints), pool, lf, hints, weighted=weighted)
+            exp = np.maximum(1, np.ceil(hints / (keys * lf))).astype(np.int64)
+            assert np.array_equal(g.GetBucketsPerVertex().astype(np.int64), exp), (weighted, lf)
+            g.close()
+    pool.close()

@@ -107,3 +107,46 @@ def test_autotune_picks_a_setting_and_keeps_parity(gpu):
     finally:
         for k_, v in before.items():
             autotune.set_knob(k_, v)
+
+
+def test_frontier_loops_beside_a_running_multi_source_query(gpu):
+    # a multi-source SSSP on its own stream (a thread) while single-source
+    # SSSP calls run their persistent loops on the graph's stream: with
+    # another of the device's streams registered, the loops are launched
+    # cooperatively (every block co-resident), and both answers stay exact.
+    # Every wait is bounded (barrier_timeout_ms): a loop that could not
+    # gather its blocks would fail its call, not hang.
+    import threading
+
+    n, s, d, w = gen.rmat(16, 16, seed=34, weighted=True)
+    pool = mk.SlabPool(1 << 28)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=True,
+                        in_degree_hints=gen.degree_hints(n, d))
+    g.InsertEdges(s, d, w)
+    ref = oracle.csr(n, s, d, w)
+    q = mk.MultiSource(g, sssp=True)
+    sp = mk.SSSP(g, 0)
+    srcs = np.random.default_rng(9).integers(0, n, 64).astype(np.uint32)
+    errors = []
+
+    def queries():
+        try:
+            for _ in range(3):
+                q.run(srcs)
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    th = threading.Thread(target=queries)
+    th.start()
+    for src in (0, 5, 17, 0):
+        sp.set_source(src)
+        sp.Static()
+    th.join()
+    assert not errors, errors
+    assert np.array_equal(sp.Distances(), oracle.dijkstra(ref, 0))
+    for slot in (0, 31, 63):
+        assert np.array_equal(q.distances(slot), oracle.dijkstra(ref, int(srcs[slot]))), slot
+    q.close()
+    sp.close()
+    g.close()
+    pool.close()

@@ -285,3 +285,57 @@ def test_parallel_sssp_matches_dijkstra_rmat():
     g = oracle.csr(n, s, d, w)
     for src in (0, 77, n - 1):
         assert np.array_equal(oracle.sssp_par(g, src, 4), oracle.dijkstra(g, src))
+
+
+def _tight_parents(g, dist, weights):
+    """A valid parent for every reached vertex: the first in-neighbour u
+    with dist[u] + w(u, v) == dist[v] (brute force over the edge list)."""
+    src = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.row_off).astype(np.int64))
+    dst = g.col.astype(np.int64)
+    w = g.w.astype(np.int64) if weights else np.ones(len(dst), np.int64)
+    ok = (dist[src] != INF) & (dist[src].astype(np.int64) + w == dist[dst].astype(np.int64))
+    parent = np.full(g.n, INF, np.uint32)
+    first = {}
+    for u, v in zip(src[ok].tolist(), dst[ok].tolist()):
+        first.setdefault(v, u)
+    for v, u in first.items():
+        parent[v] = u
+    return parent
+
+
+@pytest.mark.parametrize("weights", [False, True])
+def test_parent_checkers_catch_planted_parents(weights):
+    n, s, d, w = rmat_graph(11, 8, weighted=True)
+    g = oracle.csr(n, s, d, w if weights else None)
+    source = 0
+    dist = oracle.dijkstra(g, source) if weights else oracle.bfs(g, source)
+    parent = _tight_parents(g, dist, weights)
+    reached = np.nonzero((dist != INF) & (np.arange(n) != source))[0]
+    assert len(reached) > 100
+    assert oracle.check_parents_fast(g, dist, parent, source, weights) == 0
+    assert oracle.check_parents(g, dist, parent, source, weights) == 0
+    rng = np.random.default_rng(3)
+    bad = parent.copy()
+    victims = rng.choice(reached, size=30, replace=False)
+    planted = 0
+    keys = set(g.keys().tolist())
+    for k, v in enumerate(victims.tolist()):
+        if k % 3 == 0:  # a vertex id past n
+            bad[v] = n + k
+        elif k % 3 == 1:  # a vertex that has no edge to v
+            q = next(q for q in range(n) if q != v and ((q << 32) | v) not in keys)
+            bad[v] = q
+        else:  # an in-neighbour whose distance plus weight is not dist[v]
+            sorted_keys = g.keys()
+
+            def slack(u):
+                j = np.searchsorted(sorted_keys, np.uint64((u << 32) | v))
+                wt = int(g.w[j]) if weights else 1
+                return int(dist[u]) + wt != int(dist[v])
+
+            cand = [u for u in range(n)
+                    if ((u << 32) | v) in keys and dist[u] != INF and slack(u)]
+            bad[v] = cand[0] if cand else n + k  # none: an id past n instead
+        planted += 1
+    assert oracle.check_parents_fast(g, dist, bad, source, weights) == planted
+    assert oracle.check_parents(g, dist, bad, source, weights) == planted
