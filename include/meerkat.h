@@ -91,6 +91,7 @@ int mk_graph_destroy(mk_graph *g);
 /* Flags of mk_graph_create_ex. */
 #define MK_GRAPH_WEIGHTED 1u /* ConcurrentMap adjacency (dst -> weight)        */
 #define MK_GRAPH_REVERSE 2u  /* also keep an in-edge mirror (dst -> src [, w]) */
+#define MK_GRAPH_TRACK_UPDATES 4u /* mark inserted keys for mk_graph_updated_edges */
 /* DynamicGraph constructor with options.  MK_GRAPH_REVERSE keeps a second
  * slab-hash graph of in-edges on the same pool, updated by every
  * mk_graph_insert / mk_graph_delete; decremental BFS/SSSP then re-seed from
@@ -134,6 +135,41 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_de
  * stay valid until the batch is settled. */
 int mk_graph_set_deferred(mk_graph *g, int on);
 int mk_graph_sync(mk_graph *g);
+
+/* DynamicGraph::UpdateSlabPointers() (include/Graph/SlabGraphImpl.cuh:64-82
+ * -> device/UpdateSlablistPointers.cuh:4-104): the edges inserted so far stop
+ * being "updated"; mk_graph_updated_edges then reports only edges inserted
+ * after this call.  A no-op on a graph created without
+ * MK_GRAPH_TRACK_UPDATES (as the reference's with nothing inserted). */
+int mk_graph_update_slab_pointers(mk_graph *g);
+/* The update-aware iteration the reference's markers serve
+ * (SlabGraph.cuh:377-379, 425-439: a bucket is iterated from its first
+ * updated slab and lane): every edge inserted since the last
+ * mk_graph_update_slab_pointers (since creation before the first) and still
+ * present, written as (src_dev[i], dst_dev[i][, w_dev[i]]) for i <
+ * min(*count, capacity) in no particular order; *count receives the total,
+ * so a call with capacity 0 sizes the buffers.  An edge deleted and
+ * inserted again counts as inserted; a weight update does not.  Needs a
+ * graph created with MK_GRAPH_TRACK_UPDATES (MK_ERR_STATE otherwise), whose
+ * inserts then also mark each applied key (one more chain walk per applied
+ * edge).  w_dev may be NULL; it is ignored on an unweighted graph. */
+int mk_graph_updated_edges(mk_graph *g, uint32_t *src_dev, uint32_t *dst_dev, uint32_t *w_dev,
+                           uint64_t capacity, uint64_t *count);
+
+/* DynamicGraph::UpdateEdges(src, dst, n, NewEdgeValues, FilterMaps)
+ * (include/Graph/SlabGraphImpl.cuh:84-103, compiled out with #if 0 in the
+ * reference; device/UpdateEdgeKernel.cuh:7-37 -> warp/UpdateEdge.cuh:7-33)
+ * on a weighted graph: each batch edge present in the graph (and its
+ * in-edge mirror entry) takes the weight w_dev[i]; an absent edge is not
+ * inserted; self-loops are skipped.  Per-edge outcomes (mk_graph_last_status):
+ * MK_EDGE_APPLIED (present, weight set), MK_EDGE_UNCHANGED (absent),
+ * MK_EDGE_SELF_LOOP.  A batch naming an edge twice keeps one of its
+ * weights.  The reference's per-edge FilterMaps belong to the absent
+ * SlabHash's PhaseConcurrentMap and have no counterpart.  Errors, streams
+ * and deferred checks as mk_graph_insert (MK_ERR_RANGE leaves the graph
+ * unchanged); MK_ERR_BAD_ARG on an unweighted graph or NULL w_dev. */
+int mk_graph_update_weights(mk_graph *g, const uint32_t *src_dev, const uint32_t *dst_dev,
+                            const uint32_t *w_dev, uint32_t n, void *stream);
 
 /* Batched DynamicGraphContext::SearchEdge (include/Graph/warp/SearchEdge.cuh:
  * 5-63, 118-142): found_dev[i] = 1 if (src[i], dst[i]) is present; for a
@@ -221,6 +257,16 @@ int mk_sssp_incremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst
                         const uint32_t *w_dev, uint32_t n, float *ms);
 int mk_sssp_decremental(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst_dev,
                         const uint32_t *w_dev, uint32_t n, float *ms);
+/* Shortest paths kept exact across a weight update (no reference
+ * counterpart: the reference's UpdateEdges is compiled out and no algorithm
+ * follows it).  Call after mk_graph_update_weights with the same batch
+ * (w_dev: its new weights).  A tree edge that became heavier invalidates
+ * its subtree, which is re-seeded from valid in-neighbours (as
+ * Decremental); every batch edge is then relaxed with its weight in the
+ * graph (as Incremental), which covers the edges that became lighter.
+ * Deferred like Incremental / Decremental. */
+int mk_sssp_reweight(mk_sssp *s, const uint32_t *src_dev, const uint32_t *dst_dev,
+                     const uint32_t *w_dev, uint32_t n, float *ms);
 int mk_sssp_read(mk_sssp *s, uint32_t *dist_host, uint32_t *parent_host);
 /* As mk_bfs_set_source. */
 int mk_sssp_set_source(mk_sssp *s, uint32_t source);

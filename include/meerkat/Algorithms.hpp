@@ -105,6 +105,16 @@ struct SSSP {
           "mk_sssp_decremental");
     return ms;
   }
+  // After DynamicGraph::UpdateEdges with the same batch (no reference
+  // counterpart): heavier tree edges are handled as deletions, lighter edges
+  // as insertions (mk_sssp_reweight).
+  float Reweight(uint32_t *EdgesSrcDev, uint32_t *EdgesDstDev, uint32_t *NewWgtDev,
+                 uint32_t EdgesN) {
+    float ms = 0.f;
+    check(mk_sssp_reweight(h_, EdgesSrcDev, EdgesDstDev, NewWgtDev, EdgesN, &ms),
+          "mk_sssp_reweight");
+    return ms;
+  }
   std::vector<uint32_t> Distances() { return read(true); }
   std::vector<uint32_t> Parents() { return read(false); }
   void SetSource(uint32_t source) { check(mk_sssp_set_source(h_, source), "mk_sssp_set_source"); }

@@ -10,8 +10,10 @@
 //  - InsertEdges / DeleteEdges complete before they return (the reference
 //    launches them asynchronously and the caller synchronises; a
 //    synchronisation after the call stays correct);
-//  - UpdateSlabPointers is a no-op: slabs are not append-only here (a
-//    tombstoned slot is reused), so there is no "new edges" marker to move;
+//  - UpdateSlabPointers / UpdatedEdges track inserted keys by a per-slab
+//    mask instead of a position marker (a tombstoned slot is reused here, so
+//    slabs are not append-only); a graph tracks them when created with
+//    track_updates;
 //  - as in the reference (include/Graph/warp/InsertEdge.cuh:125-165), a
 //    graph with a Rev* policy stores each edge the caller inserts or deletes
 //    reversed: (src, dst) goes to dst's adjacency as key src.  PageRank reads
@@ -118,10 +120,11 @@ struct RevWeightedEdgePolicyMapT : WeightedEdgePolicyMapT<AllocPolicy, VertexTy,
 class GraphBase {
  public:
   GraphBase(uint32_t N, SlabPool &pool, double LoadFactor, const uint32_t *VertexDegreeHints,
-            uint32_t DeviceIdx, bool weighted, bool mirror)
+            uint32_t DeviceIdx, bool weighted, bool mirror, bool track_updates = false)
       : n_(N) {
     (void)DeviceIdx;  // the pool's device is the graph's device
-    uint32_t flags = (weighted ? MK_GRAPH_WEIGHTED : 0u) | (mirror ? MK_GRAPH_REVERSE : 0u);
+    uint32_t flags = (weighted ? MK_GRAPH_WEIGHTED : 0u) | (mirror ? MK_GRAPH_REVERSE : 0u) |
+                     (track_updates ? MK_GRAPH_TRACK_UPDATES : 0u);
     check(mk_graph_create_ex(pool.handle(), N, flags, LoadFactor, VertexDegreeHints, nullptr, &g_),
           "mk_graph_create_ex");
   }
@@ -168,7 +171,20 @@ class GraphBase {
     check(mk_graph_search(g_, src_dev, dst_dev, n, found_dev, w_out_dev, nullptr),
           "mk_graph_search");
   }
-  void UpdateSlabPointers() {}
+  // UpdateSlabPointers (SlabGraphImpl.cuh:64-82): edges inserted so far stop
+  // being reported by UpdatedEdges (a graph created with track_updates).
+  void UpdateSlabPointers() {
+    check(mk_graph_update_slab_pointers(g_), "mk_graph_update_slab_pointers");
+  }
+  // The update-aware iteration: the edges inserted since the last
+  // UpdateSlabPointers and still present, into caller-owned device arrays of
+  // `capacity` entries; returns the total (a capacity of 0 only counts).
+  uint64_t UpdatedEdges(uint32_t *src_dev, uint32_t *dst_dev, uint32_t *w_dev, uint64_t capacity) {
+    uint64_t count = 0;
+    check(mk_graph_updated_edges(g_, src_dev, dst_dev, w_dev, capacity, &count),
+          "mk_graph_updated_edges");
+    return count;
+  }
 
   // Deferred mutation checks (mk_graph_set_deferred): InsertEdges /
   // DeleteEdges return once queued; a failed batch throws from the graph's
@@ -197,8 +213,10 @@ class DynamicGraph<EdgePolicy, true> : public GraphBase {
   // `mirror` (no reference counterpart) also keeps an in-edge mirror, which
   // decremental BFS/SSSP use to re-seed and PageRank as its in-adjacency.
   DynamicGraph(uint32_t N, SlabPool &TheEdgeAllocator, double LoadFactor,
-               uint32_t *VertexDegreeHints, uint32_t DeviceIdx = 0, bool mirror = false)
-      : GraphBase(N, TheEdgeAllocator, LoadFactor, VertexDegreeHints, DeviceIdx, true, mirror) {}
+               uint32_t *VertexDegreeHints, uint32_t DeviceIdx = 0, bool mirror = false,
+               bool track_updates = false)
+      : GraphBase(N, TheEdgeAllocator, LoadFactor, VertexDegreeHints, DeviceIdx, true, mirror,
+                  track_updates) {}
 
   // SlabGraphImpl.cuh:6-20.  EdgeValues may be null (weight 0).
   void InsertEdges(VertexT *SourceVertices, VertexT *DestinationVertices, CountT NumberOfEdges,
@@ -214,6 +232,20 @@ class DynamicGraph<EdgePolicy, true> : public GraphBase {
     check(mk_graph_delete(g_, SourceVertices, DestinationVertices, NumberOfEdges, nullptr),
           "mk_graph_delete");
   }
+  // UpdateEdges (SlabGraphImpl.cuh:84-103, compiled out in the reference):
+  // present edges take NewEdgeValues[i]; absent ones are not inserted.  The
+  // reference's FilterMaps argument (a PhaseConcurrentMap detail of the
+  // absent SlabHash) is accepted only as null.
+  template <typename FilterMapTy = void>
+  void UpdateEdges(VertexT *SourceVertices, VertexT *DestinationVertices, CountT NumberOfEdges,
+                   EdgeValueT *NewEdgeValues, FilterMapTy *EdgeFilterMaps = nullptr) {
+    if (EdgeFilterMaps != nullptr)
+      throw Error(MK_ERR_BAD_ARG, "UpdateEdges: FilterMaps are not supported");
+    if (EdgePolicy::IsReversed) std::swap(SourceVertices, DestinationVertices);
+    check(mk_graph_update_weights(g_, SourceVertices, DestinationVertices, NewEdgeValues,
+                                  NumberOfEdges, nullptr),
+          "mk_graph_update_weights");
+  }
 };
 
 // Unweighted: ConcurrentSet adjacency (SlabGraph.cuh:633-717).
@@ -225,8 +257,10 @@ class DynamicGraph<EdgePolicy, false> : public GraphBase {
   using EdgeDynAllocator = typename EdgePolicy::AllocPolicyT::DynamicAllocatorT;
 
   DynamicGraph(uint32_t N, SlabPool &TheEdgeAllocator, double LoadFactor,
-               uint32_t *VertexDegreeHints, uint32_t DeviceIdx = 0, bool mirror = false)
-      : GraphBase(N, TheEdgeAllocator, LoadFactor, VertexDegreeHints, DeviceIdx, false, mirror) {}
+               uint32_t *VertexDegreeHints, uint32_t DeviceIdx = 0, bool mirror = false,
+               bool track_updates = false)
+      : GraphBase(N, TheEdgeAllocator, LoadFactor, VertexDegreeHints, DeviceIdx, false, mirror,
+                  track_updates) {}
 
   // SlabGraphImpl.cuh:22-34.
   void InsertEdges(VertexT *SourceVertices, VertexT *DestinationVertices, CountT NumberOfEdges) {

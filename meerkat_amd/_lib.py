@@ -85,6 +85,9 @@ _PROTOS = {
     "mk_graph_delete": [vp, vp, vp, u32, vp],
     "mk_graph_set_deferred": [vp, i32],
     "mk_graph_sync": [vp],
+    "mk_graph_update_weights": [vp, vp, vp, vp, u32, vp],
+    "mk_graph_update_slab_pointers": [vp],
+    "mk_graph_updated_edges": [vp, vp, vp, vp, u64, P(u64)],
     "mk_graph_search": [vp, vp, vp, u32, vp, vp, vp],
     "mk_graph_last_status": [vp, P(vp), P(u32)],
     "mk_graph_num_vertices": [vp, P(u32)],
@@ -111,6 +114,7 @@ _PROTOS = {
     "mk_sssp_static": [vp, P(f32)],
     "mk_sssp_incremental": [vp, vp, vp, vp, u32, P(f32)],
     "mk_sssp_decremental": [vp, vp, vp, vp, u32, P(f32)],
+    "mk_sssp_reweight": [vp, vp, vp, vp, u32, P(f32)],
     "mk_sssp_read": [vp, vp, vp],
     "mk_sssp_set_source": [vp, u32],
     "mk_sssp_last_stats": [vp, P(u32), P(u64), P(u64)],

@@ -27,6 +27,11 @@ class _Traversal:
         self._h = _vp()
         _lib.call(f"{self._prefix}_create", graph.handle, self.source, ctypes.byref(self._h))
 
+    @property
+    def handle(self):
+        """The C-ABI handle (mk_bfs* / mk_sssp*)."""
+        return self._h
+
     def Static(self, edges_n=0):
         ms = ctypes.c_float()
         _lib.call(f"{self._prefix}_static", self._h, ctypes.byref(ms))
@@ -128,6 +133,18 @@ class SSSP(_Traversal):
         _lib.call("mk_sssp_decremental", self._h, dptr(s), dptr(d), dptr(w), len(s),
                   ctypes.byref(ms))
         self._batch = (s, d, w)  # a deferred call reads it until it is settled
+        return ms.value
+
+    def Reweight(self, src, dst, weights):
+        """Distances kept exact after ``G.UpdateEdges`` with the same batch
+        (``weights``: its new weights): tree edges that became heavier are
+        handled as deletions, lighter ones as insertions."""
+        s, d = device_u32(src, self.G.device), device_u32(dst, self.G.device)
+        w = device_u32(weights, self.G.device)
+        ms = ctypes.c_float()
+        _lib.call("mk_sssp_reweight", self._h, dptr(s), dptr(d), dptr(w), len(s),
+                  ctypes.byref(ms))
+        self._batch = (s, d, w)
         return ms.value
 
 

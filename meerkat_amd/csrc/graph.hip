@@ -355,6 +355,56 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Weight update of a map graph (and of its mirror's entries): every batch
+// edge present gets the batch's weight (MK_EDGE_APPLIED); an absent edge is
+// not inserted (MK_EDGE_UNCHANGED); self-loops are skipped, as the
+// reference's UpdateEdgesKernel does (device/UpdateEdgeKernel.cuh:28).
+// Sides and groups as in k_insert.
+__global__ void __launch_bounds__(kBlock)
+    k_update_weights(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b,
+                     const uint32_t *src, const uint32_t *dst, const uint32_t *wt, uint32_t n,
+                     uint32_t epw, const uint32_t *bad, uint32_t epoch) {
+  const uint32_t lane = lane_id();
+  const uint32_t gl = group_lane();
+  const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const bool second = wave_all >= waves_a;
+  if (second && wave_all - waves_a >= waves_b) return;
+  const MutSide M = pick_side(second, A, B);
+  const uint32_t wave = second ? wave_all - waves_a : wave_all;
+  const uint64_t base = uint64_t(wave) * epw;
+  if (base >= n) return;
+  const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
+    if (lane < cnt) M.status[base + lane] = MK_EDGE_UNCHANGED;
+    return;
+  }
+  uint32_t s = 0, d = 0, w = 0;
+  bool act = false;
+  uint8_t st = MK_EDGE_UNCHANGED;
+  if (lane < cnt) {
+    s = src[base + lane];
+    d = dst[base + lane];
+    if (M.swap) {
+      const uint32_t t = s;
+      s = d;
+      d = t;
+    }
+    w = wt[base + lane];
+    act = s != d;
+    if (!act) st = MK_EDGE_SELF_LOOP;
+  }
+  const GraphView &G = M.G;
+  const uint32_t bk = act ? bucket_of(G, s, d) : 0u;
+  uint32_t pending = group_bits(ballot(act));
+  while (pending) {
+    const uint32_t j = ffs32(pending);
+    pending &= pending - 1;
+    const int r = slab_update_group(G, group_read(bk, j), group_read(d, j), group_read(w, j));
+    if (gl == j && r == 1) st = MK_EDGE_APPLIED;
+  }
+  if (lane < cnt) M.status[base + lane] = st;
+}
+
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_search(GraphView G, const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t epw,
@@ -565,6 +615,7 @@ int mk_pool_destroy(mk_pool *p) {
   (void)hipFree(p->view.slabs);
   (void)hipFree(p->view.bitmap);
   (void)hipFree(p->view.counters);
+  (void)hipFree(p->slab_new);
   delete p;
   return MK_OK;
 }
@@ -660,10 +711,18 @@ int mk_graph_create(mk_pool *pool, uint32_t n, int weighted, double load_factor,
 
 int mk_graph_create_ex(mk_pool *pool, uint32_t n, uint32_t flags, double load_factor,
                        const uint32_t *out_hints, const uint32_t *in_hints, mk_graph **out) {
-  if (flags & ~uint32_t(MK_GRAPH_WEIGHTED | MK_GRAPH_REVERSE))
+  if (flags & ~uint32_t(MK_GRAPH_WEIGHTED | MK_GRAPH_REVERSE | MK_GRAPH_TRACK_UPDATES))
     return fail(MK_ERR_BAD_ARG, "mk_graph_create_ex: unknown flag bits");
   const int weighted = (flags & MK_GRAPH_WEIGHTED) ? 1 : 0;
   if (int r = graph_alloc(pool, n, weighted, load_factor, out_hints, out)) return r;
+  if (flags & MK_GRAPH_TRACK_UPDATES) {
+    if (int r = track_enable(*out)) {
+      track_release(*out);
+      (void)mk_graph_destroy(*out);
+      *out = nullptr;
+      return r;
+    }
+  }
   if (flags & MK_GRAPH_REVERSE) {
     mk_graph *rev = nullptr;
     int r = graph_alloc(pool, n, weighted, load_factor, in_hints ? in_hints : out_hints, &rev);
@@ -703,6 +762,7 @@ int mk_graph_destroy(mk_graph *g) {
   (void)hipFree(g->bucket_owner);
   (void)hipFree(g->vdeg);
   (void)hipFree(g->counters);
+  track_release(g);
   if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
   if (g->mut_ev) (void)hipEventDestroy(g->mut_ev);
   if (g->trav_ev) (void)hipEventDestroy(g->trav_ev);
@@ -806,6 +866,22 @@ static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst,
               epw, only_status, bad, g->mut_epoch);
 }
 
+// Launch the weight-update kernel over g (and its mirror).
+static void launch_update_weights(mk_graph *g, const uint32_t *src, const uint32_t *dst,
+                                  const uint32_t *w, uint32_t n, hipStream_t s,
+                                  const uint32_t *bad) {
+  const uint32_t epw = edges_per_wave(n, g->device);
+  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
+  const MutSide a = side_of(g, false, g->counters),
+                b = g->rev ? side_of(g->rev, true, g->counters) : a;
+  const uint64_t total = g->rev ? 2 * waves : waves;
+  // per edge: batch entry (src, dst, weight), one slab probe, value write,
+  // status byte; the mirror side the same
+  prof_launch("k_update_weights", double(n) * (g->rev ? 2 : 1) * (12 + 256 + 4 + 1),
+              k_update_weights, dim3(blocks_for_waves(total)), dim3(kBlock), 0u, s, a, b,
+              uint32_t(waves), uint32_t(total - waves), src, dst, w, n, epw, bad, g->mut_epoch);
+}
+
 // Status buffer of the in-edge mirror for a batch of n edges.
 static int mirror_prologue(mk_graph *g, uint32_t n) {
   if (!g->rev) return MK_OK;
@@ -819,7 +895,9 @@ static int mirror_prologue(mk_graph *g, uint32_t n) {
 // after rolling the insert back on stream s, or MK_OK.
 static int mutation_outcome(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
                             uint32_t n, hipStream_t s) {
-  const char *who = op == 1 ? "mk_graph_insert" : "mk_graph_delete";
+  const char *who = op == 1 ? "mk_graph_insert"
+                   : op == 2 ? "mk_graph_delete"
+                             : "mk_graph_update_weights";
   if (int r = range_error(g, src, dst, n, s, who)) return r;
   const uint32_t pool_fail = op == 1 ? pool_failures(g) : 0u;  // both sides count here
   if (pool_fail) {
@@ -881,7 +959,7 @@ static int mutation_finish(mk_graph *g, int op, const uint32_t *src, const uint3
   g->last_op = op;
   g->last_src = src;
   g->last_dst = dst;
-  g->last_w = op == 1 ? w : nullptr;
+  g->last_w = op == 1 || op == 3 ? w : nullptr;
   if (g->deferred) {
     // BFS/SSSP calls run on own_stream: only a mutation on another stream
     // needs an event for them to wait on
@@ -908,6 +986,7 @@ int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const
   if (n == 0) return MK_OK;
   if (int r = mirror_prologue(g, n)) return r;
   launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
+  mark_new(g, src, dst, n, s);  // graphs tracking updates: the applied keys are new
   return mutation_finish(g, 1, src, dst, w, n, s);
 }
 
@@ -925,6 +1004,26 @@ int mk_graph_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint3
   if (int r = mirror_prologue(g, n)) return r;
   launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);
   return mutation_finish(g, 2, src, dst, nullptr, n, s);
+}
+
+int mk_graph_update_weights(mk_graph *g, const uint32_t *src, const uint32_t *dst,
+                            const uint32_t *w, uint32_t n, void *stream) {
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : (g ? g->own_stream : nullptr);
+  if (g && g->mirror_of)
+    return fail(MK_ERR_STATE, "mk_graph_update_weights: mirror is read-only");
+  if (g && !g->weighted)
+    return fail(MK_ERR_BAD_ARG, "mk_graph_update_weights: the graph is unweighted (a set)");
+  if (n && !w) return fail(MK_ERR_BAD_ARG, "mk_graph_update_weights: NULL weights");
+  if (int r = graph_settle(g)) return r;
+  if (int r = batch_prologue(g, src, dst, n, s, "mk_graph_update_weights")) return r;
+  if (n && s != g->own_stream) {  // after the work queued on own_stream (deferred calls)
+    MK_HIP(hipEventRecord(g->trav_ev, g->own_stream));
+    MK_HIP(hipStreamWaitEvent(s, g->trav_ev, 0));
+  }
+  if (n == 0) return MK_OK;
+  if (int r = mirror_prologue(g, n)) return r;
+  launch_update_weights(g, src, dst, w, n, s, g->counters);
+  return mutation_finish(g, 3, src, dst, w, n, s);
 }
 
 int mk_graph_set_deferred(mk_graph *g, int on) {

@@ -235,6 +235,12 @@ int graph_failed_batch(const mk_graph *g, const uint32_t *src, const uint32_t *d
 // (no other handle's kernels can hold compute units meanwhile), and
 // cooperatively otherwise.
 void loop_stream_add(int device, hipStream_t s, int delta);
+// Update-aware iteration (updated.hip): allocate a graph's new-key masks,
+// release them, and mark the keys an insert batch applied (statuses in
+// g->status) as new, on stream s (no-op unless the graph tracks updates).
+int track_enable(mk_graph *g);
+void track_release(mk_graph *g);
+void mark_new(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n, hipStream_t s);
 int device_cus(int device);
 int64_t tune_get(const char *name, int64_t dflt);
 // Grid-stride kernels: the number of waves to launch on `device`.
@@ -272,6 +278,10 @@ struct mk_pool {
   mk::PoolView view{};
   uint64_t total_slabs = 0;
   int graphs = 0;
+  // per pool slab, the mask of its words holding keys inserted since the
+  // owning graph's last UpdateSlabPointers (allocated with the first graph
+  // created with MK_GRAPH_TRACK_UPDATES; updated.hip)
+  uint64_t *slab_new = nullptr;
 };
 
 struct mk_graph {
@@ -325,6 +335,11 @@ struct mk_graph {
   } failed = {0, nullptr, nullptr, 0, std::string()};
   mk::HostBuf host_scratch;
   int users = 0;                 // algorithm handles attached
+  // update-aware iteration (MK_GRAPH_TRACK_UPDATES; updated.hip): per
+  // bucket, the new-key mask of its head slab and an "inserted into" flag
+  bool track = false;
+  uint64_t *head_new = nullptr;
+  uint8_t *bucket_upd = nullptr;
   mk_graph *rev = nullptr;       // in-edge mirror (MK_GRAPH_REVERSE), or null
   mk_graph *mirror_of = nullptr; // set on a mirror: the graph that owns it
   mk::GraphView view() const {

@@ -387,6 +387,33 @@ MK_DEV int slab_delete_group(const GraphView &G, uint32_t bucket, uint32_t dst) 
   }
 }
 
+// Map value update for one group: key dst's value word becomes w (key
+// absent: nothing is written).  Returns 1 if the key is present.  Updates
+// of one batch run concurrently; a batch naming a key twice keeps one of
+// its values.
+MK_DEV int slab_update_group(const GraphView &G, uint32_t bucket, uint32_t dst, uint32_t w) {
+  const uint32_t gl = group_lane();
+  uint32_t *slab = G.heads + size_t(bucket) * kSlabWords;
+  for (;;) {
+    uint32_t v[kWordsPerLane];
+    load_words_coherent(slab + kWordsPerLane * gl, v);
+    uint32_t hj = kWordsPerLane;
+#pragma unroll
+    for (int j = kWordsPerLane - 1; j >= 0; --j)
+      if (key_word<true>(kWordsPerLane * gl + j) && v[j] == dst) hj = uint32_t(j);
+    const uint32_t hb = group_bits(ballot(hj < kWordsPerLane));
+    if (hb) {
+      const uint32_t hl = ffs32(hb);
+      const uint32_t word = kWordsPerLane * hl + group_read(hj, hl);
+      if (gl == 0) coherent_store(slab + word + 1, w);
+      return 1;
+    }
+    const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);
+    if (nx == kEmpty) return 0;
+    slab = slab_ptr(G.pool, nx);
+  }
+}
+
 // slab_search for one group (no concurrent writers).  Returns 1 if present
 // and sets *w (map value, or 0).
 template <bool MAP>

@@ -1153,10 +1153,14 @@ __global__ void __launch_bounds__(kItemBlock)
 // v becomes invalid (Invalidate, BFSImpl.cuh:3-15).  With `collect`, such a
 // v (an orphan) gets its invalid bit and is appended once to S.inv_list
 // instead (count S.cnt->ninv) for k_repair, which looks for another parent
-// before invalidating.
+// before invalidating.  With `wts` (a weight update: edge (u, v) now weighs
+// wts[i]) a tree edge breaks only if it became heavier, dist[u] + wts[i] >
+// dist[v]; one that kept or lowered v's distance is left to the relaxation
+// that follows.  (A u invalidated concurrently reaches v through the
+// invalidation walk, v being its tree child.)
 __global__ void __launch_bounds__(kItemBlock)
     k_invalidate_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
-                       uint32_t n, uint32_t source, uint32_t collect) {
+                       const uint32_t *wts, uint32_t n, uint32_t source, uint32_t collect) {
   __shared__ uint32_t sh[kItemBlock / 64 + 1];
   const uint32_t lane = lane_id();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1165,9 +1169,15 @@ __global__ void __launch_bounds__(kItemBlock)
   if (i < n) {
     const uint32_t u = src[i];
     v = dst[i];
+    const uint32_t wn = wts ? wts[i] : 0u;
     if (u != v && u < G.n && v < G.n && v != source) {
       const uint64_t cur = S.dist[v];
-      if (cur != kInfInf && parent_of(cur) == u) {
+      bool broken = cur != kInfInf && parent_of(cur) == u;
+      if (broken && wts) {
+        const uint32_t du = dist_of(S.dist[u]);
+        broken = du != kInf && uint64_t(du) + wn > uint64_t(dist_of(cur));
+      }
+      if (broken) {
         if (collect) {
           // the invalid bit marks v as listed: a batch naming (u, v) twice
           // lists v once (the list holds at most n vertices)
@@ -1919,7 +1929,24 @@ struct Traversal {
   // loop, the re-seed and the relax loop are all queued, then end() waits
   // once.  The re-seed kernels return at once when nothing was invalidated
   // (no invalidation step ran), as the host-driven path skips them.
-  int queue_decremental(float *ms) {
+  // A weight update (`rw`: the batch and its new weights) also relaxes
+  // every batch edge from its source before the relax loop, into the same
+  // frontier as the re-seeded vertices.
+  struct Reweight {
+    const uint32_t *src, *dst;
+    uint32_t n;
+  };
+  void seed_reweighted(const Reweight &rw, uint32_t ep) {
+    // weights are read from the graph (status NULL), which holds the
+    // update's outcome: a batch naming an edge twice keeps one weight
+    const unsigned grid = unsigned((uint64_t(rw.n) + kItemBlock - 1) / kItemBlock);
+    prof_launch("k_seed_batch", 24.0 * rw.n, k_seed_batch<true, true>, dim3(grid),
+                dim3(kItemBlock), 0u, stream, g->view(), view(), rw.src, rw.dst,
+                static_cast<const uint32_t *>(nullptr), static_cast<const uint8_t *>(nullptr),
+                rw.n, ep);
+  }
+
+  int queue_decremental(float *ms, const Reweight *rw = nullptr) {
     const GraphView G = g->view();
     if (int r = queue_invalidate()) return r;
     const uint32_t ep = next_epoch();
@@ -1944,13 +1971,20 @@ struct Traversal {
       prof_launch("k_reseed", 0.0, k, dim3(grid), dim3(kBlock), 0u, stream, G, view(), ep);
       reseed_kind = 2;
     }
+    if (rw) seed_reweighted(*rw, ep);
     MK_HIP(hipGetLastError());
     if (int r = queue_relax(invbits, nwords)) return r;
     return end(ms, true);
   }
 
-  int run_decremental(const uint32_t *src, const uint32_t *dst, uint32_t n, float *ms) {
+  // Decremental (after DeleteEdges with the batch), or with `wts` a weight
+  // update (after mk_graph_update_weights with the batch): tree edges that
+  // became heavier invalidate their subtrees, which are re-seeded, and
+  // every batch edge is relaxed with its weight in the graph.
+  int run_decremental(const uint32_t *src, const uint32_t *dst, uint32_t n, float *ms,
+                      const uint32_t *wts = nullptr) {
     if (n && (!src || !dst)) return fail(MK_ERR_BAD_ARG, "decremental: NULL batch");
+    if (n && wts && !weights) return fail(MK_ERR_BAD_ARG, "weight update: not an SSSP handle");
     if (int r = settle()) return r;
     if (int r = graph_failed_batch(g, src, dst, n)) {
       stale = true;
@@ -1965,8 +1999,9 @@ struct Traversal {
         const unsigned grid = unsigned((uint64_t(n) + kItemBlock - 1) / kItemBlock);
         // with the in-edge mirror, orphans are first offered another parent
         const uint32_t collect = g->rev && repair ? 1u : 0u;
-        prof_launch("k_invalidate_batch", 24.0 * n, k_invalidate_batch, dim3(grid),
-                    dim3(kItemBlock), 0u, stream, G, view(), src, dst, n, source, collect);
+        prof_launch("k_invalidate_batch", (wts ? 36.0 : 24.0) * n, k_invalidate_batch,
+                    dim3(grid), dim3(kItemBlock), 0u, stream, G, view(), src, dst, wts, n, source,
+                    collect);
         if (collect) {
           auto k = g->weighted ? (weights ? k_repair<true, true> : k_repair<true, false>)
                                : k_repair<false, false>;
@@ -1975,12 +2010,20 @@ struct Traversal {
         }
       }
       MK_HIP(hipGetLastError());
-      if (persistent) return queue_decremental(ms);
+      const Reweight rw{src, dst, n};
+      if (persistent) return queue_decremental(ms, wts ? &rw : nullptr);
       // propagate invalidation down the tree; any step ran = something was
       // invalidated
       Totals h;
       if (int r = iterate<1>(h)) return r;
-      if (h.steps != 0) {
+      if (h.steps == 0) {
+        if (wts) {  // nothing invalidated: only the reweighted edges to relax
+          clear_slots();
+          seed_reweighted(rw, next_epoch());
+          MK_HIP(hipGetLastError());
+          if (int r = iterate<0>(h)) return r;
+        }
+      } else {
         // re-seed from valid in-neighbours; the re-seeded vertices start a
         // fresh frontier at side 0 / slot 0
         const uint32_t ep = next_epoch();
@@ -2041,6 +2084,8 @@ struct Traversal {
                                          16.0 * double(a.pushed - pushed0));
         }
         fill_u32(invbits, 0u, (uint64_t(g->n) + 31) / 32, stream);
+        if (wts) seed_reweighted(rw, ep);
+        MK_HIP(hipGetLastError());
         if (int r = iterate<0>(h)) return r;
       }
     }
@@ -2050,19 +2095,28 @@ struct Traversal {
   int read(uint32_t *d, uint32_t *p) {
     MK_HIP(hipSetDevice(g->device));
     if (int r = settle()) return r;
-    const size_t n = g->n;
     DevBuf b;
-    if (int r = b.ensure(n * 8)) return r;
-    uint32_t *dd = b.as<uint32_t>(), *pp = dd + n;
-    k_unpack<<<blocks_for_threads(n), kBlock, 0, stream>>>(dist, g->n, dd, pp);
-    hipError_t e = hipStreamSynchronize(stream);
-    if (e == hipSuccess && d) e = hipMemcpy(d, dd, n * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && p) e = hipMemcpy(p, pp, n * 4, hipMemcpyDeviceToHost);
+    if (int r = b.ensure(size_t(g->n) * 8)) return r;
+    const int r = unpack_to_host(b.as<uint32_t>(), d, p);
     b.release();
+    return r;
+  }
+
+  // Distances and parents unpacked into `staging` (2n words of device
+  // memory) on the handle's stream, then copied to the host.
+  int unpack_to_host(uint32_t *staging, uint32_t *d, uint32_t *p) {
+    const size_t n = g->n;
+    uint32_t *dd = staging, *pp = staging + n;
+    k_unpack<<<blocks_for_threads(n), kBlock, 0, stream>>>(dist, g->n, dd, pp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && d) e = hipMemcpyAsync(d, dd, n * 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess && p) e = hipMemcpyAsync(p, pp, n * 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "read distances");
     return MK_OK;
   }
 };
+
 
 }  // namespace mk
 
@@ -2166,6 +2220,12 @@ int mk_sssp_decremental(mk_sssp *s, const uint32_t *src, const uint32_t *dst, co
   (void)w;
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_decremental: NULL handle");
   return s->t.run_decremental(src, dst, n, ms);
+}
+int mk_sssp_reweight(mk_sssp *s, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                     uint32_t n, float *ms) {
+  if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_reweight: NULL handle");
+  if (n && !w) return fail(MK_ERR_BAD_ARG, "mk_sssp_reweight: NULL weights");
+  return s->t.run_decremental(src, dst, n, ms, w);
 }
 int mk_sssp_read(mk_sssp *s, uint32_t *d, uint32_t *p) {
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_read: NULL handle");

@@ -80,11 +80,13 @@ class DynamicGraph:
 
     WEIGHTED = 1
     REVERSE = 2
+    TRACK_UPDATES = 4
 
     def __init__(self, n, pool, load_factor=0.7, degree_hints=None, weighted=False, device=0,
-                 reverse=False, in_degree_hints=None):
+                 reverse=False, in_degree_hints=None, track_updates=False):
         """reverse=True also keeps an in-edge mirror, updated by every
-        InsertEdges/DeleteEdges, which decremental BFS/SSSP pull from."""
+        InsertEdges/DeleteEdges, which decremental BFS/SSSP pull from.
+        track_updates=True marks inserted keys for UpdatedEdges()."""
         self.n = int(n)
         self.pool = pool
         self.weighted = bool(weighted)
@@ -103,7 +105,9 @@ class DynamicGraph:
 
         hints, ihints = _hint(degree_hints), _hint(in_degree_hints)
         self._hints = (hints, ihints)
-        flags = (self.WEIGHTED if self.weighted else 0) | (self.REVERSE if self.reverse else 0)
+        self.track_updates = bool(track_updates)
+        flags = ((self.WEIGHTED if self.weighted else 0) | (self.REVERSE if self.reverse else 0) |
+                 (self.TRACK_UPDATES if self.track_updates else 0))
         _lib.call("mk_graph_create_ex", pool.handle, self.n, flags, float(load_factor),
                   None if hints is None else hints.ctypes.data_as(_vp),
                   None if ihints is None else ihints.ctypes.data_as(_vp), ctypes.byref(self._h))
@@ -135,6 +139,42 @@ class DynamicGraph:
         _lib.call("mk_graph_delete", self._h, dptr(s), dptr(d), len(s), stream)
         self._batch = (s, d)
         return s, d
+
+    def UpdateEdges(self, src, dst, weights, stream=None):
+        """Set the weight of every present edge of the batch (an absent edge
+        is not inserted; last_status() tells which were present).  The
+        reference's UpdateEdges (include/Graph/SlabGraphImpl.cuh:84-103,
+        compiled out there) without its FilterMaps."""
+        s, d = device_u32(src, self.device), device_u32(dst, self.device)
+        w = device_u32(weights, self.device)
+        _lib.call("mk_graph_update_weights", self._h, dptr(s), dptr(d), dptr(w), len(s), stream)
+        self._batch = (s, d, w)
+        return s, d, w
+
+    update_edges = UpdateEdges
+
+    def UpdateSlabPointers(self):
+        """Edges inserted so far stop being reported by UpdatedEdges()
+        (include/Graph/SlabGraphImpl.cuh:64-82)."""
+        _lib.call("mk_graph_update_slab_pointers", self._h)
+
+    def UpdatedEdges(self):
+        """Device tensors (src, dst, w or None) of the edges inserted since
+        the last UpdateSlabPointers and still present, in no particular
+        order; the graph must have been created with track_updates=True."""
+        torch = _torch()
+        cnt = ctypes.c_uint64()
+        _lib.call("mk_graph_updated_edges", self._h, None, None, None, 0, ctypes.byref(cnt))
+        k = cnt.value
+        dev = f"cuda:{self.device}"
+        s = torch.empty(max(k, 1), dtype=torch.int32, device=dev)
+        d = torch.empty(max(k, 1), dtype=torch.int32, device=dev)
+        w = torch.empty(max(k, 1), dtype=torch.int32, device=dev) if self.weighted else None
+        _lib.call("mk_graph_updated_edges", self._h, dptr(s), dptr(d), dptr(w), k,
+                  ctypes.byref(cnt))
+        if cnt.value != k:
+            raise RuntimeError("updated edges changed between the two passes")
+        return s[:k], d[:k], (None if w is None else w[:k])
 
     def set_deferred(self, on=True):
         """Deferred mutation checks: InsertEdges / DeleteEdges return once the

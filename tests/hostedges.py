@@ -19,7 +19,7 @@ class HostEdges:
         self.n = n
         self.keys = gen.keys_of(src, dst)
         assert np.all(self.keys[1:] > self.keys[:-1]), "initial edges must be sorted and unique"
-        self.w = None if w is None else np.asarray(w, np.uint32)
+        self.w = None if w is None else np.array(w, np.uint32)  # a copy: weights change
         self.removed = np.zeros(len(self.keys), bool)
         self.added = {}  # key -> weight, edges not in the initial set
 
@@ -47,6 +47,19 @@ class HostEdges:
             if a == b or key in self.added or (h and not self.removed[p]):
                 continue
             self.added[key] = wt
+
+    def update_weights(self, src, dst, w):
+        """Present edges take the new weight (a batch naming an edge twice
+        must give it one weight); absent edges are not inserted."""
+        src, dst = np.asarray(src, np.uint32), np.asarray(dst, np.uint32)
+        k = gen.keys_of(src, dst)
+        pos, hit = self._lookup(k)
+        for key, p, h, wt in zip(k.tolist(), pos.tolist(), hit.tolist(),
+                                 np.asarray(w, np.uint32).tolist()):
+            if key in self.added:
+                self.added[key] = wt
+            elif h and not self.removed[p]:
+                self.w[p] = wt
 
     def csr(self):
         keep = ~self.removed
