@@ -8,6 +8,10 @@ handles created afterwards:
 - ``ms_unroll`` (4 or 8, default 8): neighbour distance rows each wave keeps
   in flight in multi-source SSSP (more rows hide more HBM latency at the
   cost of registers).
+- ``pull_enter`` / ``pull_exit`` (defaults 14 / 24): the push/pull
+  thresholds of static BFS over a graph with the in-edge mirror; a step runs
+  bottom-up once the frontier holds more than 1/pull_enter of the buckets
+  (0: never) and until it holds fewer than 1/pull_exit of the vertices.
 
 The reference fixes its launch shapes at compile time (e.g.
 include/Algorithms/impl/BFSImpl.cuh:620-625); these knobs exist because the
@@ -19,7 +23,8 @@ from . import _lib
 
 KNOBS = {"waves_per_cu": (1, 64), "ms_unroll": (4, 8), "orphan_repair": (0, 1),
          "persistent_steps": (0, 1), "tail_enter": (0, 1024), "coop_launch": (0, 1),
-         "mut_waves_per_cu": (1, 256), "barrier_timeout_ms": (0, 600000)}
+         "mut_waves_per_cu": (1, 256), "barrier_timeout_ms": (0, 600000),
+         "pull_enter": (0, 4096), "pull_exit": (1, 4096)}
 
 
 def set_knob(name, value):

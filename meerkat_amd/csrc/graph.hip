@@ -57,6 +57,11 @@ static std::map<std::string, int64_t> g_tune = {
     {"coop_launch", 0},     // 1: frontier loops always launched cooperatively
     {"mut_waves_per_cu", 32},  // waves per CU the mutation kernels' batch is spread over
     {"barrier_timeout_ms", 5000},  // longest grid-barrier wait of a frontier loop
+    // static BFS over a graph with the in-edge mirror: bottom-up steps once
+    // the frontier holds more than 1/pull_enter of the buckets (0: never),
+    // until it holds fewer than 1/pull_exit of the vertices
+    {"pull_enter", 14},
+    {"pull_exit", 24},
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {
@@ -207,7 +212,12 @@ struct MutSide {
   uint8_t *status;
   uint32_t *counters;
   uint32_t swap;
-  uint32_t degrees;  // maintain G.vdeg (the mirror's in-degrees are counted on request)
+  // G.vdeg upkeep: 1 = live out-degree (atomic add / subtract); 2 = the
+  // in-edge mirror's "ever inserted into" flag (set to 1 on an applied
+  // insert, never cleared: a superset of the vertices with in-edges, read by
+  // bottom-up BFS steps to skip vertices without any; the mirror's live
+  // in-degrees are counted on request)
+  uint32_t degrees;
 };
 
 MK_DEV MutSide pick_side(bool second, const MutSide &A, const MutSide &B) {
@@ -285,7 +295,8 @@ __global__ void __launch_bounds__(kBlock)
       else if (r < 0) st = MK_EDGE_POOL_FULL;  // rolled back by the host
     }
     if (r == 1) {
-      if (gl == 0 && M.degrees) atomicAdd(G.vdeg + sj, 1u);
+      if (gl == 0 && M.degrees == 1) atomicAdd(G.vdeg + sj, 1u);
+      else if (gl == 0 && M.degrees == 2) G.vdeg[sj] = 1u;
     } else if (r < 0) {
       ++pool_fail;
     }
@@ -346,7 +357,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t sj = group_read(s, j), dj = group_read(d, j);
     const int r = slab_delete_group<MAP>(G, group_read(bk, j), dj);
     if (gl == j && r == 1) st = MK_EDGE_APPLIED;
-    if (r == 1 && gl == 0 && M.degrees) atomicSub(G.vdeg + sj, 1u);
+    if (r == 1 && gl == 0 && M.degrees == 1) atomicSub(G.vdeg + sj, 1u);
   }
   if (only_status != 0xFFu) {
     if (lane < cnt && act) M.status[base + lane] = MK_EDGE_UNCHANGED;
@@ -550,6 +561,9 @@ int mk_tune_set(const char *name, int64_t value) {
     return fail(MK_ERR_BAD_ARG, "mut_waves_per_cu must be in [1, 256]");
   if (k == "tail_enter" && (value < 0 || value > 1024))
     return fail(MK_ERR_BAD_ARG, "tail_enter must be in [0, 1024]");
+  if ((k == "pull_enter" || k == "pull_exit") && (value < 0 || value > 4096))
+    return fail(MK_ERR_BAD_ARG, k + " must be in [0, 4096]");
+  if (k == "pull_exit" && value == 0) return fail(MK_ERR_BAD_ARG, "pull_exit must be >= 1");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(k);
   if (it == g_tune.end()) return fail(MK_ERR_BAD_ARG, "mk_tune_set: unknown knob " + k);
@@ -821,11 +835,11 @@ static uint32_t pool_failures(mk_graph *g) {
 
 // The mutation side of graph g (its own view and status; the counters of
 // the graph the batch was given to, `counters`).  The mirror keeps no
-// degree array up to date: no kernel reads it, and mk_graph_degrees counts
-// its live keys instead.
+// live degree array (mk_graph_degrees counts its live keys instead), only
+// the flag of vertices that were ever given an in-edge.
 static MutSide side_of(mk_graph *g, bool swap, uint32_t *counters) {
   return MutSide{g->view(), g->status.as<uint8_t>(), counters, swap ? 1u : 0u,
-                 g->mirror_of ? 0u : 1u};
+                 g->mirror_of ? 2u : 1u};
 }
 
 // Launch the insert kernel over g and, when it has one, its mirror (one

@@ -45,6 +45,8 @@ struct MsCounts {
   uint32_t overflow;               // log capacity exceeded
   uint32_t pad;
   unsigned long long edges;        // adjacency entries read
+  unsigned long long slabs;        // slabs read
+  unsigned long long pairs;        // SSSP: (entry, active source) relaxations examined
 };
 
 struct MsView {
@@ -132,8 +134,10 @@ MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, MsQ &q, uint64_t key
 
 template <bool MAP>
 MK_DEV void msbfs_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint64_t mu,
-                        uint64_t *Fn, uint32_t nbase, uint32_t nslot, uint32_t &edges) {
+                        uint64_t *Fn, uint32_t nbase, uint32_t nslot, uint32_t &edges,
+                        uint32_t &slabs) {
   for (;;) {
+    ++slabs;
     msbfs_slab(G, S, q, key_mask<MAP>(), x, mu, Fn, nbase, nslot, edges);
     const uint32_t nx = read_lane(x, kNextLane);
     if (nx == kEmpty) break;
@@ -173,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
   }
   __shared__ uint32_t lqbuf[kBlock / 64][kMsWaveQ];
   MsQ q{lqbuf[threadIdx.x >> 6], 0u};
-  uint32_t edges = 0;
+  uint32_t edges = 0, slabs = 0;
   // hubs: their list entries
   const uint2 *ql = S.ql[slot];
   // lane l takes bucket l % kHubChunk of hub-list entry l / kHubChunk
@@ -207,7 +211,8 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
         x[qq] = qq < c ? G.heads[size_t(read_lane(b, js[qq])) * kSlabWords + lane] : kEmpty;
 #pragma unroll
       for (uint32_t qq = 0; qq < 4; ++qq)
-        if (qq < c) msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges);
+        if (qq < c)
+          msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges, slabs);
     }
   }
   // log entries: record the mask; expand non-hub vertices
@@ -245,12 +250,14 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
 #pragma unroll
         for (uint32_t qq = 0; qq < 4; ++qq)
           if (qq < c)
-            msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges);
+            msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges,
+                             slabs);
       }
     }
   }
   ms_flush(S, q, S.logv, nbase, &S.c->cnt[nslot], S.log_cap);
   if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
+  if (lane == 0 && slabs) atomicAdd(&S.c->slabs, (unsigned long long)slabs);
 }
 
 __global__ void k_msbfs_init(MsView S, const uint32_t *sources, uint32_t k) {
@@ -305,11 +312,13 @@ constexpr uint32_t kMsSparse = 4;
 
 template <bool MAP, uint32_t U>
 MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t du_lane,
-                      uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
+                      uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges,
+                      uint64_t &pairs) {
   const uint32_t lane = lane_id();
   const uint64_t km = key_mask<MAP>();
   uint64_t live = ballot(x < kTomb) & km;
   edges += uint32_t(__popcll(live));
+  pairs += uint64_t(__popcll(live)) * uint64_t(__popcll(mu));
   if (uint32_t(__popcll(mu)) <= kMsSparse) {
     // key-parallel: lane j relaxes key j of this slab for each active source
     const bool lv = (live >> lane) & 1ull;
@@ -389,10 +398,12 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
 
 template <bool MAP, uint32_t U>
 MK_DEV void mssp_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t u,
-                       uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges) {
+                       uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges,
+                       uint32_t &slabs, uint64_t &pairs) {
   const uint32_t du = S.dist[size_t(u) * 64 + lane_id()];
   for (;;) {
-    mssp_slab<MAP, U>(G, S, q, x, du, mu, Fn, qn, nslot, edges);
+    ++slabs;
+    mssp_slab<MAP, U>(G, S, q, x, du, mu, Fn, qn, nslot, edges, pairs);
     const uint32_t nx = read_lane(x, kNextLane);
     if (nx == kEmpty) break;
     x = slab_ptr(G.pool, nx)[lane_id()];
@@ -428,7 +439,8 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&S.c->steps, 1u);
   __shared__ uint32_t lqbuf[kBlock / 64][kMsWaveQ];
   MsQ q{lqbuf[threadIdx.x >> 6], 0u};
-  uint32_t edges = 0;
+  uint32_t edges = 0, slabs = 0;
+  uint64_t pairs = 0;
   const uint2 *ql = S.ql[slot];
   // lane l takes bucket l % kHubChunk of hub-list entry l / kHubChunk
   const uint32_t eper = min(per_wave(n_large, W), 64u / kHubChunk);
@@ -464,7 +476,7 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
       for (uint32_t qq = 0; qq < 4; ++qq)
         if (qq < c)
           mssp_chain<MAP, U>(G, S, q, x[qq], read_lane(u, js[qq]), read_lane64(mu, js[qq]), Fn,
-                             qn, nslot, edges);
+                             qn, nslot, edges, slabs, pairs);
     }
   }
   const uint32_t per = per_wave(count, W);
@@ -486,12 +498,15 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
         const uint32_t j = ffs64(m);
         m &= m - 1;
         const uint32_t x = G.heads[size_t(read_lane(b0, j) + kk) * kSlabWords + lane];
-        mssp_chain<MAP, U>(G, S, q, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges);
+        mssp_chain<MAP, U>(G, S, q, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges,
+                           slabs, pairs);
       }
     }
   }
   ms_flush(S, q, qn, 0, &S.c->cnt[nslot], S.n);
   if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
+  if (lane == 0 && slabs) atomicAdd(&S.c->slabs, (unsigned long long)slabs);
+  if (lane == 0 && pairs) atomicAdd(&S.c->pairs, (unsigned long long)pairs);
 }
 
 __global__ void k_mssp_init(MsView S, const uint32_t *sources, uint32_t k) {
@@ -689,6 +704,12 @@ struct MultiSource {
       k = nsrc;
       last_steps = h.steps;
       last_edges = h.edges;
+      if (profiling_on())
+        // BFS: 256 B per slab, 8 B per entry (the neighbour's seen mask);
+        // SSSP: 256 B per slab, 4 B per (entry, active source) distance
+        prof_add_bytes(sssp ? "k_mssp_step" : "k_msbfs_level",
+                       256.0 * double(h.slabs) +
+                           (sssp ? 4.0 * double(h.pairs) : 8.0 * double(h.edges)));
       if (!sssp) {
         // levels 0..steps-1 are non-empty; seg[steps] is the log length
         uint32_t segs[1];

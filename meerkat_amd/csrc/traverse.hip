@@ -67,6 +67,14 @@ struct Counts {
   uint32_t abort;                 // set when a grid-barrier wait timed out
   uint32_t loop_end[2];           // step index at which k_expand_loop<MODE> stopped
   uint32_t bar_done;              // grid barriers completed (k_expand_loop)
+  // near-far static SSSP (S.delta != 0): entries of the three rotating far
+  // lists, the complement of each list's smallest distance (atomicMax of
+  // ~d: zero means empty), and the near bound and split count a relax loop
+  // leaves for the launch that continues it (0: a fresh start)
+  uint32_t far_cnt[3];
+  uint32_t far_minx[3];
+  uint32_t nf_T, nf_split;
+  uint32_t far_over;              // a deferred hub did not fit its far list
   unsigned long long large[3];    // (entries << 32) | bucket items
   unsigned long long pull_large;  // k_reseed_pull's hub list (entries << 32) | items
   unsigned long long stats[kStatSlots][16];  // [slot][set * 4 + which], one line per slot
@@ -186,7 +194,59 @@ struct TravView {
   struct BarLines *bar;  // grid-barrier counters (k_expand_loop)
   uint32_t tail_enter, tail_stay;  // tail-mode thresholds (0: no tail mode)
   uint32_t trace;                  // record the step trace (profiling)
+  // Bottom-up (pull) steps of a static BFS over the in-edge mirror R
+  // (vbits[0] == nullptr: none).  vbits[side] marks the vertices pushed to
+  // queue side `side` (the frontier the step reading that side expands);
+  // a bitmap also keeps bits of earlier frontiers of the same parity, which
+  // is harmless: an unvisited vertex has no in-neighbour at a level below
+  // the current one.  A step runs bottom-up once the frontier holds more
+  // than 1/pull_enter of the graph's buckets, and stays so until it holds
+  // fewer than 1/pull_exit of the vertices.
+  GraphView R;
+  uint32_t *vbits[2];
+  uint32_t pull_enter, pull_exit;
+  // Near-far static SSSP (delta == 0: plain label-correcting).  A frontier
+  // vertex whose distance is at or above the near bound T is not expanded
+  // but deferred to the current far list (vertex, distance); once the near
+  // frontier is empty, a split step raises T to the smallest deferred
+  // distance + delta and queues the far entries now below it (entries whose
+  // vertex has since improved are dropped: a later entry stands for it).
+  // Label-correcting is exact in any order, so a full far list only means
+  // expanding a vertex early.
+  uint2 *far[3];
+  uint32_t far_cap;
+  uint32_t delta;
 };
+
+MK_DEV uint32_t wave_min_u32(uint32_t x) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t o = uint32_t(__shfl_xor(int(x), off, 64));
+    x = o < x ? o : x;
+  }
+  return x;
+}
+
+// Append (v, d) of the lanes with `put` to far list `f`, one atomic per
+// wave; returns whether this lane's entry was stored (false past far_cap).
+// Wave-collective.
+MK_DEV bool far_append(const TravView &S, uint32_t f, bool put, uint32_t v, uint32_t d) {
+  const uint64_t m = ballot(put);
+  if (m == 0) return false;
+  const uint32_t lane = lane_id(), leader = ffs64(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&S.cnt->far_cnt[f], uint32_t(__popcll(m)));
+  base = read_lane(base, leader);
+  const uint32_t pos = base + lanes_below(m);
+  const bool stored = put && pos < S.far_cap;
+  if (stored) qstore2(S.far[f] + pos, make_uint2(v, d));
+  const uint32_t mn = wave_min_u32(stored ? d : kInf);
+  if (lane == leader && mn != kInf) atomicMax(&S.cnt->far_minx[f], ~mn);
+  return stored;
+}
+
+MK_DEV void mark_frontier(const TravView &S, uint32_t side, bool push, uint32_t v) {
+  if (push && S.vbits[0]) atomicOr(S.vbits[side] + (v >> 5), 1u << (v & 31u));
+}
 
 // Append v (lanes with `push`) to queue side `side`, counter slot `slot`:
 // small vertices through the wave's LDS staging, hub vertices directly.
@@ -195,6 +255,7 @@ MK_DEV void enqueue_nb(const GraphView &G, const TravView &S, WaveQ &q, uint32_t
                        uint32_t slot, bool push, uint32_t v, uint32_t nb) {
   const uint64_t pm = ballot(push);
   if (pm == 0) return;
+  mark_frontier(S, side, push, v);
   const bool small = push && nb <= kSmallBuckets;
   const uint64_t sm = ballot(small);
   const uint32_t c = uint32_t(__popcll(sm));
@@ -276,6 +337,7 @@ struct TailSink {
   MK_DEV void operator()(const GraphView &G, bool push, uint32_t v, uint32_t nb) {
     const uint64_t pm = ballot(push);
     if (pm == 0) return;
+    mark_frontier(S, side, push, v);
     const uint32_t lane = lane_id();
     const bool small = push && nb <= kSmallBuckets;
     const uint64_t sm = ballot(small);
@@ -334,6 +396,11 @@ MK_DEV void block_enqueue(const GraphView &G, const TravView &S, uint32_t side, 
   push_hubs(G, S.q_large[side], &S.cnt->large[slot], push && !small, v, nb);
   const uint64_t pm = ballot(push);
   if (lane == 0) add_stat(S.cnt, kSetSeed, kStatPushed, uint32_t(__popcll(pm)));
+}
+
+MK_DEV uint32_t wave_sum(uint32_t x) {
+  for (int off = 32; off >= 1; off >>= 1) x += uint32_t(__shfl_xor(int(x), off, 64));
+  return x;
 }
 
 MK_DEV bool inv_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> (v & 31u)) & 1u; }
@@ -505,7 +572,8 @@ MK_DEV void batched_buckets(const GraphView &G, const TravView &S, Sink &sink, u
 template <bool MAP, bool WEIGHTS, int MODE, typename Sink>
 MK_DEV void expand_rows(const GraphView &G, const TravView &S, Sink &sink, uint32_t epoch,
                         const uint32_t *queue, uint32_t n, uint32_t wave, uint32_t NW,
-                        uint32_t &slabs, uint32_t &edges, uint32_t &visited) {
+                        uint32_t &slabs, uint32_t &edges, uint32_t &visited, uint32_t T = kInf,
+                        uint32_t fcur = 0) {
   const uint32_t lane = lane_id();
   const uint32_t per = per_wave(n, NW);
   for (uint32_t i0 = wave * per; i0 < n; i0 += NW * per) {
@@ -518,6 +586,10 @@ MK_DEV void expand_rows(const GraphView &G, const TravView &S, Sink &sink, uint3
       nb = G.bucket_off[u + 1] - b0;
       du = MODE == 1 ? 0u : dist_of(coherent_load64(S.dist + u));
       if (MODE == 0 && du == kInf) act = false;
+    }
+    if constexpr (MODE == 0 && WEIGHTS) {
+      // near-far: vertices at or past the near bound wait in the far list
+      if (S.delta && far_append(S, fcur, act && du >= T, u, du)) act = false;
     }
     visited += uint32_t(__popcll(ballot(act)));
     batched_buckets<MAP, WEIGHTS, MODE>(G, S, sink, epoch, ballot(act), b0, u, du, slabs, edges);
@@ -536,7 +608,8 @@ MK_DEV void expand_rows(const GraphView &G, const TravView &S, Sink &sink, uint3
 // the epoch range reserved for this launch runs out.
 template <bool MAP, bool WEIGHTS, int MODE>
 MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
-                        uint32_t epoch, uint32_t n0, uint32_t *lds) {
+                        uint32_t epoch, uint32_t n0, uint32_t *lds, uint32_t T = kInf,
+                        uint32_t fcur = 0) {
   uint32_t *qa = lds, *qb = lds + kTailCap, *ctl = lds + 2 * kTailCap;  // ctl: 2 x {count, flag}
   const uint32_t nside = side ^ 1u, nslot = slot == 2 ? 0 : slot + 1;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, NW = blockDim.x >> 6;
@@ -552,7 +625,7 @@ MK_DEV void expand_tail(const GraphView &G, const TravView &S, uint32_t side, ui
     }
     TailSink sink{S, qb, c, nside, nslot, 0u};
     expand_rows<MAP, WEIGHTS, MODE>(G, S, sink, epoch + j, qa, nin, wave, NW, slabs, edges,
-                                    visited);
+                                    visited, T, fcur);
     pushed += sink.pushed;
     __syncthreads();
     const uint32_t nb = c[0], handover = c[1];
@@ -603,7 +676,7 @@ constexpr uint32_t kExpandLds = 2 * kTailCap + 4 > kWaveLds ? 2 * kTailCap + 4 :
 
 template <bool MAP, bool WEIGHTS, int MODE>
 MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
-                        uint32_t epoch, uint32_t *lds) {
+                        uint32_t epoch, uint32_t *lds, uint32_t T = kInf, uint32_t fcur = 0) {
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
@@ -621,7 +694,8 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
   }
   if (n_small + n_large == 0) return;
   if (n_large == 0 && n_small <= S.tail_enter) {
-    if (blockIdx.x == 0) expand_tail<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, n_small, lds);
+    if (blockIdx.x == 0)
+      expand_tail<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, n_small, lds, T, fcur);
     return;
   }
   WaveQ q{lds + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
@@ -645,17 +719,135 @@ MK_DEV void expand_step(const GraphView &G, const TravView &S, uint32_t side, ui
         if (MODE == 0 && du == kInf) act = false;
       }
     }
+    if constexpr (MODE == 0 && WEIGHTS) {
+      if (S.delta) {
+        // near-far: a hub at or past the near bound waits in the far list,
+        // deferred once, by the lane of its first bucket (the split queues
+        // the whole vertex again); its other entries' lanes just skip
+        const bool later = act && du >= T;
+        const bool first = later && b == G.bucket_off[u];
+        const bool stored = far_append(S, fcur, first, u, du);
+        if (later) act = false;
+        // a hub that did not fit (full far list) has lost buckets other
+        // lanes skipped: the call is repeated without near-far
+        if (first && !stored) atomicExch(&S.cnt->far_over, 1u);
+      }
+    }
     batched_buckets<MAP, WEIGHTS, MODE>(G, S, sink, epoch, ballot(act), b, u, du, slabs, edges);
   }
   // small vertices: up to 64 per wave step, spread over all waves when few
   expand_rows<MAP, WEIGHTS, MODE>(G, S, sink, epoch, S.q_small[side], n_small, gw, W, slabs, edges,
-                                  visited);
+                                  visited, T, fcur);
   block_flush(q, S.q_small[nside], &S.cnt->small[nslot], lds + (blockDim.x / 64) * kWaveQ);
   if (lane == 0) {
     add_stat(S.cnt, MODE, kStatSlabs, slabs);
     add_stat(S.cnt, MODE, kStatEdges, edges);
     add_stat(S.cnt, MODE, kStatVisited, visited);
     add_stat(S.cnt, MODE, kStatPushed, q.pushed);
+  }
+}
+
+// One bottom-up step of a static BFS (no reference counterpart: the
+// reference's BFS is top-down only, BFSImpl.cuh:498-588; its Rev* policies,
+// SlabGraph.cuh:113-132, are the in-edge view used here).  The frontier
+// (queue side `side`, level L) is marked in S.vbits[side]; every unvisited
+// vertex v that ever had an in-edge (R.vdeg[v] != 0) scans its in-slabs in
+// the mirror R for a marked in-neighbour u and stops at the first: v takes
+// level L + 1 with parent u and is pushed to side^1 / slot+1 (marked for the
+// next step).  An unvisited vertex has no in-neighbour below level L, so
+// every marked in-neighbour is at level L and the levels are those of the
+// top-down step; the parent is a valid one (any frontier in-neighbour), not
+// necessarily the smallest.  Each wave takes 64 consecutive vertices; its
+// eight 8-lane groups scan one candidate each at a time.  Called by every
+// thread of every block; `lds` as expand_step.
+template <bool MAP>
+MK_DEV void pull_step(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                      uint32_t *lds) {
+  const uint32_t lane = lane_id(), gl = group_lane(), grp = lane / kGroupLanes;
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t nslot = slot == 2 ? 0 : slot + 1;
+  const uint32_t cslot = nslot == 2 ? 0 : nslot + 1;
+  const uint32_t nside = side ^ 1u;
+  const uint32_t n_small = coherent_load(&S.cnt->small[slot]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    coherent_store(&S.cnt->small[cslot], 0u);
+    __hip_atomic_store(reinterpret_cast<uint64_t *>(&S.cnt->large[cslot]), uint64_t(0),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicAdd(&S.cnt->steps, 1u);
+  }
+  // the frontier's level, from any of its entries (the caller saw one)
+  const uint32_t u0 = n_small ? qload(S.q_small[side]) : qload2(S.q_large[side]).x;
+  const uint32_t L = dist_of(coherent_load64(S.dist + u0));
+  const uint32_t *vb = S.vbits[side];
+  const GraphView &R = S.R;
+  WaveQ q{lds + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
+  GlobalSink sink{S, q, nside, nslot};
+  uint32_t slabs = 0, edges = 0, visited = 0;
+  for (uint64_t v0 = uint64_t(gw) * 64; v0 < G.n; v0 += uint64_t(W) * 64) {
+    const uint32_t v = uint32_t(v0) + lane;
+    bool cand = v < G.n && R.vdeg[v] != 0u;
+    if (cand) cand = dist_of(coherent_load64(S.dist + v)) == kInf;
+    uint64_t cm = ballot(cand);
+    visited += uint32_t(__popcll(cm));
+    while (cm) {
+      // candidate k of this round (k = group) is the k-th lowest set bit
+      uint64_t m = cm;
+      for (uint32_t k = 0; k < grp && m; ++k) m &= m - 1;
+      const bool has = m != 0;
+      const uint32_t vl = has ? ffs64(m) : 0u;  // its lane
+      for (uint32_t k = 0; k < kGroupsPerWave && cm; ++k) cm &= cm - 1;
+      const uint32_t vk = uint32_t(v0) + vl;
+      uint32_t parent = kEmpty;
+      if (has) {
+        const uint32_t b1 = R.bucket_off[vk + 1];
+        for (uint32_t b = R.bucket_off[vk]; b < b1 && parent == kEmpty; ++b) {
+          const uint32_t *slab = R.heads + size_t(b) * kSlabWords;
+          for (;;) {
+            uint32_t w[kWordsPerLane];
+            load_words(slab + kWordsPerLane * gl, w);
+            if (gl == 0) ++slabs;
+            // every live key's bitmap word first (independent loads), then the tests
+            uint32_t bits[kWordsPerLane];
+#pragma unroll
+            for (uint32_t k = 0; k < kWordsPerLane; ++k) {
+              const bool key = key_word<MAP>(kWordsPerLane * gl + k) && w[k] < kTomb;
+              bits[k] = key ? coherent_load(vb + (w[k] >> 5)) : 0u;
+              edges += key ? 1u : 0u;
+            }
+            uint32_t hit = kEmpty;
+#pragma unroll
+            for (uint32_t k = 0; k < kWordsPerLane; ++k)
+              if (hit == kEmpty && key_word<MAP>(kWordsPerLane * gl + k) && w[k] < kTomb &&
+                  ((bits[k] >> (w[k] & 31u)) & 1u))
+                hit = w[k];
+            const uint32_t hb = group_bits(ballot(hit != kEmpty));
+            if (hb) {
+              parent = group_read(hit, ffs32(hb));
+              break;
+            }
+            const uint32_t nx = group_read(w[kWordsPerLane - 1], kGroupLanes - 1);
+            if (nx == kEmpty) break;
+            slab = slab_ptr(R.pool, nx);
+          }
+        }
+        if (gl == 0 && parent != kEmpty) {
+          __hip_atomic_store(reinterpret_cast<uint64_t *>(S.dist + vk), pack(L + 1, parent),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      sink(G, gl == 0 && has && parent != kEmpty, vk);
+    }
+  }
+  block_flush(q, S.q_small[nside], &S.cnt->small[nslot], lds + (blockDim.x / 64) * kWaveQ);
+  // slabs were counted on group lane 0, keys on every lane
+  slabs = wave_sum(slabs);
+  edges = wave_sum(edges);
+  if (lane == 0) {
+    add_stat(S.cnt, kSetRelax, kStatSlabs, slabs);
+    add_stat(S.cnt, kSetRelax, kStatEdges, edges);
+    add_stat(S.cnt, kSetRelax, kStatVisited, visited);
+    add_stat(S.cnt, kSetRelax, kStatPushed, q.pushed);
   }
 }
 
@@ -735,6 +927,7 @@ __global__ void __launch_bounds__(kWideBlock)
   const uint32_t bdone = coherent_load(&S.cnt->bar_done);
   if (S.trace && blockIdx.x == 0 && threadIdx.x == 0) S.cnt->loop_t[MODE][0] = wall_clock64();
   uint32_t j = j0;
+  bool pulling = false;  // bottom-up steps (static BFS with the in-edge mirror)
   for (; j < j_end; ++j) {
     const uint32_t slot = j % 3;
     // every block reads the same settled counts: the uniform stop decision
@@ -746,8 +939,20 @@ __global__ void __launch_bounds__(kWideBlock)
       S.cnt->trace_items[MODE][j - j0] = uint32_t(nl);
     }
     if (ns == 0 && (nl >> 32) == 0) break;
-    expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot,
-                                    MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
+    if constexpr (MODE == 0 && !WEIGHTS) {
+      if (S.vbits[0]) {
+        // the same counts in every block: the same choice everywhere
+        const uint64_t fbuckets = uint64_t(ns) + uint32_t(nl);
+        const uint64_t fverts = uint64_t(ns) + (nl >> 32);
+        pulling = pulling ? fverts * S.pull_exit >= G.n
+                          : fbuckets * S.pull_enter > G.total_buckets;
+      }
+    }
+    if (pulling)
+      pull_step<MAP>(G, S, j & 1u, slot, lds);
+    else
+      expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot,
+                                      MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
     if (!grid_barrier(S.bar, S.cnt, bdone + (j - j0) + 1, timeout, &flag)) return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1291,9 +1496,12 @@ __global__ void __launch_bounds__(kBlock) k_repair(GraphView G, GraphView R, Tra
 
 // Start a static run: the source at distance 0 is the frontier (side 0,
 // slot 0), in the queue its bucket count selects.
+// The source alone in the frontier of step 0 (queue side 0, slot 0), and
+// marked in `vbits0` (side 0's frontier bitmap) when bottom-up steps may run.
 __global__ void k_set_source(GraphView G, uint64_t *dist, uint32_t source, uint32_t *q_small,
-                             uint2 *q_large, Counts *cnt) {
+                             uint2 *q_large, Counts *cnt, uint32_t *vbits0) {
   dist[source] = pack(0, source);
+  if (vbits0) vbits0[source >> 5] |= 1u << (source & 31u);
   const uint32_t b0 = G.bucket_off[source], nb = G.bucket_off[source + 1] - b0;
   if (nb <= kSmallBuckets) {
     q_small[0] = source;
@@ -1367,6 +1575,13 @@ struct Traversal {
   bool repair = false;  // orphan repair before invalidation (with the in-edge mirror)
   bool persistent = true;        // frontier steps in one launch (k_expand_loop)
   uint32_t tail_enter = 128;     // tail-mode entry threshold (knob "tail_enter")
+  uint32_t *vbits = nullptr;     // two frontier bitmaps (static BFS with the mirror)
+  uint32_t pull_enter = 0, pull_exit = 1;  // knobs "pull_enter" / "pull_exit"
+  bool pull_now = false;         // the launch being queued may take bottom-up steps
+  uint2 *far[3] = {nullptr, nullptr, nullptr};  // near-far lists (static SSSP)
+  uint32_t far_cap = 0;
+  uint32_t delta = 0;            // knob "sssp_delta" (0: no near-far)
+  bool delta_now = false;        // the launch being queued is a static SSSP's
   bool coop = false;             // loops always launched cooperatively (knob "coop_launch")
   unsigned loop_blocks = 0;      // resident-grid size of both loop kernels
   uint64_t barrier_timeout = 0;  // wall-clock ticks a grid-barrier wait may take
@@ -1387,8 +1602,18 @@ struct Traversal {
     v.tail_enter = tail_enter;
     v.trace = profiling_on() ? 1u : 0u;
     v.tail_stay = std::min<uint32_t>(tail_enter + tail_enter / 2, kTailCap);
+    const bool pull = pull_now && vbits && g->rev;
+    v.R = pull ? g->rev->view() : GraphView{};
+    v.vbits[0] = pull ? vbits : nullptr;
+    v.vbits[1] = pull ? vbits + bit_words() : nullptr;
+    v.pull_enter = pull_enter;
+    v.pull_exit = pull_exit;
+    for (int i = 0; i < 3; ++i) v.far[i] = far[i];
+    v.far_cap = far_cap;
+    v.delta = (delta_now && far[0]) ? delta : 0u;
     return v;
   }
+  uint32_t bit_words() const { return (g->n + 31) / 32; }
 
   int init(mk_graph *graph, uint32_t src, bool w) {
     g = graph;
@@ -1423,12 +1648,15 @@ struct Traversal {
     fill_u32(stamp, 0u, n, stream);
     fill_u32(invbits, 0u, (n + 31) / 32, stream);
     clear_counts();
-    k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
+    k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt, nullptr);
     MK_HIP(hipStreamSynchronize(stream));
     max_waves = grid_waves(g->device);
     repair = tune_get("orphan_repair", 0) != 0;
     persistent = tune_get("persistent_steps", 1) != 0;
     tail_enter = uint32_t(tune_get("tail_enter", 128));
+    pull_enter = uint32_t(tune_get("pull_enter", 14));
+    pull_exit = uint32_t(std::max<int64_t>(tune_get("pull_exit", 24), 1));
+    if (!weights && g->rev && pull_enter) MK_HIP(hipMalloc(&vbits, size_t(bit_words()) * 8));
     coop = tune_get("coop_launch", 0) != 0;
     loop_stream_add(g->device, stream, 1);
     int khz = 0;
@@ -1457,6 +1685,7 @@ struct Traversal {
     (void)hipFree(stamp);
     (void)hipFree(invbits);
     (void)hipFree(inv_list);
+    (void)hipFree(vbits);
     for (int i = 0; i < 2; ++i) {
       (void)hipFree(qs[i]);
       (void)hipFree(ql[i]);
@@ -1878,9 +2107,15 @@ struct Traversal {
     if (int r = begin()) return r;
     stale = false;
     fill_u64(dist, kInfInf, g->n, stream);
-    k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt);
+    // a BFS over a graph with the in-edge mirror may take bottom-up steps
+    pull_now = persistent && vbits != nullptr;
+    if (pull_now) fill_u32(vbits, 0u, uint64_t(bit_words()) * 2, stream);
+    k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt,
+                                      pull_now ? vbits : nullptr);
     if (persistent) {
-      if (int r = queue_relax()) return r;
+      const int r = queue_relax();
+      pull_now = false;
+      if (r) return r;
     } else {
       Totals h;
       if (int r = iterate<0>(h)) return r;
