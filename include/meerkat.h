@@ -192,6 +192,17 @@ int mk_graph_bucket_offsets(mk_graph *g, uint32_t *off_host);
 /* Per-vertex count of live keys and of tombstones found by walking every
  * slab (the counting kernels of test/GraphTest.cu:58-200). */
 int mk_graph_count_slots(mk_graph *g, uint32_t *live_host, uint32_t *tomb_host);
+/* mk_graph_count_slots without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot call hipMalloc / hipFree
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * allocation).  The per-vertex counts are staged in a fixed 32 MiB device
+ * array of the library (one per device, held until the process ends)
+ * instead of a buffer allocated per call, so graphs of more than 4,194,304
+ * vertices are refused with MK_ERR_BAD_ARG.  Every call on a device shares
+ * that array: the library serialises its own calls with a per-device lock,
+ * and a call clears the counts it uses before counting.  Results and
+ * errors otherwise as mk_graph_count_slots. */
+int mk_graph_count_slots_noalloc(mk_graph *g, uint32_t *live_host, uint32_t *tomb_host);
 /* Adjacency as CSR on the host: row_off_host[n+1], col_host[m] (and
  * w_host[m] when weighted and non-NULL), neighbours in slab order
  * (the collection kernels of test/GraphTest.cu:202-483). */

@@ -96,6 +96,7 @@ _PROTOS = {
     "mk_graph_degrees": [vp, vp],
     "mk_graph_bucket_offsets": [vp, vp],
     "mk_graph_count_slots": [vp, vp, vp],
+    "mk_graph_count_slots_noalloc": [vp, vp, vp],
     "mk_graph_export_csr": [vp, vp, vp, vp],
     "mk_graph_slab_stats": [vp, P(u64), P(u64), P(u32)],
     "mk_bfs_create": [vp, u32, P(vp)],

@@ -1154,6 +1154,71 @@ int mk_graph_count_slots(mk_graph *g, uint32_t *live, uint32_t *tomb) {
   return r;
 }
 
+}  // extern "C"
+
+namespace mk {
+// Scratch of mk_graph_count_slots_noalloc: the live and tombstone counts of
+// up to kNoAllocMaxVertices vertices (2 x 16 MiB), one copy per device,
+// held until the process ends.  Host code never passes `scratch` itself to
+// a kernel or a copy (its host-side name is not a device address): it uses
+// the address hipGetSymbolAddress returned for the device, kept below.
+constexpr uint32_t kNoAllocMaxVertices = 1u << 22;
+static __device__ __align__(256) uint32_t scratch[2 * size_t(kNoAllocMaxVertices)];
+
+struct NoAllocScratch {
+  std::mutex mu;         // one call at a time per device (they share the array)
+  uint32_t *ptr = nullptr;  // device address of `scratch` on this device
+};
+static std::mutex noalloc_table_mu;
+static std::map<int, NoAllocScratch> noalloc_table;
+
+// The device's scratch, looked up on the first call on that device (with the
+// device current); nullptr and an error set on failure.
+static NoAllocScratch *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(noalloc_table_mu);
+  NoAllocScratch &e = noalloc_table[device];  // node-based map: the address stays valid
+  if (!e.ptr) {
+    void *p = nullptr;
+    const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (r != hipSuccess || !p) {
+      hip_fail(r == hipSuccess ? hipErrorInvalidSymbol : r,
+               "mk_graph_count_slots_noalloc: hipGetSymbolAddress(scratch)");
+      return nullptr;
+    }
+    e.ptr = static_cast<uint32_t *>(p);
+  }
+  return &e;
+}
+}  // namespace mk
+
+extern "C" {
+
+int mk_graph_count_slots_noalloc(mk_graph *g, uint32_t *live, uint32_t *tomb) {
+  if (int r = graph_settle(g)) return r;
+  if (!g || (!live && !tomb))
+    return fail(MK_ERR_BAD_ARG, "mk_graph_count_slots_noalloc: NULL argument");
+  if (g->n > kNoAllocMaxVertices)
+    return fail(MK_ERR_BAD_ARG, "mk_graph_count_slots_noalloc: the graph has " +
+                                    std::to_string(g->n) + " vertices, more than the " +
+                                    std::to_string(kNoAllocMaxVertices) +
+                                    " its fixed scratch holds (use mk_graph_count_slots)");
+  MK_HIP(hipSetDevice(g->device));
+  NoAllocScratch *sc = noalloc_scratch(g->device);
+  if (!sc) return MK_ERR_HIP;
+  std::lock_guard<std::mutex> lk(sc->mu);
+  // the previous call's counts are still there: clear this call's part
+  uint32_t *l = sc->ptr, *t = sc->ptr + g->n;
+  fill_u32(l, 0u, uint64_t(g->n) * 2, nullptr);
+  int r = count_slots(g, l, t, nullptr, nullptr);
+  if (r == MK_OK) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && live) e = hipMemcpy(live, l, size_t(g->n) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && tomb) e = hipMemcpy(tomb, t, size_t(g->n) * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = hip_fail(e, "mk_graph_count_slots_noalloc");
+  }
+  return r;
+}
+
 int mk_graph_slab_stats(mk_graph *g, uint64_t *heads, uint64_t *overflow, uint32_t *max_chain) {
   if (int r = graph_settle(g)) return r;
   if (!g) return fail(MK_ERR_BAD_ARG, "mk_graph_slab_stats: NULL graph");
