@@ -62,6 +62,10 @@ static std::map<std::string, int64_t> g_tune = {
     // until it holds fewer than 1/pull_exit of the vertices
     {"pull_enter", 14},
     {"pull_exit", 24},
+    // static SSSP: near-far with this bucket width (0: label-correcting
+    // without deferral); read when an SSSP handle is created
+    {"sssp_delta", 0},
+    {"sssp_far_cap", 0},  // entries per near-far list (0: one per vertex)
 };
 
 int64_t tune_get(const char *name, int64_t dflt) {
@@ -483,6 +487,40 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// flag[owner] = 1 for every bucket holding a live key (one wave per bucket).
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock) k_live_flags(GraphView G, uint32_t *flag) {
+  const uint32_t lane = lane_id();
+  const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= G.total_buckets) return;
+  const uint32_t *slab = G.heads + size_t(b) * kSlabWords;
+  for (;;) {
+    const uint32_t v = slab[lane];
+    if (ballot(v < kTomb) & key_mask<MAP>()) {
+      if (lane == 0) flag[G.bucket_owner[b]] = 1u;
+      return;
+    }
+    const uint32_t nx = read_lane(v, kNextLane);
+    if (nx == kEmpty) return;
+    slab = slab_ptr(G.pool, nx);
+  }
+}
+
+int mirror_flags_enable(mk_graph *rev) {
+  if (rev->in_flags) return MK_OK;
+  MK_HIP(hipSetDevice(rev->device));
+  mk_graph *g = rev->mirror_of;
+  hipStream_t s = g ? g->own_stream : nullptr;
+  fill_u32(rev->vdeg, 0u, rev->n, s);
+  auto k = rev->weighted ? k_live_flags<true> : k_live_flags<false>;
+  k<<<blocks_for_waves(rev->total_buckets), kBlock, 0, s>>>(rev->view(), rev->vdeg);
+  MK_HIP(hipGetLastError());
+  MK_HIP(hipStreamSynchronize(s));
+  // every later insert keeps the flags
+  rev->in_flags = true;
+  return MK_OK;
+}
+
 // Write each bucket's live keys (and values) at cursor[owner]++.
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
@@ -564,6 +602,10 @@ int mk_tune_set(const char *name, int64_t value) {
   if ((k == "pull_enter" || k == "pull_exit") && (value < 0 || value > 4096))
     return fail(MK_ERR_BAD_ARG, k + " must be in [0, 4096]");
   if (k == "pull_exit" && value == 0) return fail(MK_ERR_BAD_ARG, "pull_exit must be >= 1");
+  if (k == "sssp_far_cap" && (value < 0 || value > 0xFFFFFFF0ll))
+    return fail(MK_ERR_BAD_ARG, "sssp_far_cap must be in [0, 4294967280]");
+  if (k == "sssp_delta" && (value < 0 || value > (1 << 24)))
+    return fail(MK_ERR_BAD_ARG, "sssp_delta must be in [0, 16777216]");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(k);
   if (it == g_tune.end()) return fail(MK_ERR_BAD_ARG, "mk_tune_set: unknown knob " + k);
@@ -835,11 +877,13 @@ static uint32_t pool_failures(mk_graph *g) {
 
 // The mutation side of graph g (its own view and status; the counters of
 // the graph the batch was given to, `counters`).  The mirror keeps no
-// live degree array (mk_graph_degrees counts its live keys instead), only
-// the flag of vertices that were ever given an in-edge.
+// live degree array (mk_graph_degrees counts its live keys instead), only,
+// once bottom-up BFS asked for it (mirror_flags_enable), the flag of
+// vertices that were ever given an in-edge: a store per applied edge that
+// costs the insert ~4% while nothing reads it.
 static MutSide side_of(mk_graph *g, bool swap, uint32_t *counters) {
   return MutSide{g->view(), g->status.as<uint8_t>(), counters, swap ? 1u : 0u,
-                 g->mirror_of ? 2u : 1u};
+                 g->mirror_of ? (g->in_flags ? 2u : 0u) : 1u};
 }
 
 // Launch the insert kernel over g and, when it has one, its mirror (one

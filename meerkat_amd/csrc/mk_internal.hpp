@@ -243,6 +243,10 @@ void track_release(mk_graph *g);
 void mark_new(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n, hipStream_t s);
 int device_cus(int device);
 int64_t tune_get(const char *name, int64_t dflt);
+// Start keeping the in-edge mirror's flags of vertices with in-edges (its
+// vdeg): set from the mirror's slabs now, then by every insert (queued on
+// the graph's stream).
+int mirror_flags_enable(mk_graph *rev);
 // Grid-stride kernels: the number of waves to launch on `device`.
 inline int grid_waves(int device) { return device_cus(device) * int(tune_get("waves_per_cu", 16)); }
 
@@ -295,6 +299,10 @@ struct mk_graph {
   uint32_t *bucket_off = nullptr;
   uint32_t *bucket_owner = nullptr;
   uint32_t *vdeg = nullptr;
+  // the in-edge mirror's vdeg holds "ever given an in-edge" flags, kept up
+  // to date by its inserts, once a bottom-up BFS handle asked for them
+  // (mirror_flags_enable); until then the mirror keeps no per-vertex array
+  bool in_flags = false;
   uint32_t *counters = nullptr;  // [0] range-check epoch [2..3] pool failures (u64)
   uint32_t mut_epoch = 0;        // epoch of the last mutation call
   hipStream_t own_stream = nullptr;  // mutations given no stream run here

@@ -124,6 +124,7 @@ struct Totals {
   uint32_t small[3];
   uint32_t steps;
   uint32_t abort, loop_end[2];
+  uint32_t far_over, nf_split;
   unsigned long long large[3];
   unsigned long long slabs, edges, visited, pushed;  // over every set
   unsigned long long set[kSets][4];                 // [set][which]
@@ -851,6 +852,56 @@ MK_DEV void pull_step(const GraphView &G, const TravView &S, uint32_t side, uint
   }
 }
 
+// The split step of near-far SSSP, run in place of a frontier step when the
+// near frontier is empty and far list `fin` is not: with the near bound
+// already raised to T, each entry (v, d) of `fin` whose vertex still has
+// distance d (else a later entry stands for it) is queued for the next step
+// (queue side^1, slot+1, once per vertex: stamp `epoch`) when d < T, and
+// kept in far list `fout` otherwise (queued instead if `fout` is full).
+// Far list `fclr` (consumed by the previous split) is cleared for the next.
+// Called by every thread of every block; `lds` as expand_step.
+MK_DEV void split_step(const GraphView &G, const TravView &S, uint32_t side, uint32_t slot,
+                       uint32_t epoch, uint32_t T, uint32_t fin, uint32_t fout, uint32_t fclr,
+                       uint32_t *lds) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t W = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t nslot = slot == 2 ? 0 : slot + 1;
+  const uint32_t cslot = nslot == 2 ? 0 : nslot + 1;
+  const uint32_t nside = side ^ 1u;
+  const uint32_t fc = min(coherent_load(&S.cnt->far_cnt[fin]), S.far_cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    coherent_store(&S.cnt->small[cslot], 0u);
+    __hip_atomic_store(reinterpret_cast<uint64_t *>(&S.cnt->large[cslot]), uint64_t(0),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    coherent_store(&S.cnt->far_cnt[fclr], 0u);
+    coherent_store(&S.cnt->far_minx[fclr], 0u);
+    atomicAdd(&S.cnt->steps, 1u);
+  }
+  WaveQ q{lds + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
+  GlobalSink sink{S, q, nside, nslot};
+  uint32_t visited = 0;
+  for (uint32_t i0 = gw * 64; i0 < fc; i0 += W * 64) {
+    const uint32_t i = i0 + lane;
+    bool live = i < fc;
+    uint2 e = make_uint2(0u, 0u);
+    if (live) {
+      e = qload2(S.far[fin] + i);
+      live = dist_of(coherent_load64(S.dist + e.x)) == e.y;
+    }
+    visited += uint32_t(__popcll(ballot(live)));
+    const bool kept = far_append(S, fout, live && e.y >= T, e.x, e.y);
+    bool push = live && !kept;
+    if (push) push = atomicExch(S.stamp + e.x, epoch) != epoch;
+    sink(G, push, e.x);
+  }
+  block_flush(q, S.q_small[nside], &S.cnt->small[nslot], lds + (blockDim.x / 64) * kWaveQ);
+  if (lane == 0) {
+    add_stat(S.cnt, kSetRelax, kStatVisited, visited);
+    add_stat(S.cnt, kSetRelax, kStatPushed, q.pushed);
+  }
+}
+
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_expand(GraphView G, TravView S, uint32_t side, uint32_t slot, uint32_t epoch) {
@@ -928,6 +979,16 @@ __global__ void __launch_bounds__(kWideBlock)
   if (S.trace && blockIdx.x == 0 && threadIdx.x == 0) S.cnt->loop_t[MODE][0] = wall_clock64();
   uint32_t j = j0;
   bool pulling = false;  // bottom-up steps (static BFS with the in-edge mirror)
+  // near-far (static SSSP): the near bound and the number of splits so far,
+  // as the launch this one continues left them
+  uint32_t T = kInf, nsplit = 0;
+  if constexpr (MODE == 0 && WEIGHTS) {
+    if (S.delta) {
+      nsplit = coherent_load(&S.cnt->nf_split);
+      T = coherent_load(&S.cnt->nf_T);
+      if (T == 0) T = S.delta;
+    }
+  }
   for (; j < j_end; ++j) {
     const uint32_t slot = j % 3;
     // every block reads the same settled counts: the uniform stop decision
@@ -938,7 +999,25 @@ __global__ void __launch_bounds__(kWideBlock)
       S.cnt->trace_small[MODE][j - j0] = ns;
       S.cnt->trace_items[MODE][j - j0] = uint32_t(nl);
     }
-    if (ns == 0 && (nl >> 32) == 0) break;
+    const uint32_t ep = MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u;
+    if (ns == 0 && (nl >> 32) == 0) {
+      // near frontier empty: done, unless near-far has deferred vertices
+      bool split = false;
+      if constexpr (MODE == 0 && WEIGHTS) {
+        const uint32_t fin = nsplit % 3;
+        if (S.delta && coherent_load(&S.cnt->far_cnt[fin]) != 0) {
+          const uint32_t fmin = ~coherent_load(&S.cnt->far_minx[fin]);
+          const uint32_t base = fmin != kInf && fmin > T ? fmin : T;
+          T = base >= kInf - 1 - S.delta ? kInf - 1 : base + S.delta;
+          split_step(G, S, j & 1u, slot, ep, T, fin, (nsplit + 1) % 3, (nsplit + 2) % 3, lds);
+          ++nsplit;
+          split = true;
+        }
+      }
+      if (!split) break;
+      if (!grid_barrier(S.bar, S.cnt, bdone + (j - j0) + 1, timeout, &flag)) return;
+      continue;
+    }
     if constexpr (MODE == 0 && !WEIGHTS) {
       if (S.vbits[0]) {
         // the same counts in every block: the same choice everywhere
@@ -951,8 +1030,7 @@ __global__ void __launch_bounds__(kWideBlock)
     if (pulling)
       pull_step<MAP>(G, S, j & 1u, slot, lds);
     else
-      expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot,
-                                      MODE == 0 ? epoch0 + (j - j0) * kEpochStride : 0u, lds);
+      expand_step<MAP, WEIGHTS, MODE>(G, S, j & 1u, slot, ep, lds, T, nsplit % 3);
     if (!grid_barrier(S.bar, S.cnt, bdone + (j - j0) + 1, timeout, &flag)) return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -965,6 +1043,10 @@ __global__ void __launch_bounds__(kWideBlock)
     }
     S.cnt->loop_end[MODE] = j;
     S.cnt->bar_done = bdone + (j - j0);
+    if (MODE == 0 && S.delta) {
+      S.cnt->nf_T = T;
+      S.cnt->nf_split = nsplit;
+    }
     if (clear_exit) {
       for (uint32_t k = 0; k < 3; ++k) {
         S.cnt->small[k] = 0u;
@@ -1581,6 +1663,7 @@ struct Traversal {
   uint2 *far[3] = {nullptr, nullptr, nullptr};  // near-far lists (static SSSP)
   uint32_t far_cap = 0;
   uint32_t delta = 0;            // knob "sssp_delta" (0: no near-far)
+  bool last_far_over = false;    // the last call's far list overflowed on a hub
   bool delta_now = false;        // the launch being queued is a static SSSP's
   bool coop = false;             // loops always launched cooperatively (knob "coop_launch")
   unsigned loop_blocks = 0;      // resident-grid size of both loop kernels
@@ -1655,8 +1738,19 @@ struct Traversal {
     persistent = tune_get("persistent_steps", 1) != 0;
     tail_enter = uint32_t(tune_get("tail_enter", 128));
     pull_enter = uint32_t(tune_get("pull_enter", 14));
+    delta = uint32_t(tune_get("sssp_delta", 0));
+    if (weights && delta) {
+      // three far lists of up to n entries each (near-far static SSSP)
+      // knob "sssp_far_cap" (0: n entries) bounds each list
+      const int64_t cap = tune_get("sssp_far_cap", 0);
+      far_cap = uint32_t(std::min<size_t>(cap > 0 ? size_t(cap) : n, 0xFFFFFFF0ull));
+      for (int i = 0; i < 3; ++i) MK_HIP(hipMalloc(&far[i], size_t(far_cap) * 8));
+    }
     pull_exit = uint32_t(std::max<int64_t>(tune_get("pull_exit", 24), 1));
-    if (!weights && g->rev && pull_enter) MK_HIP(hipMalloc(&vbits, size_t(bit_words()) * 8));
+    if (!weights && g->rev && pull_enter) {
+      MK_HIP(hipMalloc(&vbits, size_t(bit_words()) * 8));
+      if (int r = mirror_flags_enable(g->rev)) return r;
+    }
     coop = tune_get("coop_launch", 0) != 0;
     loop_stream_add(g->device, stream, 1);
     int khz = 0;
@@ -1686,6 +1780,7 @@ struct Traversal {
     (void)hipFree(invbits);
     (void)hipFree(inv_list);
     (void)hipFree(vbits);
+    for (int i = 0; i < 3; ++i) (void)hipFree(far[i]);
     for (int i = 0; i < 2; ++i) {
       (void)hipFree(qs[i]);
       (void)hipFree(ql[i]);
@@ -1812,6 +1907,8 @@ struct Traversal {
     }
     h.steps = c.steps;
     h.abort = c.abort;
+    h.far_over = c.far_over;
+    h.nf_split = c.nf_split;
     h.loop_end[0] = c.loop_end[0];
     h.loop_end[1] = c.loop_end[1];
     for (uint32_t s = 0; s < kSets; ++s)
@@ -2053,6 +2150,7 @@ struct Traversal {
     last_slabs = h.slabs;
     last_edges = h.edges;
     last_iters = h.steps;
+    last_far_over = h.far_over != 0;
     return MK_OK;
   }
 
@@ -2102,7 +2200,7 @@ struct Traversal {
     return MK_OK;
   }
 
-  int run_static(float *ms) {
+  int run_static(float *ms, bool near_far = true) {
     if (int r = settle()) return r;
     if (int r = begin()) return r;
     stale = false;
@@ -2110,17 +2208,29 @@ struct Traversal {
     // a BFS over a graph with the in-edge mirror may take bottom-up steps
     pull_now = persistent && vbits != nullptr;
     if (pull_now) fill_u32(vbits, 0u, uint64_t(bit_words()) * 2, stream);
+    // an SSSP with the knob sssp_delta set runs near-far (a loop that runs
+    // out of steps is continued near-far too: end() relaunches it)
+    delta_now = near_far && persistent && far[0] != nullptr;
     k_set_source<<<1, 1, 0, stream>>>(g->view(), dist, source, qs[0], ql[0], cnt,
                                       pull_now ? vbits : nullptr);
     if (persistent) {
       const int r = queue_relax();
       pull_now = false;
-      if (r) return r;
+      if (r) {
+        delta_now = false;
+        return r;
+      }
     } else {
       Totals h;
       if (int r = iterate<0>(h)) return r;
     }
-    return end(ms);
+    const bool nf = delta_now;
+    const int r = end(ms);
+    delta_now = false;
+    // a deferred hub that found its far list full left buckets unexpanded:
+    // the call is repeated without near-far
+    if (r == MK_OK && nf && last_far_over) return run_static(ms, false);
+    return r;
   }
 
   int run_incremental(const uint32_t *src, const uint32_t *dst, const uint32_t *w, uint32_t n,

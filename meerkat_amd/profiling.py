@@ -178,10 +178,10 @@ def traffic(fetch_csv, write_csv, region=0, calib_region=1, calib_bytes=1 << 30)
     return out, ff, wf
 
 
-def report(trace_csv, fetch_csv, write_csv, commands, top=12):
+def report(trace_csv, fetch_csv, write_csv, commands, top=12, program="bench.py"):
     """Text summary of the timed region (marker pair 0): kernel times from the
     kernel trace and HBM traffic per dispatch from the counter passes."""
-    lines = ["# rocprofv3 summary: timed region of bench.py (between marker pair 0)", ""]
+    lines = [f"# rocprofv3 summary: timed region of {program} (between marker pair 0)", ""]
     lines += ["Commands:"] + [f"  {c}" for c in commands] + [""]
     lines.append("Kernel trace, timed region:")
     lines.append(f"{'calls':>6} {'total_ms':>10} {'avg_us':>9} {'min_us':>8} {'max_us':>9}  kernel")
@@ -224,6 +224,7 @@ def _main(argv):
     rp.add_argument("--command", action="append", default=[])
     rp.add_argument("--out", required=True, help="summary text file")
     rp.add_argument("--traffic-out", required=True, help="'<kernel> <bytes>' lines")
+    rp.add_argument("--program", default="bench.py", help="what the trace is of (title)")
     b = sub.add_parser("pmc", help="mean per-dispatch counter value per kernel")
     b.add_argument("csv")
     b.add_argument("--counter", required=True)
@@ -235,7 +236,8 @@ def _main(argv):
         for name, calls, tot, avg, mn, mx, pct in kernel_stats(args.csv, args.top):
             print(f"{calls:6d} {tot:10.3f} {avg:10.2f} {mn:9.2f} {mx:10.2f} {pct:6.2f}  {name}")
     elif args.cmd == "report":
-        text, machine = report(args.trace, args.fetch, args.write, args.command)
+        text, machine = report(args.trace, args.fetch, args.write, args.command,
+                               program=args.program)
         with open(args.out, "w") as f:
             f.write(text)
         with open(args.traffic_out, "w") as f:

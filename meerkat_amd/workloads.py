@@ -9,6 +9,8 @@ kernel traces and counter passes (rocprofv3) beside bench.py's update step.
                                                # static BFS and SSSP from 8 sources
     python -m meerkat_amd.workloads multi      # RMAT-24: 64-source BFS and SSSP, twice each
 
+Launch knobs (mk_tune_set) may follow as name=value, e.g. sssp_delta=64.
+
 Inputs are built and staged in HBM before the measured region, which is
 delimited by marker pair 0 (profiling.marker); a 1 GiB copy between marker
 pair 1 follows, the calibration of rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -210,13 +212,19 @@ WORKLOADS = {"pagerank": pagerank, "tc_cc": tc_cc, "static": static, "multi": mu
 
 
 def main(argv):
-    if len(argv) != 1 or argv[0] not in WORKLOADS:
+    knobs = [a.split("=", 1) for a in argv[1:]]
+    if not argv or argv[0] not in WORKLOADS or any(len(k) != 2 for k in knobs):
         print(__doc__, file=sys.stderr)
         return 2
     import torch
 
+    from . import autotune
+
     assert torch.cuda.is_available(), "the workloads need a GPU"
+    for name, value in knobs:
+        autotune.set_knob(name, int(value))
     out = WORKLOADS[argv[0]]()
+    out["knobs"] = autotune.knobs()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -98,7 +98,7 @@ def test_tune_knobs_validate_without_gpu():
     assert before == {"waves_per_cu": 16, "ms_unroll": 8, "orphan_repair": 0,
                       "persistent_steps": 1, "tail_enter": 128,
                       "coop_launch": 0, "mut_waves_per_cu": 32, "barrier_timeout_ms": 5000,
-                      "pull_enter": 14, "pull_exit": 24}
+                      "pull_enter": 14, "pull_exit": 24, "sssp_delta": 0, "sssp_far_cap": 0}
     try:
         autotune.set_knob("waves_per_cu", 32)
         assert autotune.get_knob("waves_per_cu") == 32

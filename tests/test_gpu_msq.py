@@ -86,13 +86,23 @@ def test_autotune_picks_a_setting_and_keeps_parity(gpu):
     g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True)
     g.InsertEdges(s, d, w)
     try:
-        r = autotune.tune(g, waves=(8, 16), unrolls=(4, 8), repeats=1)
+        r = autotune.tune(g, waves=(8, 16), unrolls=(4, 8), deltas=(0, 40), tails=(0, 128),
+                          repeats=1)
         assert r["waves_per_cu"] in (8, 16) and r["ms_unroll"] in (4, 8)
+        assert r["sssp_delta"] in (0, 40) and r["tail_enter"] in (0, 128)
         assert set(r["timings"]["waves_per_cu"]) == {8, 16}
+        assert set(r["timings"]["sssp_delta"]) == {0, 40}
+        assert "pull_enter/pull_exit" not in r["timings"]  # no in-edge mirror
+        tuned = ("waves_per_cu", "ms_unroll", "sssp_delta", "tail_enter")
         k = autotune.knobs()
-        assert (k["waves_per_cu"], k["ms_unroll"]) == (r["waves_per_cu"], r["ms_unroll"])
-        assert {x: k[x] for x in k if x not in ("waves_per_cu", "ms_unroll")} == {
-            x: before[x] for x in before if x not in ("waves_per_cu", "ms_unroll")}
+        assert all(k[x] == r[x] for x in tuned)
+        assert {x: k[x] for x in k if x not in tuned} == {
+            x: before[x] for x in before if x not in tuned}
+        # a tuned static SSSP is still exact
+        sp = mk.SSSP(g, 0)
+        sp.Static()
+        assert np.array_equal(sp.Distances(), oracle.dijkstra(oracle.csr(n, s, d, w), 0))
+        sp.close()
         srcs = np.array([0, 5, 77], np.uint32)
         q = mk.MultiSource(g, sssp=True)
         q.run(srcs)
@@ -105,6 +115,37 @@ def test_autotune_picks_a_setting_and_keeps_parity(gpu):
         autotune.tune(g, waves=(8,), unrolls=(4,), repeats=1, apply=False)
         assert autotune.get_knob("waves_per_cu") == 12
     finally:
+        for k_, v in before.items():
+            autotune.set_knob(k_, v)
+
+
+def test_autotune_push_pull_thresholds_and_load_factor(gpu):
+    from meerkat_amd import autotune
+
+    before = autotune.knobs()
+    n, s, d, _ = gen.rmat(13, 16, seed=34)
+    pool = mk.SlabPool(1 << 27)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), reverse=True,
+                        in_degree_hints=gen.degree_hints(n, d))
+    g.InsertEdges(s, d)
+    try:
+        r = autotune.tune(g, waves=(16,), pulls=((14, 24), (0, 24)), tails=(128,), repeats=1,
+                          sources=np.zeros(64, np.uint32))
+        assert (r["pull_enter"], r["pull_exit"]) in ((14, 24), (0, 24))
+        assert set(r["timings"]["pull_enter/pull_exit"]) == {(14, 24), (0, 24)}
+        b = mk.BFS(g, 0)
+        b.Static()
+        assert np.array_equal(b.Distances(), oracle.bfs(oracle.csr(n, s, d), 0))
+        b.close()
+        lf = autotune.tune_load_factor(n, s, d, factors=(0.5, 0.9), reverse=True, repeats=1,
+                                       pool_bytes=1 << 27)
+        assert lf["load_factor"] in (0.5, 0.9)
+        t = lf["timings"]
+        # a lower load factor spends more head slabs
+        assert t[0.5]["slabs"]["head_slabs"] >= t[0.9]["slabs"]["head_slabs"]
+    finally:
+        g.close()
+        pool.close()
         for k_, v in before.items():
             autotune.set_knob(k_, v)
 
