@@ -110,9 +110,12 @@ def verify_sssp(n, s, d, w, del_sel, ins_s, ins_d, ins_w, dist, parent, source=0
 def run_serving(args, n, g, s, d, del_idx, used, log):
     """Serving-loop phase on the resident graph: the same request mix served
     one request at a time (single-source static BFS / SSSP) and with
-    continuous batching (64-source traversals, BFS and SSSP rounds
-    overlapped), all requests arriving at once (saturation throughput); then
-    a Poisson stream at a fixed offered load for latency percentiles."""
+    continuous batching (queries admitted into running 64-slot multi-source
+    traversals, each done on its own step), all requests arriving at once
+    (each mode's saturation throughput); then Poisson streams offered at
+    fractions of the batched capacity, for latency percentiles (all queries,
+    and per kind) at each load beside the ratio of the requests/s sustained
+    there to the one-at-a-time capacity."""
     import time
 
     import numpy as np
@@ -126,13 +129,18 @@ def run_serving(args, n, g, s, d, del_idx, used, log):
     avail = np.nonzero(keep)[0]
     rng = np.random.default_rng(args.seed + 7)
     rng.shuffle(avail)
-    thirds = np.array_split(avail[: min(len(avail), 30_000_000)], 3)
+    loads = [float(x) for x in args.serve_loads.split(",")]
+    parts = np.array_split(avail[: min(len(avail), 40_000_000)], 2 + len(loads))
     R = args.serve_requests
 
     def stream(i, rate, all_at_once):
-        keys = gen.keys_of(s[thirds[i]], d[thirds[i]])
+        keys = gen.keys_of(s[parts[i]], d[parts[i]])
         return serving.make_stream(n, keys, R, rate=rate, seed=args.seed + 100 + i,
                                    all_at_once=all_at_once)
+
+    def pct(reqs, kind, q):
+        lat = [r.latency * 1e3 for r in reqs if r.kind == kind]
+        return round(float(np.percentile(lat, q)), 2) if lat else None
 
     loop = serving.ServingLoop(g)
     res = {}
@@ -144,27 +152,37 @@ def run_serving(args, n, g, s, d, del_idx, used, log):
         res[mode] = serving.summarize(done, el)
         res[mode]["phase_s"] = {k: round(v, 3) for k, v in loop.phase_s.items()}
         log(f"serving {mode}: {res[mode]}")
+    seq_cap = res["sequential"]["requests_per_s"]
     cap = res["batched"]["requests_per_s"]
-    rate = 0.5 * cap
-    reqs = stream(2, rate, False)
-    t0 = time.perf_counter()
-    done = loop.run(reqs, mode="batched")
-    el = time.perf_counter() - t0
-    lat = serving.summarize(done, el)
-    log(f"serving at {rate:.0f} req/s offered: {lat}")
+    at_load = []
+    for j, frac in enumerate(loads):
+        rate = frac * cap
+        reqs = stream(2 + j, rate, False)
+        t0 = time.perf_counter()
+        done = loop.run(reqs, mode="batched")
+        el = time.perf_counter() - t0
+        lat = serving.summarize(done, el)
+        row = {"offered_fraction_of_capacity": frac,
+               "offered_requests_per_s": round(rate, 1),
+               "sustained_requests_per_s": round(lat["requests_per_s"], 1),
+               "ratio_to_one_at_a_time_capacity": round(lat["requests_per_s"] / seq_cap, 2),
+               "query_p50_ms": round(lat.get("query_p50_ms", 0.0), 2),
+               "query_p99_ms": round(lat.get("query_p99_ms", 0.0), 2),
+               "bfs_p50_ms": pct(done, "bfs", 50), "bfs_p99_ms": pct(done, "bfs", 99),
+               "sssp_p50_ms": pct(done, "sssp", 50), "sssp_p99_ms": pct(done, "sssp", 99),
+               "update_p99_ms": round(lat.get("update_p99_ms", 0.0), 2)}
+        at_load.append(row)
+        log(f"serving at {rate:.0f} req/s offered: {row}")
     loop.close()
     return {
         "workload": f"RMAT-{args.scale} resident; {R} requests per run: 30% update batches "
                     "(lognormal sizes, median 10,000, max 1,000,000 edges, 50/50 insert/delete), "
                     "70% BFS/SSSP queries (50/50) from uniform random sources",
-        "sequential_requests_per_s": round(res["sequential"]["requests_per_s"], 1),
+        "sequential_requests_per_s": round(seq_cap, 1),
         "batched_requests_per_s": round(cap, 1),
-        "speedup": round(cap / res["sequential"]["requests_per_s"], 2),
+        "speedup": round(cap / seq_cap, 2),
         "saturation_query_p99_ms": round(res["batched"].get("query_p99_ms", 0.0), 2),
-        "offered_load_requests_per_s": round(rate, 1),
-        "query_p50_ms": round(lat.get("query_p50_ms", 0.0), 2),
-        "query_p99_ms": round(lat.get("query_p99_ms", 0.0), 2),
-        "update_p99_ms": round(lat.get("update_p99_ms", 0.0), 2),
+        "at_load": at_load,
         "sequential_query_p50_ms": round(res["sequential"].get("query_p50_ms", 0.0), 2),
         "phase_seconds": {"sequential": res["sequential"]["phase_s"],
                           "batched": res["batched"]["phase_s"]},
@@ -187,6 +205,9 @@ def main():
     ap.add_argument("--no-verify", action="store_true",
                     help="skip the check of the final SSSP distances against the oracle")
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-loads", default="0.25,0.5",
+                    help="offered loads of the latency runs, as fractions of the batched "
+                         "capacity measured first")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="leave the library's kernel timing off in the timed region "
                          "(no roofline)")
@@ -199,6 +220,10 @@ def main():
     ap.add_argument("--sync-mutations", action="store_true",
                     help="each insert/delete waits for its own outcome instead of the "
                          "following SSSP call settling it (deferred checks)")
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="steps after the breakdown replayed from a hipGraph-captured step "
+                         "(mk_sssp_capture_step): batches loaded into its buffers, replays "
+                         "queued back to back (0: skip)")
     ap.add_argument("--sync-calls", action="store_true",
                     help="each SSSP call waits for its own outcome instead of returning once "
                          "queued and being settled by the next call (deferred calls)")
@@ -231,7 +256,7 @@ def main():
     # edges (each still present when deleted), insertions new edges
     # after the timed steps: breakdown steps with synchronous calls, first
     # for the per-call rates, then for the per-kernel table
-    steps_total = args.warmup + args.steps + 2 * args.breakdown_steps
+    steps_total = args.warmup + args.steps + 2 * args.breakdown_steps + args.graph_steps
     timed_end = args.warmup + args.steps
     half = args.batch // 2
     rng = np.random.default_rng(args.seed + 1)
@@ -344,6 +369,29 @@ def main():
     settle()
     profiling.enable(False)
     prof_all = profiling.totals()
+    # the same steps replayed from a captured hipGraph: each step's batches
+    # are copied into the step's buffers (mk_step_load) and the replays are
+    # queued back to back, one host wait at the end
+    captured = None
+    if args.graph_steps:
+        i0 = args.warmup + args.steps + 2 * args.breakdown_steps
+        step_graph = sp.capture_step(*dels[i0], *inss[i0])
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        for i in range(i0, i0 + args.graph_steps):
+            if i > i0:
+                step_graph.load(*dels[i], *inss[i])
+            step_graph.launch()
+        step_graph.sync()
+        torch.cuda.synchronize()
+        graph_elapsed = time.perf_counter() - tg
+        step_graph.close()
+        captured = {"steps": args.graph_steps,
+                    "ms_per_step": round(graph_elapsed / args.graph_steps * 1e3, 3),
+                    "edge_updates_per_s": round(args.batch * args.graph_steps / graph_elapsed, 1),
+                    "note": "each step also copies its 1M-edge batch into the captured "
+                            "buffers (5 device-to-device copies)"}
+        log(f"captured step replays: {captured}")
     calibration_copy(dev)
 
     # correctness of the maintained shortest paths after every step, against
@@ -457,6 +505,8 @@ def main():
         # the reference's protocol (every mutation and SSSP call waits for its
         # own outcome), wall clock over the synchronous breakdown steps
         "sync_protocol_ms_per_step": round(sync_elapsed / max(args.breakdown_steps, 1) * 1e3, 3),
+        # the step replayed from a hipGraph-captured step, queued back to back
+        "captured_step": captured,
         "sync_protocol_edge_updates_per_s": round(args.batch * args.breakdown_steps /
                                                   sync_elapsed, 1) if args.breakdown_steps else None,
         # deferred SSSP calls of the warm-up and timed steps; a recomputed one

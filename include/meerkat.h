@@ -288,6 +288,41 @@ int mk_sssp_set_deferred(mk_sssp *s, int on);
 int mk_sssp_sync(mk_sssp *s);
 int mk_sssp_deferred_stats(mk_sssp *s, uint64_t *settled, uint64_t *recomputed);
 
+/* Captured update steps (no reference counterpart: the reference launches
+ * every kernel from the host; north_star's hipGraph-captured
+ * update -> incremental-recompute step).  mk_*_capture_step records one step
+ * of the handle on its graph, DeleteEdges(del) -> Decremental(del) ->
+ * InsertEdges(ins) -> Incremental(ins), into a hipGraph, reading the batch
+ * buffers given here (device memory the caller refills between replays; the
+ * sizes are fixed).  Nothing runs at capture.  mk_step_launch queues one
+ * replay on the graph's stream and returns; replays may be queued back to
+ * back.  mk_step_sync waits for the queued replays and reports their
+ * outcome: MK_ERR_RANGE if a batch named a vertex >= n (that batch changed
+ * nothing), MK_ERR_POOL_FULL if an insert ran out of slabs (rolled back when
+ * it was the last replay's, else left partly applied), MK_ERR_STATE if a
+ * frontier loop gave up; after any of these the handle needs Static().  A
+ * relax loop that ran out of its steps is recomputed from scratch by the
+ * sync.  Calls through the API may follow a sync.  One step per graph; the
+ * handle cannot be destroyed before its step.  Needs persistent_steps = 1
+ * and profiling off at capture. */
+typedef struct mk_step mk_step;
+int mk_sssp_capture_step(mk_sssp *s, const uint32_t *del_src_dev, const uint32_t *del_dst_dev,
+                         uint32_t n_del, const uint32_t *ins_src_dev, const uint32_t *ins_dst_dev,
+                         const uint32_t *ins_w_dev, uint32_t n_ins, mk_step **out);
+int mk_bfs_capture_step(mk_bfs *b, const uint32_t *del_src_dev, const uint32_t *del_dst_dev,
+                        uint32_t n_del, const uint32_t *ins_src_dev, const uint32_t *ins_dst_dev,
+                        uint32_t n_ins, mk_step **out);
+int mk_step_launch(mk_step *st);
+/* Queue copies of the next step's batches (device arrays of the captured
+ * sizes; ins_w_dev NULL keeps the weights) into the step's buffers, on the
+ * graph's stream: ordered after the replays queued so far and before the
+ * next, so replays with different batches can be queued back to back. */
+int mk_step_load(mk_step *st, const uint32_t *del_src_dev, const uint32_t *del_dst_dev,
+                 const uint32_t *ins_src_dev, const uint32_t *ins_dst_dev,
+                 const uint32_t *ins_w_dev);
+int mk_step_sync(mk_step *st);
+int mk_step_destroy(mk_step *st);
+
 /* ------------------------------------------------------ A12 PageRank
  * PageRank(G, DampingFactor, ErrorMargin, N, VertexOutDegree, MaxIter)
  * (include/Algorithms/PageRank.cuh:32-67; impl/PageRankImpl.cuh:146-251).
@@ -364,6 +399,31 @@ int mk_msq_destroy(mk_msq *q);
 int mk_msq_run(mk_msq *q, const uint32_t *sources_host, uint32_t k, float *ms);
 int mk_msq_read(mk_msq *q, uint32_t slot, uint32_t *dist_host);
 int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
+/* Continuous batching (no reference counterpart): a session admits sources
+ * into free slots of a traversal that is already running, and each slot
+ * finishes on its own step.
+ *  - mk_msq_open starts a session on the graph as it is now (its mutations
+ *    must wait until the session's slots have finished: a slot's distances
+ *    are those of the graph version it ran on);
+ *  - mk_msq_admit puts sources_host[i] into slot slots_host[i] (< 64, each
+ *    slot once per session), joining the frontier of the next step (BFS
+ *    levels then count from that step);
+ *  - mk_msq_launch queues `steps` more steps (no-ops once the frontier is
+ *    empty) and returns; mk_msq_ready tells without waiting whether they
+ *    have run; mk_msq_poll waits for them and sets *active to the
+ *    admitted slots whose traversal still has work and *steps (nullable) to
+ *    the steps queued so far: an admitted slot not in *active is final, and
+ *    mk_msq_read(q, slot, ...) returns its distances.
+ * mk_msq_run ends any session.  A BFS session whose level log fills up
+ * returns MK_ERR_STATE from mk_msq_poll; its slots still active are then
+ * incomplete and belong in a new session. */
+int mk_msq_open(mk_msq *q);
+int mk_msq_admit(mk_msq *q, const uint32_t *sources_host, const uint32_t *slots_host, uint32_t k);
+int mk_msq_launch(mk_msq *q, uint32_t steps);
+int mk_msq_poll(mk_msq *q, uint64_t *active, uint32_t *steps);
+/* *ready = 1 when the steps launched so far have run (mk_msq_poll will not
+ * wait), else 0; never waits. */
+int mk_msq_ready(mk_msq *q, int *ready);
 
 /* ------------------------------------------------------ tuning knobs
  * Launch-shape knobs, read when a handle is created (no reference

@@ -295,6 +295,21 @@ class MultiSource {
     check(mk_msq_read(h_, slot, d.data()), "mk_msq_read");
     return d;
   }
+  // Continuous batching: sources join a running traversal in free slots and
+  // each slot finishes on its own step (mk_msq_open / admit / launch / poll).
+  void Open() { check(mk_msq_open(h_), "mk_msq_open"); }
+  void Admit(const std::vector<uint32_t> &sources, const std::vector<uint32_t> &slots) {
+    if (sources.size() != slots.size()) throw std::runtime_error("Admit: size mismatch");
+    check(mk_msq_admit(h_, sources.data(), slots.data(), uint32_t(sources.size())),
+          "mk_msq_admit");
+  }
+  void Launch(uint32_t steps = 1) { check(mk_msq_launch(h_, steps), "mk_msq_launch"); }
+  // Waits for the launched steps; the admitted slots with work left.
+  uint64_t Poll() {
+    uint64_t active = 0;
+    check(mk_msq_poll(h_, &active, nullptr), "mk_msq_poll");
+    return active;
+  }
 
  private:
   mk_msq *h_ = nullptr;

@@ -122,6 +122,12 @@ _PROTOS = {
     "mk_sssp_set_deferred": [vp, i32],
     "mk_sssp_sync": [vp],
     "mk_sssp_deferred_stats": [vp, P(u64), P(u64)],
+    "mk_sssp_capture_step": [vp, vp, vp, u32, vp, vp, vp, u32, P(vp)],
+    "mk_bfs_capture_step": [vp, vp, vp, u32, vp, vp, u32, P(vp)],
+    "mk_step_launch": [vp],
+    "mk_step_load": [vp, vp, vp, vp, vp, vp],
+    "mk_step_sync": [vp],
+    "mk_step_destroy": [vp],
     "mk_pagerank_create": [vp, f32, f32, u32, vp, P(vp)],
     "mk_pagerank_destroy": [vp],
     "mk_pagerank_static": [vp, P(f32)],
@@ -146,6 +152,11 @@ _PROTOS = {
     "mk_msq_run": [vp, vp, u32, P(f32)],
     "mk_msq_read": [vp, u32, vp],
     "mk_msq_last_stats": [vp, P(u32), P(u64)],
+    "mk_msq_open": [vp],
+    "mk_msq_admit": [vp, vp, vp, u32],
+    "mk_msq_launch": [vp, u32],
+    "mk_msq_poll": [vp, P(u64), P(u32)],
+    "mk_msq_ready": [vp, P(i32)],
     "mk_tune_set": [ctypes.c_char_p, ctypes.c_int64],
     "mk_tune_get": [ctypes.c_char_p, P(ctypes.c_int64)],
     "mk_profile_enable": [i32],

@@ -52,6 +52,25 @@ class _Traversal:
     def Distances(self):
         return self.read()[0]
 
+    def capture_step(self, del_src, del_dst, ins_src, ins_dst, ins_w=None):
+        """Record one update step on fixed device batch buffers (torch
+        tensors or device pointers), DeleteEdges -> Decremental ->
+        InsertEdges -> Incremental, into a hipGraph; returns a CapturedStep
+        whose launch() replays it (refill the buffers between replays)."""
+        from .graph import device_u32, dptr
+
+        dev = self.G.device
+        bufs = [device_u32(b, dev) for b in (del_src, del_dst, ins_src, ins_dst, ins_w)]
+        ds, dd, is_, id_, iw = bufs
+        out = _vp()
+        if self._prefix == "mk_sssp":
+            _lib.call("mk_sssp_capture_step", self._h, dptr(ds), dptr(dd), len(ds), dptr(is_),
+                      dptr(id_), dptr(iw), len(is_), ctypes.byref(out))
+        else:
+            _lib.call("mk_bfs_capture_step", self._h, dptr(ds), dptr(dd), len(ds), dptr(is_),
+                      dptr(id_), len(is_), ctypes.byref(out))
+        return CapturedStep(out, bufs)
+
     def Parents(self):
         return self.read()[1]
 
@@ -88,6 +107,50 @@ class _Traversal:
         try:
             if _lib.finalizer_ok() and self._h:
                 getattr(_lib.lib(), f"{self._prefix}_destroy")(self._h)
+        except Exception:
+            pass
+
+
+class CapturedStep:
+    """A captured update step (mk_step): launch() queues a replay on the
+    graph's stream, sync() waits for the queued replays and raises on a
+    failed batch or loop, close() releases it (before its handle)."""
+
+    def __init__(self, h, buffers):
+        self._h = h
+        # the device buffers the replays read (refill them in place between
+        # replays: buffers[0..4] = delete src, dst, insert src, dst, weights)
+        self.buffers = buffers
+        self._pending = []
+
+    def launch(self):
+        _lib.call("mk_step_launch", self._h)
+
+    def load(self, del_src, del_dst, ins_src, ins_dst, ins_w=None):
+        """Queue copies of the next batches (device tensors of the captured
+        sizes) into the step's buffers, ordered between the replays."""
+        from .graph import dptr
+
+        # the sources stay alive until the copies are known to have run
+        self._pending.append((del_src, del_dst, ins_src, ins_dst, ins_w))
+        _lib.call("mk_step_load", self._h, dptr(del_src), dptr(del_dst), dptr(ins_src),
+                  dptr(ins_dst), dptr(ins_w))
+
+    def sync(self):
+        try:
+            _lib.call("mk_step_sync", self._h)
+        finally:
+            self._pending = []
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, _vp()
+            _lib.call("mk_step_destroy", h)
+
+    def __del__(self):
+        try:
+            if _lib.finalizer_ok() and self._h:
+                _lib.lib().mk_step_destroy(self._h)
         except Exception:
             pass
 
@@ -316,6 +379,33 @@ class MultiSource:
         out = np.empty(self.G.n, np.uint32)
         _lib.call("mk_msq_read", self._h, int(slot), out.ctypes.data_as(_vp))
         return out
+
+    # continuous batching: a session admits sources into free slots while
+    # it runs; each slot finishes on its own step
+    def open(self):
+        _lib.call("mk_msq_open", self._h)
+
+    def admit(self, sources, slots):
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        sl = np.ascontiguousarray(slots, dtype=np.uint32)
+        assert len(src) == len(sl)
+        _lib.call("mk_msq_admit", self._h, src.ctypes.data_as(_vp), sl.ctypes.data_as(_vp),
+                  len(src))
+
+    def launch(self, steps=1):
+        _lib.call("mk_msq_launch", self._h, int(steps))
+
+    def ready(self):
+        """Whether the launched steps have run (poll() would not wait)."""
+        r = ctypes.c_int()
+        _lib.call("mk_msq_ready", self._h, ctypes.byref(r))
+        return bool(r.value)
+
+    def poll(self):
+        """Wait for the launched steps; returns (active slot mask, steps so far)."""
+        a, st = ctypes.c_uint64(), ctypes.c_uint32()
+        _lib.call("mk_msq_poll", self._h, ctypes.byref(a), ctypes.byref(st))
+        return a.value, st.value
 
     def last_stats(self):
         st, e = ctypes.c_uint32(), ctypes.c_uint64()

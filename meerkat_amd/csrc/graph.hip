@@ -171,10 +171,14 @@ __global__ void k_bucket_owner(const uint32_t *off, uint32_t n, uint32_t total, 
 // cleared before each call (no clearing launch): counters[0] holds the
 // epoch of the last batch that failed the range check; counters[2..3] (one
 // 64-bit word) hold (epoch << 32) | slab-pool failures of that epoch.
+// counters[1] is added to every epoch a kernel is given: zero for calls
+// made through the API, the replay's base while a captured update step
+// (mk_step) runs, so each replay has epochs of its own.
 
 // Range check of a batch: counters[0] = epoch when an endpoint is >= nv.
 __global__ void k_validate(const uint32_t *src, const uint32_t *dst, uint32_t n, uint32_t nv,
                            uint32_t *counters, uint32_t epoch) {
+  epoch += counters[1];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool bad = false;
   if (i < n) bad = src[i] >= nv || dst[i] >= nv;
@@ -264,6 +268,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  epoch += bad[1];  // the replay base of a captured step (0 otherwise)
   if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
     if (lane < cnt) M.status[base + lane] = MK_EDGE_UNCHANGED;
     return;
@@ -330,6 +335,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  epoch += bad[1];  // the replay base of a captured step (0 otherwise)
   if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
     if (lane < cnt && only_status == 0xFFu) M.status[base + lane] = MK_EDGE_UNCHANGED;
     return;
@@ -389,6 +395,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t base = uint64_t(wave) * epw;
   if (base >= n) return;
   const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  epoch += bad[1];  // the replay base of a captured step (0 otherwise)
   if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
     if (lane < cnt) M.status[base + lane] = MK_EDGE_UNCHANGED;
     return;
@@ -860,10 +867,11 @@ static int batch_prologue(mk_graph *g, const uint32_t *src, const uint32_t *dst,
 static int range_error(mk_graph *g, const uint32_t *src, const uint32_t *dst, uint32_t n,
                        hipStream_t s, const char *who) {
   if (g->host_scratch.as<uint32_t>()[0] != g->mut_epoch) return MK_OK;
-  fill_u32(g->counters + 1, 0u, 1, s);
-  k_count_bad<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters + 1);
+  // counters[4] counts the offending edges (counters[1] is the epoch base)
+  fill_u32(g->counters + 4, 0u, 1, s);
+  k_count_bad<<<blocks_for_threads(n), kBlock, 0, s>>>(src, dst, n, g->n, g->counters + 4);
   uint32_t bad = 0;
-  MK_HIP(hipMemcpyAsync(&bad, g->counters + 1, 4, hipMemcpyDeviceToHost, s));
+  MK_HIP(hipMemcpyAsync(&bad, g->counters + 4, 4, hipMemcpyDeviceToHost, s));
   MK_HIP(hipStreamSynchronize(s));
   return fail(MK_ERR_RANGE, std::string(who) + ": " + std::to_string(bad) +
                                 " edges have a vertex id >= n; graph unchanged");
@@ -1030,6 +1038,68 @@ static int mutation_finish(mk_graph *g, int op, const uint32_t *src, const uint3
   if (r) record_failure(g, r, src, dst, n);
   return r;
 }
+
+}  // extern "C"
+
+namespace mk {
+// A batch mutation queued for a captured update step (mk_step): the range
+// check, the kernels (forward and mirror) and, with `host_dev` (pinned) set,
+// the copy of the four counter words there, on the graph's own stream, with
+// no host wait and nothing left for a later call to settle.  op 1 insert,
+// 2 delete.
+// *epoch is the epoch the kernels were given (a replay adds its base).
+int graph_queue_mutation(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
+                         const uint32_t *w, uint32_t n, uint32_t *host_dev, uint32_t *epoch) {
+  hipStream_t s = g->own_stream;
+  const char *who = op == 1 ? "captured insert" : "captured delete";
+  if (int r = batch_prologue(g, src, dst, n, s, who)) return r;
+  if (int r = mirror_prologue(g, n)) return r;
+  if (op == 1) {
+    launch_insert(g, src, dst, w, n, s, "k_insert", g->counters);
+    mark_new(g, src, dst, n, s);
+  } else {
+    launch_delete(g, src, dst, n, s, 0xFFu, "k_delete", g->counters);
+  }
+  MK_HIP(hipGetLastError());
+  if (host_dev) publish_words(g->counters, host_dev, 4, g->mut_epoch, s);
+  g->last_op = op;
+  g->last_src = src;
+  g->last_dst = dst;
+  g->last_w = op == 1 ? w : nullptr;
+  *epoch = g->mut_epoch;
+  return MK_OK;
+}
+
+// The outcome of a replayed mutation from its four counter words (`words`,
+// host) and the epoch its kernels ran with: MK_ERR_RANGE (nothing changed),
+// MK_ERR_POOL_FULL after rolling an insert back, or MK_OK.
+int graph_replay_outcome(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
+                         uint32_t n, const uint32_t *words, uint32_t epoch) {
+  const char *who = op == 1 ? "captured insert" : "captured delete";
+  if (words[0] == epoch) {
+    const int r = fail(MK_ERR_RANGE, std::string(who) +
+                                         ": the batch has a vertex id >= n; graph unchanged");
+    record_failure(g, r, src, dst, n);
+    return r;
+  }
+  const uint64_t pw = (uint64_t(words[3]) << 32) | words[2];
+  const uint32_t pool_fail = op == 1 && uint32_t(pw >> 32) == epoch ? uint32_t(pw) : 0u;
+  if (pool_fail) {
+    // the insert's statuses are still the graph's last batch's
+    launch_delete(g, src, dst, n, g->own_stream, MK_EDGE_APPLIED, "k_insert_rollback",
+                  g->counters);
+    MK_HIP(hipStreamSynchronize(g->own_stream));
+    const int r = fail(MK_ERR_POOL_FULL, std::string(who) + ": slab pool exhausted (" +
+                                             std::to_string(pool_fail) +
+                                             " edges); batch rolled back, edge set unchanged");
+    record_failure(g, r, src, dst, n);
+    return r;
+  }
+  return MK_OK;
+}
+}  // namespace mk
+
+extern "C" {
 
 int mk_graph_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
                     uint32_t n, void *stream) {

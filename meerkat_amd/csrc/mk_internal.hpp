@@ -247,6 +247,12 @@ int64_t tune_get(const char *name, int64_t dflt);
 // vdeg): set from the mirror's slabs now, then by every insert (queued on
 // the graph's stream).
 int mirror_flags_enable(mk_graph *rev);
+// Captured update steps (mk_step): a mutation queued without host waits,
+// and the outcome of its replay from the four counter words it published.
+int graph_queue_mutation(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
+                         const uint32_t *w, uint32_t n, uint32_t *host_dev, uint32_t *epoch);
+int graph_replay_outcome(mk_graph *g, int op, const uint32_t *src, const uint32_t *dst,
+                         uint32_t n, const uint32_t *words, uint32_t epoch);
 // Grid-stride kernels: the number of waves to launch on `device`.
 inline int grid_waves(int device) { return device_cus(device) * int(tune_get("waves_per_cu", 16)); }
 
@@ -303,6 +309,9 @@ struct mk_graph {
   // to date by its inserts, once a bottom-up BFS handle asked for them
   // (mirror_flags_enable); until then the mirror keeps no per-vertex array
   bool in_flags = false;
+  // captured update steps (mk_step) on this graph: at most one, since its
+  // replays use the graph's mutation epoch base (counters[1])
+  uint32_t steps = 0;
   uint32_t *counters = nullptr;  // [0] range-check epoch [2..3] pool failures (u64)
   uint32_t mut_epoch = 0;        // epoch of the last mutation call
   hipStream_t own_stream = nullptr;  // mutations given no stream run here

@@ -47,6 +47,10 @@ struct MsCounts {
   unsigned long long edges;        // adjacency entries read
   unsigned long long slabs;        // slabs read
   unsigned long long pairs;        // SSSP: (entry, active source) relaxations examined
+  // per slot: the sources with an entry in that slot's queue (the OR of the
+  // bits the step writing it set, and of admissions into it); a source
+  // missing from the queue of the next step has converged
+  unsigned long long active[kMsSlots];
 };
 
 struct MsView {
@@ -59,8 +63,18 @@ struct MsView {
   uint32_t log_cap;
   uint2 *ql[kMsSlots];  // hub lists
   MsCounts *c;
+  uint32_t *off;       // BFS: level at which each source slot was admitted
   uint32_t n;
 };
+
+MK_DEV uint64_t wave_or64(uint64_t x) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(x)), o, 64));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(x >> 32)), o, 64));
+    x |= (uint64_t(hi) << 32) | lo;
+  }
+  return x;
+}
 
 MK_DEV uint64_t read_lane64(uint64_t x, uint32_t l) {
   return (uint64_t(read_lane(uint32_t(x >> 32), l)) << 32) | read_lane(uint32_t(x), l);
@@ -115,7 +129,7 @@ MK_DEV void ms_push(const GraphView &G, const MsView &S, MsQ &q, bool push, uint
 // neighbour; a vertex entering level L+1 is pushed once (first bits).
 MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, MsQ &q, uint64_t key_mask_all,
                        uint32_t x, uint64_t mu, uint64_t *Fn, uint32_t nbase, uint32_t nslot,
-                       uint32_t &edges) {
+                       uint32_t &edges, uint64_t &acc) {
   const uint32_t lane = lane_id();
   const bool live = ((key_mask_all >> lane) & 1ull) && x < kTomb;
   edges += uint32_t(__popcll(ballot(live)));
@@ -125,7 +139,10 @@ MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, MsQ &q, uint64_t key
     if (nw) {
       const uint64_t old = atomicOr((unsigned long long *)(S.seen + x), (unsigned long long)nw);
       const uint64_t mine = nw & ~old;
-      if (mine) push = atomicOr((unsigned long long *)(Fn + x), (unsigned long long)mine) == 0;
+      if (mine) {
+        acc |= mine;
+        push = atomicOr((unsigned long long *)(Fn + x), (unsigned long long)mine) == 0;
+      }
     }
   }
   ms_push(G, S, q, push, x, S.logv, nbase, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
@@ -135,10 +152,10 @@ MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, MsQ &q, uint64_t key
 template <bool MAP>
 MK_DEV void msbfs_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint64_t mu,
                         uint64_t *Fn, uint32_t nbase, uint32_t nslot, uint32_t &edges,
-                        uint32_t &slabs) {
+                        uint32_t &slabs, uint64_t &acc) {
   for (;;) {
     ++slabs;
-    msbfs_slab(G, S, q, key_mask<MAP>(), x, mu, Fn, nbase, nslot, edges);
+    msbfs_slab(G, S, q, key_mask<MAP>(), x, mu, Fn, nbase, nslot, edges, acc);
     const uint32_t nx = read_lane(x, kNextLane);
     if (nx == kEmpty) break;
     x = slab_ptr(G.pool, nx)[lane_id()];
@@ -163,21 +180,25 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
     S.seg[L + 1] = nbase;
     S.c->cnt[pslot] = 0;
     S.c->large[pslot] = 0ull;
+    S.c->active[pslot] = 0ull;
     if (count) atomicAdd(&S.c->steps, 1u);
   }
-  if (count == 0) return;
   uint64_t *Fc = S.F[slot], *Fn = S.F[nslot], *Fp = S.F[pslot];
-  // after a log overflow (the run is repeated) only stored entries are read
-  const uint32_t end = min(nbase, S.log_cap);
-  // masks of level L-1 are no longer read: clear them for level L+2
+  // masks of level L-1 are no longer read: clear them for level L+2 (also
+  // when level L is empty: a source admitted later makes level L+1 run, and
+  // level L+2's masks must then start clear)
   if (L > 0) {
     const uint32_t ps = S.seg[L - 1];
     for (uint32_t i = ps + gw * 64 + lane; i < min(start, S.log_cap); i += W * 64)
       Fp[S.logv[i]] = 0ull;
   }
+  if (count == 0) return;
+  // after a log overflow (the run is repeated) only stored entries are read
+  const uint32_t end = min(nbase, S.log_cap);
   __shared__ uint32_t lqbuf[kBlock / 64][kMsWaveQ];
   MsQ q{lqbuf[threadIdx.x >> 6], 0u};
   uint32_t edges = 0, slabs = 0;
+  uint64_t acc = 0;  // source bits this lane set in the next level's masks
   // hubs: their list entries
   const uint2 *ql = S.ql[slot];
   // lane l takes bucket l % kHubChunk of hub-list entry l / kHubChunk
@@ -212,7 +233,8 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
 #pragma unroll
       for (uint32_t qq = 0; qq < 4; ++qq)
         if (qq < c)
-          msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges, slabs);
+          msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges, slabs,
+                           acc);
     }
   }
   // log entries: record the mask; expand non-hub vertices
@@ -251,11 +273,13 @@ __global__ void __launch_bounds__(kBlock) k_msbfs_level(GraphView G, MsView S, u
         for (uint32_t qq = 0; qq < 4; ++qq)
           if (qq < c)
             msbfs_chain<MAP>(G, S, q, x[qq], read_lane64(mu, js[qq]), Fn, nbase, nslot, edges,
-                             slabs);
+                             slabs, acc);
       }
     }
   }
   ms_flush(S, q, S.logv, nbase, &S.c->cnt[nslot], S.log_cap);
+  acc = wave_or64(acc);
+  if (lane == 0 && acc) atomicOr(&S.c->active[nslot], (unsigned long long)acc);
   if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
   if (lane == 0 && slabs) atomicAdd(&S.c->slabs, (unsigned long long)slabs);
 }
@@ -298,7 +322,7 @@ __global__ void k_msbfs_read(MsView S, uint32_t s, uint32_t total, uint32_t leve
     if (S.seg[mid] <= i) lo = mid;
     else hi = mid;
   }
-  out[S.logv[i]] = lo;
+  out[S.logv[i]] = lo - S.off[s];
 }
 
 // ------------------------------------------------------------------ SSSP
@@ -313,7 +337,7 @@ constexpr uint32_t kMsSparse = 4;
 template <bool MAP, uint32_t U>
 MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t du_lane,
                       uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges,
-                      uint64_t &pairs) {
+                      uint64_t &pairs, uint64_t &acc) {
   const uint32_t lane = lane_id();
   const uint64_t km = key_mask<MAP>();
   uint64_t live = ballot(x < kTomb) & km;
@@ -327,13 +351,16 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
       const uint32_t src = ffs64(m);
       const uint32_t du = read_lane(du_lane, src);
       if (du == kInf) continue;
-      bool push = false;
+      bool push = false, imp = false;
       if (lv) {
         const uint32_t cand = du + wv;
         uint32_t *dp = S.dist + size_t(x) * 64 + src;
-        if (cand >= du && cand < *dp && cand < atomicMin(dp, cand))
+        if (cand >= du && cand < *dp && cand < atomicMin(dp, cand)) {
+          imp = true;
           push = atomicOr((unsigned long long *)(Fn + x), 1ull << src) == 0;
+        }
       }
+      if (ballot(imp)) acc |= 1ull << src;
       ms_push(G, S, q, push, x, qn, 0, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot], S.n,
               false);
     }
@@ -375,7 +402,10 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
       old[qq] = imp[qq] ? atomicMin(S.dist + size_t(v[qq]) * 64 + lane, cand[qq]) : 0u;
     uint64_t im[U];
 #pragma unroll
-    for (uint32_t qq = 0; qq < U; ++qq) im[qq] = ballot(imp[qq] && cand[qq] < old[qq]);
+    for (uint32_t qq = 0; qq < U; ++qq) {
+      im[qq] = ballot(imp[qq] && cand[qq] < old[qq]);
+      acc |= im[qq];
+    }
     unsigned long long po[U];
 #pragma unroll
     for (uint32_t qq = 0; qq < U; ++qq)
@@ -399,11 +429,11 @@ MK_DEV void mssp_slab(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, u
 template <bool MAP, uint32_t U>
 MK_DEV void mssp_chain(const GraphView &G, const MsView &S, MsQ &q, uint32_t x, uint32_t u,
                        uint64_t mu, uint64_t *Fn, uint32_t *qn, uint32_t nslot, uint32_t &edges,
-                       uint32_t &slabs, uint64_t &pairs) {
+                       uint32_t &slabs, uint64_t &pairs, uint64_t &acc) {
   const uint32_t du = S.dist[size_t(u) * 64 + lane_id()];
   for (;;) {
     ++slabs;
-    mssp_slab<MAP, U>(G, S, q, x, du, mu, Fn, qn, nslot, edges, pairs);
+    mssp_slab<MAP, U>(G, S, q, x, du, mu, Fn, qn, nslot, edges, pairs, acc);
     const uint32_t nx = read_lane(x, kNextLane);
     if (nx == kEmpty) break;
     x = slab_ptr(G.pool, nx)[lane_id()];
@@ -429,6 +459,7 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S.c->cnt[zslot] = 0;
     S.c->large[zslot] = 0ull;
+    S.c->active[zslot] = 0ull;
   }
   // queue i-1 is done: clear its masks before its slot is reused
   const uint32_t pcount = i > 0 ? S.c->cnt[pslot] : 0u;
@@ -440,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
   __shared__ uint32_t lqbuf[kBlock / 64][kMsWaveQ];
   MsQ q{lqbuf[threadIdx.x >> 6], 0u};
   uint32_t edges = 0, slabs = 0;
-  uint64_t pairs = 0;
+  uint64_t pairs = 0, acc = 0;  // acc: source bits improved into the next queue
   const uint2 *ql = S.ql[slot];
   // lane l takes bucket l % kHubChunk of hub-list entry l / kHubChunk
   const uint32_t eper = min(per_wave(n_large, W), 64u / kHubChunk);
@@ -476,7 +507,7 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
       for (uint32_t qq = 0; qq < 4; ++qq)
         if (qq < c)
           mssp_chain<MAP, U>(G, S, q, x[qq], read_lane(u, js[qq]), read_lane64(mu, js[qq]), Fn,
-                             qn, nslot, edges, slabs, pairs);
+                             qn, nslot, edges, slabs, pairs, acc);
     }
   }
   const uint32_t per = per_wave(count, W);
@@ -499,11 +530,13 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
         m &= m - 1;
         const uint32_t x = G.heads[size_t(read_lane(b0, j) + kk) * kSlabWords + lane];
         mssp_chain<MAP, U>(G, S, q, x, read_lane(u, j), read_lane64(mu, j), Fn, qn, nslot, edges,
-                           slabs, pairs);
+                           slabs, pairs, acc);
       }
     }
   }
   ms_flush(S, q, qn, 0, &S.c->cnt[nslot], S.n);
+  acc = wave_or64(acc);  // wave-uniform already; the reduction keeps it exact
+  if (lane == 0 && acc) atomicOr(&S.c->active[nslot], (unsigned long long)acc);
   if (lane == 0 && edges) atomicAdd(&S.c->edges, (unsigned long long)edges);
   if (lane == 0 && slabs) atomicAdd(&S.c->slabs, (unsigned long long)slabs);
   if (lane == 0 && pairs) atomicAdd(&S.c->pairs, (unsigned long long)pairs);
@@ -519,6 +552,63 @@ __global__ void k_mssp_init(MsView S, const uint32_t *sources, uint32_t k) {
     S.dist[size_t(v) * 64 + s] = 0;
   }
   S.c->cnt[0] = cntv;
+}
+
+// Sources admitted into free slots of a running traversal (continuous
+// batching), before step `step`: the sources join that step's frontier
+// (queue slot `qs`: step % kMsSlots for SSSP, step % 3 for BFS) and their
+// bits mark the slot active.  SSSP: distance 0 in the source's lane.  BFS:
+// the source's bit in seen; its level 0 is `step`, recorded in off[slot].
+// Passed by value: no copy to stage.
+struct MsAdmit {
+  uint32_t v[64];
+  uint32_t slot[64];
+  uint32_t k;
+};
+
+// One wave: lane 0 appends in order (a vertex named twice, or already in
+// the frontier, is queued once), then the new hub vertices go to the hub
+// list together.
+__global__ void __launch_bounds__(64) k_ms_admit(GraphView G, MsView S, MsAdmit a, uint32_t step,
+                                                 uint32_t sssp) {
+  __shared__ uint32_t fresh[64];
+  const uint32_t t = threadIdx.x;
+  const uint32_t qs = sssp ? step % kMsSlots : step % 3;
+  if (t == 0) {
+    uint32_t cnt = S.c->cnt[qs];
+    const uint32_t base = sssp ? qs * S.n : S.seg[step];
+    uint64_t act = 0;
+    for (uint32_t i = 0; i < a.k; ++i) {
+      const uint32_t v = a.v[i], s = a.slot[i];
+      const uint64_t bit = 1ull << s;
+      fresh[i] = 0u;
+      if (S.F[qs][v] == 0) {
+        if (sssp || base + cnt < S.log_cap) S.logv[base + cnt] = v;
+        else S.c->overflow = 1u;
+        ++cnt;
+        fresh[i] = 1u;
+      }
+      S.F[qs][v] |= bit;
+      if (sssp) {
+        S.dist[size_t(v) * 64 + s] = 0;
+      } else {
+        S.seen[v] |= bit;
+        S.off[s] = step;
+      }
+      act |= bit;
+    }
+    S.c->cnt[qs] = cnt;
+    S.c->active[qs] |= act;
+  }
+  __syncthreads();
+  uint32_t v = 0, nb = 0;
+  bool hub = false;
+  if (t < a.k && fresh[t]) {
+    v = a.v[t];
+    nb = G.bucket_off[v + 1] - G.bucket_off[v];
+    hub = nb > kMsSmall;
+  }
+  push_hubs(G, S.ql[qs], &S.c->large[qs], hub, v, nb);
 }
 
 __global__ void k_mssp_read(const uint32_t *dist, uint32_t n, uint32_t s, uint32_t *out) {
@@ -542,7 +632,13 @@ struct MultiSource {
   uint2 *ql[kMsSlots] = {nullptr, nullptr, nullptr, nullptr};
   MsCounts *c = nullptr;
   uint32_t *src_dev = nullptr;
+  uint32_t *off = nullptr;  // BFS: admission level per slot
   HostBuf host;
+  // continuous batching (open / admit / launch / poll): the session's next
+  // step, the slots admitted since open, and whether a session is open
+  bool session = false;
+  uint32_t step = 0;
+  uint64_t admitted = 0;
   uint32_t k = 0;
   uint32_t last_steps = 0;
   uint64_t last_edges = 0;
@@ -563,6 +659,7 @@ struct MultiSource {
     v.seg = seg;
     v.log_cap = log_cap;
     v.c = c;
+    v.off = off;
     v.n = g->n;
     return v;
   }
@@ -596,6 +693,7 @@ struct MultiSource {
     }
     MK_HIP(hipMalloc(&c, sizeof(MsCounts)));
     MK_HIP(hipMalloc(&src_dev, 64 * 4));
+    MK_HIP(hipMalloc(&off, 64 * 4));
     if (sssp) {
       MK_HIP(hipMalloc(&dist, n * 64 * 4));
       MK_HIP(hipMalloc(&logv, kMsSlots * n * 4));  // one queue per slot
@@ -626,6 +724,7 @@ struct MultiSource {
     (void)hipFree(seg);
     (void)hipFree(c);
     (void)hipFree(src_dev);
+    (void)hipFree(off);
     host.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -646,12 +745,97 @@ struct MultiSource {
     const size_t n = g->n;
     for (uint32_t i = 0; i < kMsSlots; ++i) fill_u64(F[i], 0ull, n, stream);
     fill_u32(reinterpret_cast<uint32_t *>(c), 0u, sizeof(MsCounts) / 4, stream);
+    fill_u32(off, 0u, 64, stream);
     if (sssp) fill_u32(dist, kInf, n * 64, stream);
     else fill_u64(seen, 0ull, n, stream);
   }
 
+  // ---- continuous batching
+  int open() {
+    MK_HIP(hipSetDevice(g->device));
+    reset_state();
+    if (!sssp) fill_u32(seg, 0u, 1, stream);  // level 0 starts the log
+    MK_HIP(hipGetLastError());
+    session = true;
+    step = 0;
+    admitted = 0;
+    k = 0;
+    last_steps = 0;
+    last_edges = 0;
+    return MK_OK;
+  }
+
+  int admit(const uint32_t *sources, const uint32_t *slots, uint32_t n) {
+    if (!session) return fail(MK_ERR_STATE, "mk_msq_admit: no open session (mk_msq_open)");
+    MsAdmit a;
+    a.k = n;
+    uint64_t bits = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      if (slots[i] >= 64) return fail(MK_ERR_BAD_ARG, "mk_msq_admit: slot >= 64");
+      if (sources[i] >= g->n) return fail(MK_ERR_RANGE, "mk_msq_admit: source >= n");
+      const uint64_t bit = 1ull << slots[i];
+      if ((admitted | bits) & bit)
+        return fail(MK_ERR_STATE, "mk_msq_admit: slot already used in this session");
+      bits |= bit;
+      a.v[i] = sources[i];
+      a.slot[i] = slots[i];
+    }
+    if (n == 0) return MK_OK;
+    if (!sssp && step + 1 >= g->n + 2) return fail(MK_ERR_STATE, "mk_msq_admit: session too long");
+    MK_HIP(hipSetDevice(g->device));
+    k_ms_admit<<<1, 64, 0, stream>>>(g->view(), view(), a, step, sssp ? 1u : 0u);
+    MK_HIP(hipGetLastError());
+    admitted |= bits;
+    return MK_OK;
+  }
+
+  // Queue `steps` more steps of the session (they are no-ops once its
+  // frontier is empty); poll() waits for them.
+  int launch(uint32_t steps) {
+    if (!session) return fail(MK_ERR_STATE, "mk_msq_launch: no open session (mk_msq_open)");
+    MK_HIP(hipSetDevice(g->device));
+    const GraphView G = g->view();
+    const MsView S = view();
+    const unsigned grid = blocks_for_waves(uint64_t(max_waves));
+    for (uint32_t t = 0; t < steps; ++t, ++step) {
+      ProfScope prof(stream, sssp ? "k_mssp_step" : "k_msbfs_level", 0.0);
+      if (sssp) {
+        if (unroll == 8) {
+          if (g->weighted) k_mssp_step<true, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
+          else k_mssp_step<false, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
+        } else {
+          if (g->weighted) k_mssp_step<true, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
+          else k_mssp_step<false, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
+        }
+      } else {
+        if (step + 1 >= g->n + 2) return fail(MK_ERR_STATE, "multi-source BFS: session too long");
+        if (g->weighted) k_msbfs_level<true><<<grid, kBlock, 0, stream>>>(G, S, step);
+        else k_msbfs_level<false><<<grid, kBlock, 0, stream>>>(G, S, step);
+      }
+    }
+    MK_HIP(hipGetLastError());
+    return MK_OK;
+  }
+
+  // Wait for the queued steps; `active` = admitted slots with work left
+  // (an admitted slot not in it has its final distances).
+  int poll(uint64_t *active, uint32_t *steps_done) {
+    if (!session) return fail(MK_ERR_STATE, "mk_msq_poll: no open session (mk_msq_open)");
+    MsCounts h{};
+    if (int r = read_counts(h)) return r;
+    if (!sssp && h.overflow)
+      return fail(MK_ERR_STATE, "multi-source BFS: the level log is full; the slots still "
+                                "active are incomplete (open a new session for them)");
+    last_steps = h.steps;
+    last_edges = h.edges;
+    *active = h.active[step % (sssp ? kMsSlots : 3)] & admitted;
+    if (steps_done) *steps_done = step;
+    return MK_OK;
+  }
+
   int run(const uint32_t *sources, uint32_t nsrc, float *ms) {
     MK_HIP(hipSetDevice(g->device));
+    session = false;
     for (int attempt = 0; attempt < 4; ++attempt) {
       MK_HIP(hipEventRecord(ev0, stream));
       reset_state();
@@ -722,10 +906,20 @@ struct MultiSource {
   }
 
   int read(uint32_t slot, uint32_t *out) {
-    if (slot >= k) return fail(MK_ERR_RANGE, "mk_msq_read: slot not in the last run");
+    if (session ? (slot >= 64 || !((admitted >> slot) & 1ull)) : slot >= k)
+      return fail(MK_ERR_RANGE, session ? "mk_msq_read: slot not admitted in this session"
+                                        : "mk_msq_read: slot not in the last run");
     MK_HIP(hipSetDevice(g->device));
     DevBuf b;
     if (int r = b.ensure(size_t(g->n) * 4)) return r;
+    if (session && !sssp) {
+      // the log so far: levels 0 .. step - 1 are written
+      uint32_t total = 0;
+      MK_HIP(hipMemcpyAsync(&total, seg + step, 4, hipMemcpyDeviceToHost, stream));
+      MK_HIP(hipStreamSynchronize(stream));
+      log_total = std::min(total, log_cap);
+      last_steps = step;
+    }
     if (sssp) {
       k_mssp_read<<<blocks_for_threads(g->n), kBlock, 0, stream>>>(dist, g->n, slot,
                                                                    b.as<uint32_t>());
@@ -785,6 +979,42 @@ int mk_msq_run(mk_msq *q, const uint32_t *sources_host, uint32_t k, float *ms) {
     if (sources_host[i] >= q->m.g->n) return fail(MK_ERR_RANGE, "mk_msq_run: source >= n");
   if (int r = graph_settle(q->m.g)) return r;
   return q->m.run(sources_host, k, ms);
+}
+
+int mk_msq_open(mk_msq *q) {
+  if (!q) return fail(MK_ERR_BAD_ARG, "mk_msq_open: NULL handle");
+  if (int r = graph_settle(q->m.g)) return r;
+  return q->m.open();
+}
+
+int mk_msq_admit(mk_msq *q, const uint32_t *sources_host, const uint32_t *slots_host, uint32_t k) {
+  if (!q || (k && (!sources_host || !slots_host)))
+    return fail(MK_ERR_BAD_ARG, "mk_msq_admit: NULL argument");
+  if (k > 64) return fail(MK_ERR_BAD_ARG, "mk_msq_admit: at most 64 sources");
+  return q->m.admit(sources_host, slots_host, k);
+}
+
+int mk_msq_launch(mk_msq *q, uint32_t steps) {
+  if (!q) return fail(MK_ERR_BAD_ARG, "mk_msq_launch: NULL handle");
+  return q->m.launch(steps);
+}
+
+int mk_msq_ready(mk_msq *q, int *ready) {
+  if (!q || !ready) return fail(MK_ERR_BAD_ARG, "mk_msq_ready: NULL argument");
+  MK_HIP(hipSetDevice(q->m.g->device));
+  const hipError_t e = hipStreamQuery(q->m.stream);
+  if (e == hipErrorNotReady) {
+    *ready = 0;
+    return MK_OK;
+  }
+  if (e != hipSuccess) return hip_fail(e, "mk_msq_ready");
+  *ready = 1;
+  return MK_OK;
+}
+
+int mk_msq_poll(mk_msq *q, uint64_t *active, uint32_t *steps) {
+  if (!q || !active) return fail(MK_ERR_BAD_ARG, "mk_msq_poll: NULL argument");
+  return q->m.poll(active, steps);
 }
 
 int mk_msq_read(mk_msq *q, uint32_t slot, uint32_t *dist_host) {

@@ -119,6 +119,30 @@ __global__ void __launch_bounds__(256) k_publish_counts(const Counts *c, Counts 
   }
 }
 
+// A captured call's outcome flags (mk_step), ORed into `sticky` (device):
+// 1 a grid-barrier wait timed out, 2 the relax loop used up the steps it was
+// given (relax_end), 4 the invalidation loop did not drain (inval_end).
+__global__ void k_step_check(const Counts *c, uint32_t relax_end, uint32_t inval_end,
+                             uint32_t *sticky) {
+  uint32_t f = c->abort ? 1u : 0u;
+  if (relax_end && c->loop_end[0] == relax_end) f |= 2u;
+  if (inval_end && c->loop_end[1] >= inval_end) f |= 4u;
+  if (f) atomicOr(sticky, f);
+}
+
+// Replay bases of a captured update step: set (before the first replay) or
+// advanced by one replay's epochs (the step graph's last node).
+__global__ void k_step_bases(uint32_t *mut_base, uint32_t *trav_base, uint32_t m, uint32_t t,
+                             uint32_t add) {
+  if (add) {
+    *mut_base += m;
+    *trav_base += t;
+  } else {
+    *mut_base = m;
+    *trav_base = t;
+  }
+}
+
 // Host-side totals of a Counts snapshot.
 struct Totals {
   uint32_t small[3];
@@ -217,6 +241,10 @@ struct TravView {
   uint2 *far[3];
   uint32_t far_cap;
   uint32_t delta;
+  // added to every stamp epoch a kernel is given: zero for calls made
+  // through the API, the replay's base while a captured update step
+  // (mk_step) runs, so each replay stamps with epochs of its own
+  const uint32_t *ep_base;
 };
 
 MK_DEV uint32_t wave_min_u32(uint32_t x) {
@@ -905,6 +933,7 @@ MK_DEV void split_step(const GraphView &G, const TravView &S, uint32_t side, uin
 template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_expand(GraphView G, TravView S, uint32_t side, uint32_t slot, uint32_t epoch) {
+  epoch += *S.ep_base;
   __shared__ uint32_t lds[kExpandLds];
   expand_step<MAP, WEIGHTS, MODE>(G, S, side, slot, epoch, lds);
 }
@@ -968,6 +997,7 @@ template <bool MAP, bool WEIGHTS, int MODE>
 __global__ void __launch_bounds__(kWideBlock)
     k_expand_loop(GraphView G, TravView S, uint32_t j0, uint32_t j_end, uint32_t epoch0,
                   uint64_t timeout, uint32_t *zero, uint32_t nzero, uint32_t clear_exit) {
+  if (MODE == 0) epoch0 += *S.ep_base;
   __shared__ uint32_t lds[kLoopLds];
   __shared__ uint32_t flag;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += gridDim.x * blockDim.x)
@@ -1063,6 +1093,7 @@ __global__ void __launch_bounds__(kWideBlock)
 template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock)
     k_reseed(GraphView G, TravView S, uint32_t epoch) {
+  epoch += *S.ep_base;
   if (S.cnt->steps == 0) return;  // no invalidation step ran: nothing to re-seed
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1252,6 +1283,7 @@ MK_DEV void pull_buckets(const GraphView &G, const GraphView &R, const TravView 
 template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock)
     k_reseed_pull(GraphView G, GraphView R, TravView S, uint32_t epoch) {
+  epoch += *S.ep_base;
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
@@ -1338,6 +1370,7 @@ __global__ void __launch_bounds__(kBlock) k_inv_compact(TravView S, uint32_t nwo
 template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kBlock)
     k_pull_hubs(GraphView G, GraphView R, TravView S, uint32_t epoch) {
+  epoch += *S.ep_base;
   const uint32_t lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t W = (gridDim.x * blockDim.x) >> 6;
@@ -1383,6 +1416,7 @@ template <bool MAP, bool WEIGHTS>
 __global__ void __launch_bounds__(kItemBlock)
     k_seed_batch(GraphView G, TravView S, const uint32_t *src, const uint32_t *dst,
                  const uint32_t *wts, const uint8_t *status, uint32_t n, uint32_t epoch) {
+  epoch += *S.ep_base;
   __shared__ uint32_t sh[kItemBlock / 64 + 1];
   const uint32_t lane = lane_id();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1664,6 +1698,9 @@ struct Traversal {
   uint32_t far_cap = 0;
   uint32_t delta = 0;            // knob "sssp_delta" (0: no near-far)
   bool last_far_over = false;    // the last call's far list overflowed on a hub
+  uint32_t *ep_base = nullptr;   // device word added to stamp epochs (captured steps)
+  uint32_t *capture_sticky = nullptr;  // recording a step graph: its outcome word
+  uint32_t steps = 0;            // captured steps (mk_step) built on this handle
   bool delta_now = false;        // the launch being queued is a static SSSP's
   bool coop = false;             // loops always launched cooperatively (knob "coop_launch")
   unsigned loop_blocks = 0;      // resident-grid size of both loop kernels
@@ -1694,6 +1731,7 @@ struct Traversal {
     for (int i = 0; i < 3; ++i) v.far[i] = far[i];
     v.far_cap = far_cap;
     v.delta = (delta_now && far[0]) ? delta : 0u;
+    v.ep_base = ep_base;
     return v;
   }
   uint32_t bit_words() const { return (g->n + 31) / 32; }
@@ -1721,6 +1759,8 @@ struct Traversal {
       MK_HIP(hipMalloc(&ql[i], (n + buckets / kHubChunk + 64) * 8));
     }
     MK_HIP(hipMalloc(&cnt, bar_offset() + sizeof(BarLines)));
+    MK_HIP(hipMalloc(&ep_base, 64));
+    fill_u32(ep_base, 0u, 16, stream);
     bar = reinterpret_cast<BarLines *>(reinterpret_cast<char *>(cnt) + bar_offset());
     if (int r = host.ensure(sizeof(Counts) + 64)) return r;  // + the sequence word
     for (int i = 0; i < 2; ++i) {
@@ -1786,6 +1826,7 @@ struct Traversal {
       (void)hipFree(ql[i]);
     }
     (void)hipFree(cnt);
+    (void)hipFree(ep_base);
     host.release();
     for (int i = 0; i < 2; ++i) {
       snap[i].release();
@@ -2093,6 +2134,15 @@ struct Traversal {
   // kernels was measured to leave ~10 us of device idle.
   int begin(bool timed = true) {
     MK_HIP(hipSetDevice(g->device));
+    if (capture_sticky) {
+      // recorded into a captured update step: counters start clear
+      relax_end = 0;
+      inval_queued = false;
+      reseed_kind = 0;
+      clear_counts();
+      dirty = true;
+      return MK_OK;
+    }
     if (pending) return fail(MK_ERR_STATE, "internal: a deferred call was not settled");
     // a deferred mutation of the graph is ordered before this call's
     // kernels on the device; its outcome is settled in end()
@@ -2110,6 +2160,19 @@ struct Traversal {
   // incremental/decremental call (`may_defer`), once the copy is queued:
   // settle() reads them at the handle's next call, read or sync.
   int end(float *ms, bool may_defer = false) {
+    if (capture_sticky) {
+      // recorded into a captured update step: its outcome flags go to the
+      // step's sticky word, the counters are cleared for the next call
+      k_step_check<<<1, 1, 0, stream>>>(cnt, relax_end, inval_queued ? inval_limit() : 0u,
+                                        capture_sticky);
+      clear_counts();
+      dirty = false;
+      relax_end = 0;
+      inval_queued = false;
+      reseed_kind = 0;
+      if (ms) *ms = 0.f;
+      return MK_OK;
+    }
     if (may_defer && defer) {
       pend_seq = publish_counts();  // no events: the host waits for pend_seq
       int lr = MK_OK;
@@ -2494,6 +2557,7 @@ template <typename H> static int trav_create(mk_graph *g, uint32_t source, bool 
 
 template <typename H> static int trav_destroy(H *h) {
   if (!h) return fail(MK_ERR_BAD_ARG, "destroy: NULL handle");
+  if (h->t.steps) return fail(MK_ERR_STATE, "destroy: a captured step still uses the handle");
   h->t.g->users--;
   h->t.destroy();
   delete h;
@@ -2612,6 +2676,265 @@ int mk_sssp_set_deferred(mk_sssp *s, int on) {
 int mk_sssp_sync(mk_sssp *s) {
   if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_sync: NULL handle");
   return s->t.settle();
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- captured update steps
+// One update step of a BFS/SSSP handle, DeleteEdges -> Decremental ->
+// InsertEdges -> Incremental on fixed batch buffers, captured once into a
+// hipGraph and replayed (no reference counterpart: the reference launches
+// every kernel from the host, src/SSSP.cu:100-125).  The replays' kernels
+// take their stamp and mutation epochs relative to two device words (the
+// handle's ep_base and the graph's counters[1]); the host sets them before
+// the first replay of a run of replays and the graph's last node advances
+// them by one replay's epochs, so replays queued back to back never reuse an
+// epoch and need no host work between them.  Outcomes are collected on the
+// device: the mutation counters keep the epoch of the last failed batch, and
+// each call ORs its loop flags into a sticky word (k_step_check).
+struct mk_step {
+  mk::Traversal *t = nullptr;
+  const uint32_t *ds = nullptr, *dd = nullptr, *is = nullptr, *id = nullptr, *iw = nullptr;
+  uint32_t nd = 0, ni = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t *sticky = nullptr;     // device: OR of the calls' outcome flags
+  uint32_t mut0 = 0;              // the delete's captured mutation epoch (the insert's is +1)
+  uint32_t e0 = 0, span = 0;      // captured stamp epochs lie in (e0, e0 + span + stride)
+  uint32_t replays = 0;           // queued since the bases were last set
+  uint32_t mut_first = 0;         // effective mutation epoch of the first of them
+  bool bases_set = false;
+};
+
+namespace mk {
+static int step_capture(mk_step *st) {
+  Traversal &t = *st->t;
+  mk_graph *g = t.g;
+  hipStream_t s = t.stream;
+  MK_HIP(hipSetDevice(g->device));
+  // room for one capture's epochs, outside the capture (nothing wraps in it)
+  const uint64_t room = uint64_t(4) * (Traversal::kLoopSteps + 4) * kEpochStride;
+  if (uint64_t(t.epoch) + room > 0xFFFFFFFFull) {
+    fill_u32(t.stamp, 0u, g->n, s);
+    t.epoch = 0;
+  }
+  if (g->mut_epoch > 0xFFFFFF00u) {
+    fill_u32(g->counters, 0u, 4, s);
+    g->mut_epoch = 0;
+  }
+  if (int r = g->status.ensure(std::max<size_t>(std::max(st->nd, st->ni), 1))) return r;
+  if (g->rev)
+    if (int r = g->rev->status.ensure(std::max<size_t>(std::max(st->nd, st->ni), 1))) return r;
+  if (int r = g->host_scratch.ensure(64)) return r;
+  MK_HIP(hipMalloc(&st->sticky, 64));
+  fill_u32(st->sticky, 0u, 16, s);
+  MK_HIP(hipStreamSynchronize(s));
+  st->e0 = t.epoch;
+  MK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  t.capture_sticky = st->sticky;
+  uint32_t mi = 0;
+  int r = graph_queue_mutation(g, 2, st->ds, st->dd, nullptr, st->nd, nullptr, &st->mut0);
+  if (!r) r = t.run_decremental(st->ds, st->dd, st->nd, nullptr);
+  if (!r) r = graph_queue_mutation(g, 1, st->is, st->id, st->iw, st->ni, nullptr, &mi);
+  if (!r) r = t.run_incremental(st->is, st->id, st->iw, st->ni, nullptr);
+  t.capture_sticky = nullptr;
+  st->span = t.epoch - st->e0;
+  // the last node: the next replay's bases
+  k_step_bases<<<1, 1, 0, s>>>(g->counters + 1, t.ep_base, 2u, st->span, 1u);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (ec != hipSuccess) return hip_fail(ec, "mk_*_capture_step: stream capture");
+  if (mi != st->mut0 + 1) return fail(MK_ERR_STATE, "mk_*_capture_step: mutation epochs");
+  st->graph = graph;
+  MK_HIP(hipGraphInstantiate(&st->exec, graph, nullptr, nullptr, 0));
+  // nothing ran: the handle and graph hold what they held before capture
+  t.dirty = true;
+  g->last_op = 0;
+  return MK_OK;
+}
+
+// Queue one replay; the bases are set first when the previous replays were
+// settled (or none ran), else the last replay's final node has advanced them.
+static int step_launch(mk_step *st) {
+  Traversal &t = *st->t;
+  mk_graph *g = t.g;
+  hipStream_t s = t.stream;
+  MK_HIP(hipSetDevice(g->device));
+  if (int r = t.settle()) return r;
+  if (int r = graph_settle(g)) return r;
+  if (t.stale) return fail(MK_ERR_STATE, "mk_step_launch: results are invalid; run Static() first");
+  const uint64_t need = uint64_t(st->span) + 2 * kEpochStride;
+  const bool wrap_t = uint64_t(t.epoch) + need > 0xFFFFFFFFull;
+  const bool wrap_m = g->mut_epoch > 0xFFFFFF00u;
+  if (st->bases_set && (wrap_t || wrap_m)) {
+    // an epoch counter would wrap: settle the queued replays first
+    if (int r = mk_step_sync(st)) return r;
+  }
+  if (!st->bases_set) {
+    if (uint64_t(t.epoch) + need > 0xFFFFFFFFull) {
+      fill_u32(t.stamp, 0u, g->n, s);
+      t.epoch = 0;
+    }
+    if (g->mut_epoch > 0xFFFFFF00u) {
+      fill_u32(g->counters, 0u, 4, s);
+      g->mut_epoch = 0;
+    }
+    // replay k's epochs: captured + base (unsigned arithmetic wraps)
+    k_step_bases<<<1, 1, 0, s>>>(g->counters + 1, t.ep_base, g->mut_epoch + 1 - st->mut0,
+                                 t.epoch - st->e0, 0u);
+    MK_HIP(hipGetLastError());
+    st->bases_set = true;
+    st->replays = 0;
+    st->mut_first = g->mut_epoch + 1;
+  }
+  MK_HIP(hipGraphLaunch(st->exec, s));
+  t.epoch += st->span;
+  g->mut_epoch += 2;
+  ++st->replays;
+  t.dirty = false;  // each call of the step clears its counters
+  g->last_op = 0;
+  return MK_OK;
+}
+}  // namespace mk
+
+extern "C" {
+
+// Wait for the queued replays and report their outcome: MK_ERR_RANGE /
+// MK_ERR_POOL_FULL if a batch failed (the handle then needs Static()),
+// MK_ERR_STATE if a loop gave up; a relax loop that ran out of its steps is
+// recomputed from scratch (exact for the graph as it is).  The bases go back
+// to zero for the calls made through the API.
+int mk_step_sync(mk_step *st) {
+  if (!st) return fail(MK_ERR_BAD_ARG, "mk_step_sync: NULL step");
+  Traversal &t = *st->t;
+  mk_graph *g = t.g;
+  hipStream_t s = t.stream;
+  MK_HIP(hipSetDevice(g->device));
+  if (!st->bases_set) return MK_OK;
+  uint32_t words[8] = {0};
+  MK_HIP(hipMemcpyAsync(words, g->counters, 16, hipMemcpyDeviceToHost, s));
+  MK_HIP(hipMemcpyAsync(words + 4, st->sticky, 4, hipMemcpyDeviceToHost, s));
+  MK_HIP(hipStreamSynchronize(s));
+  k_step_bases<<<1, 1, 0, s>>>(g->counters + 1, t.ep_base, 0u, 0u, 0u);
+  fill_u32(st->sticky, 0u, 1, s);
+  MK_HIP(hipGetLastError());
+  const uint32_t first = st->mut_first, count = 2 * st->replays;
+  st->bases_set = false;
+  st->replays = 0;
+  auto in_run = [&](uint32_t e) { return uint32_t(e - first) < count; };
+  // a failed batch: its epoch is the last one recorded in the counters
+  if (in_run(words[0])) {
+    t.stale = true;
+    const bool last = words[0] == first + count - 1 || words[0] == first + count - 2;
+    return fail(MK_ERR_RANGE, std::string("captured step: a batch had a vertex id >= n (") +
+                                  (last ? "the last replay's" : "an earlier replay's") +
+                                  "); that batch changed nothing; run Static()");
+  }
+  if (in_run(words[3])) {
+    t.stale = true;
+    std::string msg = "captured step: the slab pool ran out inserting a batch";
+    if (words[3] == first + count - 1) {
+      // the last replay's insert: its statuses are the graph's last batch's
+      const uint32_t w4[4] = {0u, words[1], words[2], words[3]};
+      const int r = graph_replay_outcome(g, 1, st->is, st->id, st->ni, w4, words[3]);
+      return r ? r : fail(MK_ERR_POOL_FULL, msg + "; rolled back; run Static()");
+    }
+    return fail(MK_ERR_POOL_FULL, msg + " in an earlier replay, which stays partly applied");
+  }
+  const uint32_t flags = words[4];
+  if (flags & 5u) {
+    t.stale = true;
+    return fail(MK_ERR_STATE, (flags & 1u) ? "captured step: a grid-barrier wait timed out"
+                                           : "captured step: the frontier did not drain");
+  }
+  if (flags & 2u) return t.run_static(nullptr);  // a relax loop ran out of its steps
+  return MK_OK;
+}
+
+static int step_create(mk::Traversal *t, const uint32_t *ds, const uint32_t *dd, uint32_t nd,
+                       const uint32_t *is, const uint32_t *id, const uint32_t *iw, uint32_t ni,
+                       mk_step **out) {
+  if (!out) return fail(MK_ERR_BAD_ARG, "capture_step: out is NULL");
+  *out = nullptr;
+  if (!ds || !dd || !is || !id || nd == 0 || ni == 0)
+    return fail(MK_ERR_BAD_ARG, "capture_step: both batches must be non-empty device buffers");
+  if (!t->persistent) return fail(MK_ERR_STATE, "capture_step: needs persistent_steps = 1");
+  if (profiling_on()) return fail(MK_ERR_STATE, "capture_step: profiling must be off");
+  if (int r = t->settle()) return r;
+  if (int r = graph_settle(t->g)) return r;
+  if (t->stale) return fail(MK_ERR_STATE, "capture_step: results are invalid; run Static() first");
+  if (t->g->steps) return fail(MK_ERR_STATE, "capture_step: the graph has a captured step already");
+  mk_step *st = new mk_step();
+  st->t = t;
+  st->ds = ds;
+  st->dd = dd;
+  st->nd = nd;
+  st->is = is;
+  st->id = id;
+  st->iw = iw;
+  st->ni = ni;
+  if (int r = step_capture(st)) {
+    (void)hipFree(st->sticky);
+    delete st;
+    return r;
+  }
+  t->g->steps++;
+  t->steps++;
+  *out = st;
+  return MK_OK;
+}
+
+int mk_sssp_capture_step(mk_sssp *s, const uint32_t *del_src, const uint32_t *del_dst,
+                         uint32_t n_del, const uint32_t *ins_src, const uint32_t *ins_dst,
+                         const uint32_t *ins_w, uint32_t n_ins, mk_step **out) {
+  if (!s) return fail(MK_ERR_BAD_ARG, "mk_sssp_capture_step: NULL handle");
+  return step_create(&s->t, del_src, del_dst, n_del, ins_src, ins_dst, ins_w, n_ins, out);
+}
+
+int mk_bfs_capture_step(mk_bfs *b, const uint32_t *del_src, const uint32_t *del_dst,
+                        uint32_t n_del, const uint32_t *ins_src, const uint32_t *ins_dst,
+                        uint32_t n_ins, mk_step **out) {
+  if (!b) return fail(MK_ERR_BAD_ARG, "mk_bfs_capture_step: NULL handle");
+  return step_create(&b->t, del_src, del_dst, n_del, ins_src, ins_dst, nullptr, n_ins, out);
+}
+
+int mk_step_launch(mk_step *st) {
+  if (!st) return fail(MK_ERR_BAD_ARG, "mk_step_launch: NULL step");
+  return step_launch(st);
+}
+
+int mk_step_load(mk_step *st, const uint32_t *del_src, const uint32_t *del_dst,
+                 const uint32_t *ins_src, const uint32_t *ins_dst, const uint32_t *ins_w) {
+  if (!st || !del_src || !del_dst || !ins_src || !ins_dst)
+    return fail(MK_ERR_BAD_ARG, "mk_step_load: NULL argument");
+  if (ins_w && !st->iw) return fail(MK_ERR_BAD_ARG, "mk_step_load: the step has no weights");
+  hipStream_t s = st->t->stream;
+  MK_HIP(hipSetDevice(st->t->g->device));
+  const size_t bd = size_t(st->nd) * 4, bi = size_t(st->ni) * 4;
+  MK_HIP(hipMemcpyAsync(const_cast<uint32_t *>(st->ds), del_src, bd, hipMemcpyDeviceToDevice, s));
+  MK_HIP(hipMemcpyAsync(const_cast<uint32_t *>(st->dd), del_dst, bd, hipMemcpyDeviceToDevice, s));
+  MK_HIP(hipMemcpyAsync(const_cast<uint32_t *>(st->is), ins_src, bi, hipMemcpyDeviceToDevice, s));
+  MK_HIP(hipMemcpyAsync(const_cast<uint32_t *>(st->id), ins_dst, bi, hipMemcpyDeviceToDevice, s));
+  if (ins_w)
+    MK_HIP(hipMemcpyAsync(const_cast<uint32_t *>(st->iw), ins_w, bi, hipMemcpyDeviceToDevice, s));
+  return MK_OK;
+}
+
+int mk_step_destroy(mk_step *st) {
+  if (!st) return fail(MK_ERR_BAD_ARG, "mk_step_destroy: NULL step");
+  const int r = mk_step_sync(st);
+  (void)hipSetDevice(st->t->g->device);
+  if (st->exec) (void)hipGraphExecDestroy(st->exec);
+  if (st->graph) (void)hipGraphDestroy(st->graph);
+  (void)hipFree(st->sticky);
+  st->t->g->steps--;
+  st->t->steps--;
+  delete st;
+  return r;
 }
 
 }  // extern "C"

@@ -3,13 +3,13 @@
 Requests arrive on a timeline (seconds from the start): update batches
 (insertions + deletions) and single-source BFS / SSSP queries.  Two modes:
 
-- ``batched`` (continuous batching): whenever the device is free, every
-  update batch that has arrived and fits the slab pool's free memory is
-  applied (in arrival order), then up to 64 waiting BFS queries and up to 64
-  waiting SSSP queries run as two multi-source traversals on two HIP streams
-  at the same time (each ``MultiSource.run`` releases the GIL, so both are in
-  flight together).  Requests that arrive while a round runs join the next
-  round.
+- ``batched`` (continuous batching): BFS and SSSP queries are admitted
+  into free slots (up to 64) of a multi-source traversal of their kind that
+  is already running, joining its next step, and each is done on the step
+  its own traversal finishes; the two kinds' steps run on two HIP streams
+  at once.  Update batches are applied, in arrival order and while they fit
+  the slab pool's free memory, once no traversal runs; while one waits, no
+  query is admitted, so the running traversals drain first.
 - ``sequential``: one request at a time in arrival order, queries by the
   single-source static BFS / SSSP (the one-at-a-time incremental path).
 
@@ -19,7 +19,6 @@ equals the same query run alone on that version.  No reference counterpart:
 the reference has no serving loop.
 """
 import collections
-import threading
 import time
 
 import numpy as np
@@ -122,20 +121,6 @@ class ServingLoop:
         self.phase_s["updates"] += time.perf_counter() - t
 
     # ------------------------------------------------------------ queries
-    def _run_queries(self, engine, reqs, now):
-        if not reqs:
-            return
-        t_start = now()
-        for r in reqs:
-            r.start = t_start  # one launch for the whole batch
-            r.version = self.version
-        engine.run(np.array([r.src for r in reqs], np.uint32))
-        t = now()
-        for slot, r in enumerate(reqs):
-            if self._keeps(r):
-                r.result = engine.distances(slot)
-            r.done = t
-
     def _run_single(self, r, now):
         eng = self.bfs1 if r.kind == "bfs" else self.sssp1
         r.start = now()
@@ -171,47 +156,118 @@ class ServingLoop:
                     self.phase_s["queries"] += time.perf_counter() - tq
             return reqs
 
+        return self._run_batched(reqs, now)
+
+    def _run_batched(self, reqs, now):
+        """Continuous batching: each query kind has a multi-source engine
+        running a session into which waiting queries are admitted, up to 64
+        slots, while it runs (mk_msq_admit between steps); every query is
+        done on the step its traversal finishes (mk_msq_poll), not when the
+        whole batch does.  Each engine keeps one step in flight on its own
+        stream, so BFS and SSSP work overlaps on the device and neither kind
+        waits for the other's steps.  An update batch waits until
+        no session runs (a session answers on one graph version), and while
+        one waits no query is admitted, so the sessions drain; the updates
+        waiting then are applied in arrival order while they fit the slab
+        pool, and admission resumes."""
+        engines = {"bfs": _Session(self.bfs)}
+        if self.sssp is not None:
+            engines["sssp"] = _Session(self.sssp)
         pending_u = collections.deque()
-        pending = {"bfs": collections.deque(), "sssp": collections.deque()}
+        waiting = {k: collections.deque() for k in engines}
         i, n_done, total = 0, 0, len(reqs)
         while n_done < total:
             t = now()
             while i < total and reqs[i].arrival <= t:
-                (pending_u if reqs[i].kind == "update" else pending[reqs[i].kind]).append(reqs[i])
+                r = reqs[i]
+                (pending_u if r.kind == "update" else waiting[r.kind]).append(r)
                 i += 1
-            if not pending_u and not pending["bfs"] and not pending["sssp"]:
-                if i < total:
+            running = [e for e in engines.values() if e.running]
+            blocked = False  # the next update does not fit the pool's free slabs
+            if pending_u and not running:
+                tu = time.perf_counter()
+                while pending_u:
+                    r = pending_u[0]
+                    if not self._admit(r):
+                        self.rejected_rounds += 1
+                        if not any(waiting.values()):
+                            raise RuntimeError("update batch does not fit the slab pool")
+                        blocked = True
+                        break
+                    pending_u.popleft()
+                    r.admitted = now()
+                    self._apply(r, now)
+                    n_done += 1
+                self.phase_s["updates"] += time.perf_counter() - tu
+            if not pending_u or blocked:
+                for kind, e in engines.items():
+                    if waiting[kind] and e.free:
+                        e.admit(waiting[kind], self.version, now)
+            running = [e for e in engines.values() if e.running]
+            if not running:
+                if i < total and not pending_u and not any(waiting.values()):
                     while now() < reqs[i].arrival:
                         pass
                 continue
-            # updates first, in arrival order, while they fit the pool
-            while pending_u:
-                r = pending_u[0]
-                if not self._admit(r):
-                    self.rejected_rounds += 1
-                    if not pending["bfs"] and not pending["sssp"]:
-                        raise RuntimeError("update batch does not fit the slab pool")
-                    break
-                pending_u.popleft()
-                r.admitted = now()
-                self._apply(r, now)
-                n_done += 1
-            # then one round of queries: BFS and SSSP batches overlap
-            qb = [pending["bfs"].popleft() for _ in range(min(self.MAX_BATCH, len(pending["bfs"])))]
-            qs = [pending["sssp"].popleft()
-                  for _ in range(min(self.MAX_BATCH, len(pending["sssp"])))]
+            # each engine keeps one step in flight and is polled when that
+            # step has run: a kind with short steps (BFS) does not wait for
+            # the other kind's long ones
             tq = time.perf_counter()
-            if qb and qs:
-                th = threading.Thread(target=self._run_queries, args=(self.sssp, qs, now))
-                th.start()
-                self._run_queries(self.bfs, qb, now)
-                th.join()
-            else:
-                self._run_queries(self.bfs, qb, now)
-                self._run_queries(self.sssp, qs, now)
+            for e in running:
+                if not e.inflight:
+                    e.engine.launch(1)
+                    e.inflight = True
+                elif e.engine.ready():
+                    n_done += e.poll(now, self._keeps)
             self.phase_s["queries"] += time.perf_counter() - tq
-            n_done += len(qb) + len(qs)
         return reqs
+
+
+class _Session:
+    """One engine's continuous-batching session: slot bookkeeping."""
+
+    def __init__(self, engine):
+        self.engine = engine
+        self.slots = {}       # slot -> request still running
+        self.used = 0         # slots taken since the session opened
+        self.running = False
+        self.inflight = False  # a launched step not yet polled
+
+    @property
+    def free(self):
+        return MultiSource.MAX_SOURCES - self.used if self.running else MultiSource.MAX_SOURCES
+
+    def admit(self, queue, version, now):
+        if not self.running:
+            self.engine.open()
+            self.running = True
+            self.used = 0
+        take = [queue.popleft() for _ in range(min(self.free, len(queue)))]
+        slots = list(range(self.used, self.used + len(take)))
+        self.engine.admit(np.array([r.src for r in take], np.uint32), np.array(slots, np.uint32))
+        t = now()
+        for s, r in zip(slots, take):
+            r.start = t
+            r.version = version
+            self.slots[s] = r
+        self.used += len(take)
+
+    def poll(self, now, keeps):
+        """Read the launched step's outcome (it has run); complete the slots
+        that finished.  Returns how many did; the session ends when none is
+        left running."""
+        active, _ = self.engine.poll()
+        self.inflight = False
+        t = now()
+        done = [s for s in self.slots if not (active >> s) & 1]
+        for s in done:
+            r = self.slots.pop(s)
+            if keeps(r):
+                r.result = self.engine.distances(s)
+            r.done = t
+        if not self.slots:
+            self.running = False
+        return len(done)
 
 
 def make_stream(n, keys, count, rate, seed=9, update_frac=0.3, median=10_000, max_size=1_000_000,

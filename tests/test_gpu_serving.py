@@ -110,3 +110,28 @@ def test_sequential_mode_answers_match_oracle(gpu):
         ref = versions[r.version]
         exp = oracle.bfs(ref, r.src) if r.kind == "bfs" else oracle.dijkstra(ref, r.src)
         assert np.array_equal(r.result, exp), (r.kind, r.src, r.version)
+
+
+def test_queries_join_running_traversals_and_finish_on_their_own(gpu):
+    # queries only, arriving faster than a traversal finishes: later ones are
+    # admitted into the traversal already running, and each finishes on its
+    # own step (a source without out-edges at once), exact
+    n, s, d, w, pool, g = build(13, 44)
+    keys = gen.keys_of(s, d)
+    reqs = serving.make_stream(n, keys, 120, rate=40000.0, seed=9, update_frac=0.0)
+    loop = serving.ServingLoop(g, keep_results=True)
+    done = loop.run(reqs, mode="batched")
+    ref = oracle.csr(n, s, d, w)
+    for r in done:
+        exp = oracle.bfs(ref, r.src) if r.kind == "bfs" else oracle.dijkstra(ref, r.src)
+        assert np.array_equal(r.result, exp), (r.kind, r.src)
+    joined = 0
+    for kind in ("bfs", "sssp"):
+        rs = [r for r in done if r.kind == kind]
+        joined += sum(1 for a in rs for b in rs if a.start < b.start < a.done)
+        # within one admission, traversals end at different steps
+        by_start = {}
+        for r in rs:
+            by_start.setdefault(r.start, set()).add(r.done)
+        assert any(len(v) > 1 for v in by_start.values())
+    assert joined > 0
