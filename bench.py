@@ -364,7 +364,8 @@ def main():
     # per-kernel table: a few more with every kernel timed
     profiling.reset()
     profiling.enable(True)
-    for i in range(rate_end, steps_total):
+    prof_end = rate_end + args.breakdown_steps
+    for i in range(rate_end, prof_end):
         step(i)
     settle()
     profiling.enable(False)
@@ -374,7 +375,7 @@ def main():
     # queued back to back, one host wait at the end
     captured = None
     if args.graph_steps:
-        i0 = args.warmup + args.steps + 2 * args.breakdown_steps
+        i0 = prof_end  # batches no earlier step applied
         step_graph = sp.capture_step(*dels[i0], *inss[i0])
         torch.cuda.synchronize()
         tg = time.perf_counter()
