@@ -16,19 +16,21 @@ timed region ends once the last one is settled); --sync-calls makes each
 wait for its own outcome.
 
 value = edge updates per second over the timed steps, shortest paths
-maintained (B * K / seconds).  Also reported: update-only throughput,
-incremental/decremental SSSP and incremental BFS traversed-edge rates,
-per-step latency percentiles, the roofline of the dominant kernel (timing
-inside the library over the timed region: the device clock of the persistent
-frontier loops), a CPU baseline (the
-oracle's multithreaded static SSSP recompute on the same graph, all host
-threads) and a
-serving-loop phase (continuous batching vs one request at a time on the same
-request mix; query latency percentiles under a Poisson load).
+maintained (B * K / seconds).  Also reported: the same steps with every call
+waiting for its own outcome (the reference's protocol) and replayed from a
+hipGraph-captured step; update-only throughput; incremental/decremental SSSP
+and incremental BFS traversed-edge rates; per-step latency percentiles; the
+roofline of the dominant kernel (timing inside the library over the timed
+region: the device clock of the persistent frontier loops); the final
+distances checked against the oracle; a CPU baseline (the oracle's
+multithreaded static SSSP recompute on the same graph, all host threads);
+and a serving phase on a fresh RMAT-24 graph (continuous batching vs one
+request at a time on the same request mix; latency percentiles at several
+offered loads).
 
   python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--scale 24]
                   [--batch 1000000] [--no-cpu-baseline] [--sync-mutations]
-                  [--sync-calls]
+                  [--sync-calls] [--graph-steps 10] [--serve-loads 0.1,0.25,0.5]
 """
 import argparse
 import json
@@ -107,7 +109,7 @@ def verify_sssp(n, s, d, w, del_sel, ins_s, ins_d, ins_w, dist, parent, source=0
     return out
 
 
-def run_serving(args, n, g, s, d, del_idx, used, log):
+def run_serving(args, n, g, s, d, log):
     """Serving-loop phase on the resident graph: the same request mix served
     one request at a time (single-source static BFS / SSSP) and with
     continuous batching (queries admitted into running 64-slot multi-source
@@ -122,13 +124,10 @@ def run_serving(args, n, g, s, d, del_idx, used, log):
 
     from meerkat_amd import gen, serving
 
-    # deletions for the serving streams come from initial edges the update
-    # steps did not delete
-    keep = np.ones(len(s), bool)
-    keep[del_idx[:used]] = False
-    avail = np.nonzero(keep)[0]
+    # deletions for the serving streams come from the graph's edges, a
+    # disjoint share per run
     rng = np.random.default_rng(args.seed + 7)
-    rng.shuffle(avail)
+    avail = rng.permutation(len(s))
     loads = [float(x) for x in args.serve_loads.split(",")]
     parts = np.array_split(avail[: min(len(avail), 40_000_000)], 2 + len(loads))
     R = args.serve_requests
@@ -144,6 +143,7 @@ def run_serving(args, n, g, s, d, del_idx, used, log):
 
     loop = serving.ServingLoop(g)
     res = {}
+    # the serving runs change the graph in turn (each its own update batches)
     for i, mode in enumerate(("sequential", "batched")):
         reqs = stream(i, 1.0, True)
         t0 = time.perf_counter()
@@ -205,7 +205,7 @@ def main():
     ap.add_argument("--no-verify", action="store_true",
                     help="skip the check of the final SSSP distances against the oracle")
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-loads", default="0.25,0.5",
+    ap.add_argument("--serve-loads", default="0.1,0.25,0.5",
                     help="offered loads of the latency runs, as fractions of the batched "
                          "capacity measured first")
     ap.add_argument("--no-kernel-timing", action="store_true",
@@ -418,15 +418,26 @@ def main():
         if ms > 0:
             bfs_rates.append(er / (ms * 1e-3) / 1e9)
 
-    serving_out = None
-    if not args.no_serving:
-        serving_out = run_serving(args, n, g, s, d, del_idx, steps_total * half, log)
     # every handle closed here, in reverse order of creation: nothing is left
     # to release at interpreter exit
     bfs.close()
     sp.close()
     g.close()
     pool.close()
+    serving_out = None
+    if not args.no_serving:
+        # the serving configuration is the RMAT graph as generated, updated
+        # only by the serving stream's own batches: the update steps above
+        # added 20M+ uniformly random edges, which give most of the 56% of
+        # vertices without out-edges some (their queries stop being trivial)
+        # and double the serving work
+        pool = mk.SlabPool(16 << 30)
+        g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=True,
+                            in_degree_hints=gen.degree_hints(n, d))
+        g.InsertEdges(s, d, w)
+        serving_out = run_serving(args, n, g, s, d, log)
+        g.close()
+        pool.close()
 
     K = args.steps
     ms_per_step = elapsed / K * 1e3

@@ -55,7 +55,7 @@ constexpr uint32_t kTraceSteps = 64;  // steps of a loop launch recorded while p
 enum { kStatSlabs = 0, kStatEdges = 1, kStatVisited = 2, kStatPushed = 3 };
 // statistics sets, one per kind of kernel: frontier steps relaxing (MODE 0),
 // invalidating (MODE 1), pull/bucket re-seeds (MODE 2), batch seeds
-enum { kSetRelax = 0, kSetInvalidate = 1, kSetReseed = 2, kSetSeed = 3, kSets = 4 };
+enum { kSetRelax = 0, kSetInvalidate = 1, kSetReseed = 2, kSetSeed = 3, kSetHubs = 4, kSets = 5 };
 
 // Device counters.  Frontier slots rotate per step; the statistics
 // accumulate over one compute call, each block adding to slot
@@ -77,7 +77,7 @@ struct Counts {
   uint32_t far_over;              // a deferred hub did not fit its far list
   unsigned long long large[3];    // (entries << 32) | bucket items
   unsigned long long pull_large;  // k_reseed_pull's hub list (entries << 32) | items
-  unsigned long long stats[kStatSlots][16];  // [slot][set * 4 + which], one line per slot
+  unsigned long long stats[kStatSlots][kSets * 4];  // [slot][set * 4 + which], a line per slot
   // step trace of k_expand_loop<MODE> while profiling: the device clock at
   // the start of step k (k = steps: the loop's end) and its frontier
   unsigned long long trace_t[2][kTraceSteps + 1];
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) k_publish_counts(const Counts *c, Counts 
   uint32_t *dst = reinterpret_cast<uint32_t *>(out);
   constexpr uint32_t head = offsetof(Counts, stats) / 4;
   for (uint32_t i = t; i < head; i += blockDim.x) dst[i] = src[i];
-  if (t < 16) {
+  if (t < kSets * 4) {
     unsigned long long x = 0;
     for (uint32_t k = 0; k < kStatSlots; ++k) x += c->stats[k][t];
     out->stats[0][t] = x;
@@ -1378,7 +1378,7 @@ __global__ void __launch_bounds__(kBlock)
   if (n_large == 0) return;
   __shared__ uint32_t lq[kWaveLds];
   WaveQ q{lq + (threadIdx.x >> 6) * kWaveQ, 0u, 0u};
-  uint32_t slabs = 0, edges = 0;
+  uint32_t slabs = 0, edges = 0, items = 0;
   const uint2 *ql = S.q_large[1];
   // lane l takes in-bucket l % kHubChunk of entry l / kHubChunk
   const uint32_t eper = min(per_wave(n_large, W), 64u / kHubChunk);
@@ -1392,13 +1392,15 @@ __global__ void __launch_bounds__(kBlock)
       b = ent.y + lane % kHubChunk;
       act = b < R.bucket_off[v + 1];
     }
+    items += uint32_t(__popcll(ballot(act)));
     pull_buckets<MAP, WEIGHTS>(G, R, S, q, epoch, ballot(act), b, v, slabs, edges);
   }
   block_flush(q, S.q_small[0], &S.cnt->small[0], lq + (kBlock / 64) * kWaveQ);
   if (lane == 0) {
-    add_stat(S.cnt, kSetReseed, kStatSlabs, slabs);
-    add_stat(S.cnt, kSetReseed, kStatEdges, edges);
-    add_stat(S.cnt, kSetReseed, kStatPushed, q.pushed);
+    add_stat(S.cnt, kSetHubs, kStatSlabs, slabs);
+    add_stat(S.cnt, kSetHubs, kStatEdges, edges);
+    add_stat(S.cnt, kSetHubs, kStatVisited, items);
+    add_stat(S.cnt, kSetHubs, kStatPushed, q.pushed);
   }
 }
 
@@ -2056,14 +2058,21 @@ struct Traversal {
       if (relax_end) prof_add_bytes("k_expand_relax", steps_bytes(kSetRelax));
       if (inval_queued) prof_add_bytes("k_expand_invalidate", steps_bytes(kSetInvalidate));
       const unsigned long long *r = h.set[kSetReseed];
-      if (reseed_kind == 1)
+      if (reseed_kind == 1) {
         // 12 B per listed vertex (entry, in-bucket range), 256 B per
         // in-slab, 4 + 8 B per in-neighbour (invalid bit, distance), 16 B
-        // per re-seeded vertex
+        // per re-seeded vertex; the hubs' in-buckets likewise, 12 B per
+        // bucket work item
         prof_add_bytes("k_reseed_pull", 12.0 * double(r[kStatVisited]) +
                                             256.0 * double(r[kStatSlabs]) +
                                             12.0 * double(r[kStatEdges]) +
                                             16.0 * double(r[kStatPushed]));
+        const unsigned long long *hb = h.set[kSetHubs];
+        prof_add_bytes("k_pull_hubs", 12.0 * double(hb[kStatVisited]) +
+                                          256.0 * double(hb[kStatSlabs]) +
+                                          12.0 * double(hb[kStatEdges]) +
+                                          16.0 * double(hb[kStatPushed]));
+      }
       else if (reseed_kind == 2 && h.set[kSetInvalidate][kStatVisited])
         // 20 B per bucket (owner, degree, distance, invalid bit of the
         // owner), 256 B per slab read, 4 B per key (invalid bit), 16 B per

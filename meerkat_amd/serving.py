@@ -68,13 +68,16 @@ def _stage(req, device):
 class ServingLoop:
     MAX_BATCH = MultiSource.MAX_SOURCES
 
-    def __init__(self, graph, keep_results=False, slab_margin=1.25):
+    def __init__(self, graph, keep_results=False, slab_margin=1.25, lockstep=False):
         """keep_results: True keeps every query's distances on the host
         (``Request.result``), a callable keeps those of the requests it
         accepts (a sample, on graphs where each answer is large)."""
         self.g = graph
         self.keep_results = keep_results
         self.slab_margin = slab_margin
+        # lockstep: both engines take one step, then the loop waits for both;
+        # otherwise each is polled when its own step has run
+        self.lockstep = lockstep
         self.bfs = MultiSource(graph, sssp=False)
         self.sssp = MultiSource(graph, sssp=True) if graph.weighted else None
         # one-at-a-time mode answers with the single-source engines
@@ -213,12 +216,20 @@ class ServingLoop:
             # step has run: a kind with short steps (BFS) does not wait for
             # the other kind's long ones
             tq = time.perf_counter()
-            for e in running:
-                if not e.inflight:
+            if self.lockstep:
+                # one step of every engine, then wait for each
+                for e in running:
                     e.engine.launch(1)
                     e.inflight = True
-                elif e.engine.ready():
+                for e in running:
                     n_done += e.poll(now, self._keeps)
+            else:
+                for e in running:
+                    if not e.inflight:
+                        e.engine.launch(1)
+                        e.inflight = True
+                    elif e.engine.ready():
+                        n_done += e.poll(now, self._keeps)
             self.phase_s["queries"] += time.perf_counter() - tq
         return reqs
 
