@@ -415,8 +415,11 @@ int mk_msq_last_stats(mk_msq *q, uint32_t *steps, uint64_t *edges_read);
  *    the steps queued so far: an admitted slot not in *active is final, and
  *    mk_msq_read(q, slot, ...) returns its distances.
  * mk_msq_run ends any session.  A BFS session whose level log fills up
+ * (sources admitted at many different steps put a vertex in many levels)
  * returns MK_ERR_STATE from mk_msq_poll; its slots still active are then
- * incomplete and belong in a new session. */
+ * incomplete and belong in a new session, which gets a log four times
+ * larger, up to the 64 entries per vertex a session can need; the slots
+ * reported finished stay readable until then. */
 int mk_msq_open(mk_msq *q);
 int mk_msq_admit(mk_msq *q, const uint32_t *sources_host, const uint32_t *slots_host, uint32_t k);
 int mk_msq_launch(mk_msq *q, uint32_t steps);

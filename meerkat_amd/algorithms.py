@@ -402,7 +402,11 @@ class MultiSource:
         return bool(r.value)
 
     def poll(self):
-        """Wait for the launched steps; returns (active slot mask, steps so far)."""
+        """Wait for the launched steps; returns (active slot mask, steps so far).
+        Raises MeerkatError (MK_ERR_STATE) when a BFS session's level log
+        has filled up: its active slots are incomplete and belong in a new
+        session (with a larger log); the finished slots stay readable until
+        then."""
         a, st = ctypes.c_uint64(), ctypes.c_uint32()
         _lib.call("mk_msq_poll", self._h, ctypes.byref(a), ctypes.byref(st))
         return a.value, st.value

@@ -8,6 +8,9 @@ handles created afterwards:
 - ``ms_unroll`` (4 or 8, default 8): neighbour distance rows each wave keeps
   in flight in multi-source SSSP (more rows hide more HBM latency at the
   cost of registers).
+- ``ms_waves_per_cu`` (1..64, default 24): waves per compute unit of the
+  multi-source BFS / SSSP steps (random row reads: more waves keep more in
+  flight, up to what a CU holds).
 - ``pull_enter`` / ``pull_exit`` (defaults 14 / 24): the push/pull
   thresholds of static BFS over a graph with the in-edge mirror; a step runs
   bottom-up once the frontier holds more than 1/pull_enter of the buckets
@@ -25,7 +28,8 @@ import numpy as np
 
 from . import _lib
 
-KNOBS = {"waves_per_cu": (1, 64), "ms_unroll": (4, 8), "orphan_repair": (0, 1),
+KNOBS = {"waves_per_cu": (1, 64), "ms_unroll": (4, 8), "ms_waves_per_cu": (1, 64),
+         "orphan_repair": (0, 1),
          "persistent_steps": (0, 1), "tail_enter": (0, 1024), "coop_launch": (0, 1),
          "mut_waves_per_cu": (1, 256), "barrier_timeout_ms": (0, 600000),
          "pull_enter": (0, 4096), "pull_exit": (1, 4096), "sssp_delta": (0, 1 << 24),
@@ -58,7 +62,7 @@ def _time(make, run, repeats):
 
 def tune(graph, sources=None, waves=(8, 16, 32), unrolls=(4, 8),
          pulls=((14, 24), (32, 24), (8, 48)), deltas=(0, 32, 96), tails=(64, 128, 256),
-         repeats=2, apply=True):
+         ms_waves=(16, 24), repeats=2, apply=True):
     """Measure on `graph` and (with `apply`) keep the fastest setting of
     each knob in turn, the ones already chosen held:
 
@@ -68,7 +72,8 @@ def tune(graph, sources=None, waves=(8, 16, 32), unrolls=(4, 8),
     - ``sssp_delta`` over `deltas`: static SSSP (weighted graphs);
     - ``tail_enter`` over `tails`: static BFS + SSSP (the frontier's small
       steps);
-    - ``ms_unroll`` over `unrolls`: a 64-source SSSP (weighted graphs).
+    - ``ms_unroll`` over `unrolls`, then ``ms_waves_per_cu`` over
+      `ms_waves`: a 64-source SSSP (weighted graphs).
 
     Returns the knobs after the call with ``"timings"``: milliseconds per
     setting of each knob measured.  The knobs are restored when `apply` is
@@ -103,6 +108,7 @@ def tune(graph, sources=None, waves=(8, 16, 32), unrolls=(4, 8),
     plan.append((("tail_enter",), [(t,) for t in tails], both))
     if graph.weighted:
         plan.append((("ms_unroll",), [(u,) for u in unrolls], msssp))
+        plan.append((("ms_waves_per_cu",), [(w,) for w in ms_waves], msssp))
     timings = {}
     try:
         for names, cands, objective in plan:

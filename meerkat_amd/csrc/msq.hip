@@ -125,8 +125,11 @@ MK_DEV void ms_push(const GraphView &G, const MsView &S, MsQ &q, bool push, uint
 }
 
 // ------------------------------------------------------------------ BFS
-// BFS work on one slab of u (mask mu): claim new source bits of each
-// neighbour; a vertex entering level L+1 is pushed once (first bits).
+// BFS work on one slab of u (mask mu): each neighbour takes the source bits
+// of mu it has not seen before this level; a vertex entering level L+1 is
+// pushed once (first bits).  The seen update does not return: two waves
+// that both take bit s of x in this level both put it into the same level's
+// mask, so each (vertex, source) bit still lands in exactly one level.
 MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, MsQ &q, uint64_t key_mask_all,
                        uint32_t x, uint64_t mu, uint64_t *Fn, uint32_t nbase, uint32_t nslot,
                        uint32_t &edges, uint64_t &acc) {
@@ -137,12 +140,9 @@ MK_DEV void msbfs_slab(const GraphView &G, const MsView &S, MsQ &q, uint64_t key
   if (live) {
     const uint64_t nw = mu & ~S.seen[x];
     if (nw) {
-      const uint64_t old = atomicOr((unsigned long long *)(S.seen + x), (unsigned long long)nw);
-      const uint64_t mine = nw & ~old;
-      if (mine) {
-        acc |= mine;
-        push = atomicOr((unsigned long long *)(Fn + x), (unsigned long long)mine) == 0;
-      }
+      atomicOr((unsigned long long *)(S.seen + x), (unsigned long long)nw);
+      acc |= nw;
+      push = atomicOr((unsigned long long *)(Fn + x), (unsigned long long)nw) == 0;
     }
   }
   ms_push(G, S, q, push, x, S.logv, nbase, &S.c->cnt[nslot], S.ql[nslot], &S.c->large[nslot],
@@ -503,11 +503,15 @@ __global__ void __launch_bounds__(kBlock) k_mssp_step(GraphView G, MsView S, uin
 #pragma unroll
       for (uint32_t qq = 0; qq < 4; ++qq)
         x[qq] = qq < c ? G.heads[size_t(read_lane(b, js[qq])) * kSlabWords + lane] : kEmpty;
-#pragma unroll
-      for (uint32_t qq = 0; qq < 4; ++qq)
-        if (qq < c)
-          mssp_chain<MAP, U>(G, S, q, x[qq], read_lane(u, js[qq]), read_lane64(mu, js[qq]), Fn,
-                             qn, nslot, edges, slabs, pairs, acc);
+      // one copy of the expansion for the four heads (their loads above are
+      // already in flight): fewer registers than four inlined copies
+#pragma unroll 1
+      for (uint32_t qq = 0; qq < c; ++qq) {
+        const uint32_t xq = qq == 0 ? x[0] : qq == 1 ? x[1] : qq == 2 ? x[2] : x[3];
+        const uint32_t jq = qq == 0 ? js[0] : qq == 1 ? js[1] : qq == 2 ? js[2] : js[3];
+        mssp_chain<MAP, U>(G, S, q, xq, read_lane(u, jq), read_lane64(mu, jq), Fn, qn, nslot, edges,
+                           slabs, pairs, acc);
+      }
     }
   }
   const uint32_t per = per_wave(count, W);
@@ -637,6 +641,7 @@ struct MultiSource {
   // continuous batching (open / admit / launch / poll): the session's next
   // step, the slots admitted since open, and whether a session is open
   bool session = false;
+  bool grow_log = false;  // a session's log filled up: the next one gets a larger log
   uint32_t step = 0;
   uint64_t admitted = 0;
   uint32_t k = 0;
@@ -664,11 +669,15 @@ struct MultiSource {
     return v;
   }
 
+  // BFS level log of `entries` (vertex, mask) entries; no run or session
+  // needs more than 64 per vertex (each source bit of a vertex is set at one
+  // level) plus the admissions
   int alloc_log(uint64_t entries) {
     (void)hipFree(logv);
     (void)hipFree(logm);
     logv = nullptr;
     logm = nullptr;
+    entries = std::min<uint64_t>(entries, 64 * uint64_t(g->n) + 64);
     entries = std::min<uint64_t>(entries, 0xFFFFFFF0ull);
     MK_HIP(hipMalloc(&logv, entries * 4));
     MK_HIP(hipMalloc(&logm, entries * 8));
@@ -705,8 +714,8 @@ struct MultiSource {
     }
     if (int r = host.ensure(sizeof(MsCounts))) return r;
     MK_HIP(hipStreamSynchronize(stream));
-    max_waves = grid_waves(g->device);
     unroll = int(tune_get("ms_unroll", 8));
+    if (int r = size_grid()) return r;
     return MK_OK;
   }
 
@@ -741,6 +750,28 @@ struct MultiSource {
     return MK_OK;
   }
 
+  // The step kernel this handle launches.
+  const void *step_kernel() const {
+    if (!sssp)
+      return g->weighted ? reinterpret_cast<const void *>(&k_msbfs_level<true>)
+                         : reinterpret_cast<const void *>(&k_msbfs_level<false>);
+    return unroll == 8 ? reinterpret_cast<const void *>(&k_mssp_step<true, 8>)
+                       : reinterpret_cast<const void *>(&k_mssp_step<true, 4>);
+  }
+
+  // The steps are latency-bound: `ms_waves_per_cu` waves per CU (more than
+  // the frontier loops use keep more rows in flight), capped at what a CU
+  // holds of the step kernel, so that no part of the grid waits for a
+  // second round.
+  int size_grid() {
+    int per_cu = 0;
+    MK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel(), kBlock, 0));
+    const int resident = std::max(1, per_cu) * (kBlock / 64);
+    const int want = int(tune_get("ms_waves_per_cu", 24));
+    max_waves = device_cus(g->device) * std::min(want, resident);
+    return MK_OK;
+  }
+
   void reset_state() {
     const size_t n = g->n;
     for (uint32_t i = 0; i < kMsSlots; ++i) fill_u64(F[i], 0ull, n, stream);
@@ -750,9 +781,25 @@ struct MultiSource {
     else fill_u64(seen, 0ull, n, stream);
   }
 
+  void launch_step(const GraphView &G, const MsView &S, uint32_t s, unsigned grid) {
+    // multi-source SSSP needs a weighted graph (mk_msq_create): map slabs
+    if (sssp) {
+      if (unroll == 8) k_mssp_step<true, 8><<<grid, kBlock, 0, stream>>>(G, S, s);
+      else k_mssp_step<true, 4><<<grid, kBlock, 0, stream>>>(G, S, s);
+    } else {
+      if (g->weighted) k_msbfs_level<true><<<grid, kBlock, 0, stream>>>(G, S, s);
+      else k_msbfs_level<false><<<grid, kBlock, 0, stream>>>(G, S, s);
+    }
+  }
+
   // ---- continuous batching
   int open() {
     MK_HIP(hipSetDevice(g->device));
+    if (grow_log) {
+      MK_HIP(hipStreamSynchronize(stream));
+      if (int r = alloc_log(uint64_t(log_cap) * 4)) return r;
+      grow_log = false;
+    }
     reset_state();
     if (!sssp) fill_u32(seg, 0u, 1, stream);  // level 0 starts the log
     MK_HIP(hipGetLastError());
@@ -799,19 +846,9 @@ struct MultiSource {
     const unsigned grid = blocks_for_waves(uint64_t(max_waves));
     for (uint32_t t = 0; t < steps; ++t, ++step) {
       ProfScope prof(stream, sssp ? "k_mssp_step" : "k_msbfs_level", 0.0);
-      if (sssp) {
-        if (unroll == 8) {
-          if (g->weighted) k_mssp_step<true, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
-          else k_mssp_step<false, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
-        } else {
-          if (g->weighted) k_mssp_step<true, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
-          else k_mssp_step<false, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
-        }
-      } else {
-        if (step + 1 >= g->n + 2) return fail(MK_ERR_STATE, "multi-source BFS: session too long");
-        if (g->weighted) k_msbfs_level<true><<<grid, kBlock, 0, stream>>>(G, S, step);
-        else k_msbfs_level<false><<<grid, kBlock, 0, stream>>>(G, S, step);
-      }
+      if (!sssp && step + 1 >= g->n + 2)
+        return fail(MK_ERR_STATE, "multi-source BFS: session too long");
+      launch_step(G, S, step, grid);
     }
     MK_HIP(hipGetLastError());
     return MK_OK;
@@ -823,9 +860,12 @@ struct MultiSource {
     if (!session) return fail(MK_ERR_STATE, "mk_msq_poll: no open session (mk_msq_open)");
     MsCounts h{};
     if (int r = read_counts(h)) return r;
-    if (!sssp && h.overflow)
+    if (!sssp && h.overflow) {
+      grow_log = true;
       return fail(MK_ERR_STATE, "multi-source BFS: the level log is full; the slots still "
-                                "active are incomplete (open a new session for them)");
+                                "active are incomplete (open a new session for them: it gets a "
+                                "log four times larger)");
+    }
     last_steps = h.steps;
     last_edges = h.edges;
     *active = h.active[step % (sssp ? kMsSlots : 3)] & admitted;
@@ -853,19 +893,8 @@ struct MultiSource {
       for (;;) {
         for (uint32_t q = 0; q < group; ++q, ++step) {
           ProfScope prof(stream, sssp ? "k_mssp_step" : "k_msbfs_level", 0.0);
-          if (sssp) {
-            if (unroll == 8) {
-              if (g->weighted) k_mssp_step<true, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
-              else k_mssp_step<false, 8><<<grid, kBlock, 0, stream>>>(G, S, step);
-            } else {
-              if (g->weighted) k_mssp_step<true, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
-              else k_mssp_step<false, 4><<<grid, kBlock, 0, stream>>>(G, S, step);
-            }
-          } else {
-            if (step + 1 >= g->n + 2) break;  // seg has n + 2 entries
-            if (g->weighted) k_msbfs_level<true><<<grid, kBlock, 0, stream>>>(G, S, step);
-            else k_msbfs_level<false><<<grid, kBlock, 0, stream>>>(G, S, step);
-          }
+          if (!sssp && step + 1 >= g->n + 2) break;  // seg has n + 2 entries
+          launch_step(G, S, step, grid);
         }
         MK_HIP(hipGetLastError());
         if (int r = read_counts(h)) return r;

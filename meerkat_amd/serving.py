@@ -23,6 +23,7 @@ import time
 
 import numpy as np
 
+from ._lib import MK_ERR_STATE, MeerkatError
 from .algorithms import BFS, SSSP, MultiSource
 from .graph import device_u32
 
@@ -85,6 +86,7 @@ class ServingLoop:
         self.sssp1 = SSSP(graph, 0) if graph.weighted else None
         self.version = 0
         self.rejected_rounds = 0
+        self.session_restarts = 0  # sessions whose running slots started over (last run)
         self.phase_s = {"updates": 0.0, "queries": 0.0}  # wall time per phase, last run
 
     def _keeps(self, r):
@@ -176,6 +178,7 @@ class ServingLoop:
         engines = {"bfs": _Session(self.bfs)}
         if self.sssp is not None:
             engines["sssp"] = _Session(self.sssp)
+        self.session_restarts = 0
         pending_u = collections.deque()
         waiting = {k: collections.deque() for k in engines}
         i, n_done, total = 0, 0, len(reqs)
@@ -231,6 +234,7 @@ class ServingLoop:
                     elif e.engine.ready():
                         n_done += e.poll(now, self._keeps)
             self.phase_s["queries"] += time.perf_counter() - tq
+        self.session_restarts = sum(e.total_restarts for e in engines.values())
         return reqs
 
 
@@ -243,6 +247,8 @@ class _Session:
         self.used = 0         # slots taken since the session opened
         self.running = False
         self.inflight = False  # a launched step not yet polled
+        self.restarts = 0     # sessions restarted in a row without a slot finishing
+        self.total_restarts = 0
 
     @property
     def free(self):
@@ -263,14 +269,40 @@ class _Session:
             self.slots[s] = r
         self.used += len(take)
 
+    def restart(self):
+        """The session cannot finish its running slots (its BFS level log
+        filled up: sources admitted over many steps put each vertex in many
+        levels): they run again in a new session, from their sources,
+        admitted together, keeping their start times.  The engine gives the
+        new session a log four times larger, up to what any session needs,
+        so at most two restarts in a row find no slot finished."""
+        self.restarts += 1
+        self.total_restarts += 1
+        if self.restarts > 2:
+            raise RuntimeError("multi-source session restarted twice without progress")
+        run = [self.slots[s] for s in sorted(self.slots)]
+        self.engine.open()
+        self.slots = {s: r for s, r in enumerate(run)}
+        self.used = len(run)
+        self.engine.admit(np.array([r.src for r in run], np.uint32),
+                          np.arange(len(run), dtype=np.uint32))
+
     def poll(self, now, keeps):
         """Read the launched step's outcome (it has run); complete the slots
         that finished.  Returns how many did; the session ends when none is
         left running."""
-        active, _ = self.engine.poll()
         self.inflight = False
+        try:
+            active, _ = self.engine.poll()
+        except MeerkatError as e:
+            if e.code != MK_ERR_STATE:
+                raise
+            self.restart()
+            return 0
         t = now()
         done = [s for s in self.slots if not (active >> s) & 1]
+        if done:
+            self.restarts = 0
         for s in done:
             r = self.slots.pop(s)
             if keeps(r):

@@ -87,13 +87,14 @@ def test_autotune_picks_a_setting_and_keeps_parity(gpu):
     g.InsertEdges(s, d, w)
     try:
         r = autotune.tune(g, waves=(8, 16), unrolls=(4, 8), deltas=(0, 40), tails=(0, 128),
-                          repeats=1)
+                          ms_waves=(16, 24), repeats=1)
         assert r["waves_per_cu"] in (8, 16) and r["ms_unroll"] in (4, 8)
+        assert r["ms_waves_per_cu"] in (16, 24)
         assert r["sssp_delta"] in (0, 40) and r["tail_enter"] in (0, 128)
         assert set(r["timings"]["waves_per_cu"]) == {8, 16}
         assert set(r["timings"]["sssp_delta"]) == {0, 40}
         assert "pull_enter/pull_exit" not in r["timings"]  # no in-edge mirror
-        tuned = ("waves_per_cu", "ms_unroll", "sssp_delta", "tail_enter")
+        tuned = ("waves_per_cu", "ms_unroll", "ms_waves_per_cu", "sssp_delta", "tail_enter")
         k = autotune.knobs()
         assert all(k[x] == r[x] for x in tuned)
         assert {x: k[x] for x in k if x not in tuned} == {
@@ -112,7 +113,7 @@ def test_autotune_picks_a_setting_and_keeps_parity(gpu):
         q.close()
         # apply=False restores what was set before the call
         autotune.set_knob("waves_per_cu", 12)
-        autotune.tune(g, waves=(8,), unrolls=(4,), repeats=1, apply=False)
+        autotune.tune(g, waves=(8,), unrolls=(4,), ms_waves=(16,), repeats=1, apply=False)
         assert autotune.get_knob("waves_per_cu") == 12
     finally:
         for k_, v in before.items():
