@@ -432,8 +432,12 @@ int mk_msq_ready(mk_msq *q, int *ready);
  * Launch-shape knobs, read when a handle is created (no reference
  * counterpart; the reference hard-codes its shapes, e.g. BFSImpl.cuh:
  * 620-625).  "waves_per_cu" (1..64, default 16): waves the grid-stride
- * kernels launch per compute unit.  "ms_unroll" (4 or 8, default 8):
- * neighbour distance rows in flight per wave in multi-source SSSP.
+ * frontier, CC and triangle kernels launch per compute unit.  "ms_unroll"
+ * (4 or 8, default 8): neighbour distance rows in flight per wave in
+ * multi-source SSSP.  "ms_waves_per_cu" (1..64, default 24) and
+ * "pr_waves_per_cu" (1..64, default 32): waves per compute unit of the
+ * multi-source steps and of the PageRank iteration, each capped at what a
+ * compute unit holds of the kernel.
  * "orphan_repair" (0 or 1, default 0): a decremental BFS/SSSP call on a
  * graph with the in-edge mirror first offers each vertex whose tree edge
  * was deleted another parent at the same distance, sparing its subtree
@@ -452,6 +456,13 @@ int mk_msq_ready(mk_msq *q, int *ready);
  * waves per compute unit a mutation batch is spread over.
  * "barrier_timeout_ms" (0..600000, default 5000): the longest a frontier
  * loop's grid barrier waits before the call gives up with MK_ERR_STATE.
+ * "pull_enter" / "pull_exit" (0..4096, defaults 14 / 24): static BFS over a
+ * graph with the in-edge mirror runs a step bottom-up once the frontier
+ * holds more than 1/pull_enter of the buckets (0: never) and until it holds
+ * fewer than 1/pull_exit of the vertices.  "sssp_delta" (0..2^24, default
+ * 0): static SSSP runs near-far with this bucket width (0: label-correcting
+ * without deferral); "sssp_far_cap" (default 0: one entry per vertex)
+ * bounds each far list.
  * The Python autotuner (meerkat_amd/autotune.py) sets them from
  * measurements. */
 int mk_tune_set(const char *name, int64_t value);

@@ -4,13 +4,15 @@ Knobs are read when an algorithm handle is created, so a change applies to
 handles created afterwards:
 
 - ``waves_per_cu`` (1..64, default 16): waves the grid-stride frontier,
-  PageRank, CC and triangle kernels launch per compute unit.
+  CC and triangle kernels launch per compute unit.
 - ``ms_unroll`` (4 or 8, default 8): neighbour distance rows each wave keeps
   in flight in multi-source SSSP (more rows hide more HBM latency at the
   cost of registers).
 - ``ms_waves_per_cu`` (1..64, default 24): waves per compute unit of the
   multi-source BFS / SSSP steps (random row reads: more waves keep more in
   flight, up to what a CU holds).
+- ``pr_waves_per_cu`` (1..64, default 32): waves per compute unit of the
+  PageRank iteration, also capped at what a CU holds.
 - ``pull_enter`` / ``pull_exit`` (defaults 14 / 24): the push/pull
   thresholds of static BFS over a graph with the in-edge mirror; a step runs
   bottom-up once the frontier holds more than 1/pull_enter of the buckets
@@ -29,7 +31,7 @@ import numpy as np
 from . import _lib
 
 KNOBS = {"waves_per_cu": (1, 64), "ms_unroll": (4, 8), "ms_waves_per_cu": (1, 64),
-         "orphan_repair": (0, 1),
+         "pr_waves_per_cu": (1, 64), "orphan_repair": (0, 1),
          "persistent_steps": (0, 1), "tail_enter": (0, 1024), "coop_launch": (0, 1),
          "mut_waves_per_cu": (1, 256), "barrier_timeout_ms": (0, 600000),
          "pull_enter": (0, 4096), "pull_exit": (1, 4096), "sssp_delta": (0, 1 << 24),

@@ -52,6 +52,7 @@ static std::map<std::string, int64_t> g_tune = {
     {"waves_per_cu", 16},   // resident waves per CU the grid-stride kernels ask for
     {"ms_unroll", 8},       // neighbour rows in flight per wave in multi-source SSSP
     {"ms_waves_per_cu", 24},  // waves per CU of the multi-source BFS / SSSP steps
+    {"pr_waves_per_cu", 32},  // waves per CU of the PageRank iteration
     {"orphan_repair", 0},   // decremental with the in-edge mirror re-parents orphans first
     {"persistent_steps", 1},  // BFS/SSSP frontier steps in one launch with grid barriers
     {"tail_enter", 128},    // frontiers up to this size run in one block's LDS (0: never)
@@ -596,8 +597,8 @@ int mk_tune_set(const char *name, int64_t value) {
   const std::string k(name);
   if (k == "waves_per_cu" && (value < 1 || value > 64))
     return fail(MK_ERR_BAD_ARG, "waves_per_cu must be in [1, 64]");
-  if (k == "ms_waves_per_cu" && (value < 1 || value > 64))
-    return fail(MK_ERR_BAD_ARG, "ms_waves_per_cu must be in [1, 64]");
+  if ((k == "ms_waves_per_cu" || k == "pr_waves_per_cu") && (value < 1 || value > 64))
+    return fail(MK_ERR_BAD_ARG, k + " must be in [1, 64]");
   if (k == "ms_unroll" && value != 4 && value != 8)
     return fail(MK_ERR_BAD_ARG, "ms_unroll must be 4 or 8");
   if ((k == "orphan_repair" || k == "persistent_steps" || k == "coop_launch") && value != 0 &&

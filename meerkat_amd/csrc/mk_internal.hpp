@@ -382,6 +382,18 @@ inline unsigned blocks_for_threads(uint64_t threads) {
   return unsigned(b);
 }
 inline unsigned blocks_for_waves(uint64_t waves) { return blocks_for_threads(waves * 64); }
+// Grid-stride kernels bound by the latency of independent random loads
+// (more waves keep more in flight): `want` waves per CU, capped at what a CU
+// holds of `kernel` in kBlock-thread blocks, so that no part of the grid
+// waits for a second round.  Sets *waves to the grid's waves on `device`.
+inline hipError_t resident_waves(int device, const void *kernel, int want, int *waves) {
+  int per_cu = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0);
+  if (e != hipSuccess) return e;
+  const int fit = (per_cu > 0 ? per_cu : 1) * (kBlock / 64);
+  *waves = device_cus(device) * (want < fit ? want : fit);
+  return hipSuccess;
+}
 
 // Device-wide fill (used instead of hipMemsetAsync so callers may capture).
 void fill_u32(uint32_t *p, uint32_t v, uint64_t count, hipStream_t s);

@@ -761,14 +761,10 @@ struct MultiSource {
 
   // The steps are latency-bound: `ms_waves_per_cu` waves per CU (more than
   // the frontier loops use keep more rows in flight), capped at what a CU
-  // holds of the step kernel, so that no part of the grid waits for a
-  // second round.
+  // holds of the step kernel.
   int size_grid() {
-    int per_cu = 0;
-    MK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel(), kBlock, 0));
-    const int resident = std::max(1, per_cu) * (kBlock / 64);
-    const int want = int(tune_get("ms_waves_per_cu", 24));
-    max_waves = device_cus(g->device) * std::min(want, resident);
+    MK_HIP(resident_waves(g->device, step_kernel(), int(tune_get("ms_waves_per_cu", 24)),
+                          &max_waves));
     return MK_OK;
   }
 

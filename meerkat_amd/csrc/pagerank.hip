@@ -395,7 +395,11 @@ struct PageRank {
       if (int r = recount()) return r;
     }
     MK_HIP(hipStreamSynchronize(stream));
-    max_waves = grid_waves(g->device);
+    // the iteration gathers 4-byte contributions at random: as many waves
+    // as a CU holds (`pr_waves_per_cu`, 32) keep the most loads in flight
+    const void *kern = in->weighted ? reinterpret_cast<const void *>(&k_pr_iterate<true>)
+                                    : reinterpret_cast<const void *>(&k_pr_iterate<false>);
+    MK_HIP(resident_waves(g->device, kern, int(tune_get("pr_waves_per_cu", 32)), &max_waves));
     cur = 0;
     return MK_OK;
   }

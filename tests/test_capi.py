@@ -96,7 +96,7 @@ def test_tune_knobs_validate_without_gpu():
 
     before = autotune.knobs()
     assert before == {"waves_per_cu": 16, "ms_unroll": 8, "ms_waves_per_cu": 24,
-                      "orphan_repair": 0,
+                      "pr_waves_per_cu": 32, "orphan_repair": 0,
                       "persistent_steps": 1, "tail_enter": 128,
                       "coop_launch": 0, "mut_waves_per_cu": 32, "barrier_timeout_ms": 5000,
                       "pull_enter": 14, "pull_exit": 24, "sssp_delta": 0, "sssp_far_cap": 0}
@@ -104,7 +104,7 @@ def test_tune_knobs_validate_without_gpu():
         autotune.set_knob("waves_per_cu", 32)
         assert autotune.get_knob("waves_per_cu") == 32
         for name, bad in [("waves_per_cu", 0), ("waves_per_cu", 65), ("ms_unroll", 6),
-                          ("ms_waves_per_cu", 0), ("ms_waves_per_cu", 65),
+                          ("ms_waves_per_cu", 0), ("ms_waves_per_cu", 65), ("pr_waves_per_cu", 0),
                           ("orphan_repair", 2), ("persistent_steps", 3), ("tail_enter", 1025),
                           ("pull_enter", -1), ("pull_exit", 0), ("pull_exit", 4097),
                           ("no_such_knob", 1)]:
