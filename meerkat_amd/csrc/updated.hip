@@ -103,50 +103,104 @@ __global__ void __launch_bounds__(kBlock)
   bucket_upd[b] = 0;
 }
 
-// The update iterator: every live key of a flagged bucket whose slot is
-// marked new, as (owner, key[, value]) at out[atomicAdd(count)] while below
-// `cap` (the count includes the ones that did not fit).  One 8-lane group
-// per bucket; group lane gl reads words 8gl .. 8gl+7 of each slab (a map
-// key's value is the next word, in the same lane).
+// The live keys marked new along bucket b's chain, for this group lane (it
+// reads words 8gl .. 8gl+7 of each slab; a map key's value is the next word,
+// in the same lane): their number, and with `write` each is stored as
+// (owner, key[, value]) at out[pos++] while pos < cap.  Group-collective.
 template <bool MAP>
-__global__ void __launch_bounds__(kBlock)
-    k_emit_new(GraphView G, const uint64_t *head_new, const uint64_t *slab_new,
-               const uint8_t *bucket_upd, uint32_t *osrc, uint32_t *odst, uint32_t *ow,
-               uint64_t cap, unsigned long long *count) {
-  const uint32_t gl = group_lane();
-  const uint64_t b = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kGroupLanes;
-  if (b >= G.total_buckets || !bucket_upd[b]) return;
+MK_DEV uint32_t chain_new_keys(const GraphView &G, const uint64_t *head_new,
+                               const uint64_t *slab_new, uint64_t b, uint32_t gl, bool write,
+                               uint64_t pos, uint64_t cap, uint32_t *osrc, uint32_t *odst,
+                               uint32_t *ow) {
   const uint32_t owner = G.bucket_owner[b];
   const uint32_t *slab = G.heads + b * kSlabWords;
   uint64_t mask = head_new[b];
+  uint32_t c = 0;
   for (;;) {
     uint32_t v[kWordsPerLane];
     load_words(slab + kWordsPerLane * gl, v);
-    uint32_t c = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kWordsPerLane; ++k) {
       const uint32_t wi = kWordsPerLane * gl + k;
-      if (key_word<MAP>(wi) && v[k] < kTomb && ((mask >> wi) & 1ull)) ++c;
-    }
-    if (c) {
-      unsigned long long pos = atomicAdd(count, (unsigned long long)c);
-#pragma unroll
-      for (uint32_t k = 0; k < kWordsPerLane; ++k) {
-        const uint32_t wi = kWordsPerLane * gl + k;
-        if (key_word<MAP>(wi) && v[k] < kTomb && ((mask >> wi) & 1ull)) {
-          if (pos < cap) {
-            osrc[pos] = owner;
-            odst[pos] = v[k];
-            if (MAP && ow) ow[pos] = v[k + 1 < kWordsPerLane ? k + 1 : k];
-          }
-          ++pos;
+      if (key_word<MAP>(wi) && v[k] < kTomb && ((mask >> wi) & 1ull)) {
+        if (write && pos < cap) {
+          osrc[pos] = owner;
+          odst[pos] = v[k];
+          if (MAP && ow) ow[pos] = v[k + 1 < kWordsPerLane ? k + 1 : k];
         }
+        ++pos;
+        ++c;
       }
     }
     const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);
     if (nx == kEmpty) break;
     slab = slab_ptr(G.pool, nx);
     mask = slab_new[nx];
+  }
+  return c;
+}
+
+// Exclusive prefix sum of x over the wave (every lane active); *total gets
+// the sum.
+MK_DEV uint32_t wave_excl_scan(uint32_t x, uint32_t *total) {
+  const uint32_t lane = lane_id();
+  uint32_t inc = x;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += y;
+  }
+  *total = read_lane(inc, 63);
+  return inc - x;
+}
+
+// The update iterator: every live key of a flagged bucket whose slot is
+// marked new, as (owner, key[, value]) at out[position] while below `cap`
+// (the count includes the ones that did not fit).  Each wave takes a
+// contiguous range of buckets, eight at a time (one per 8-lane group): the
+// first pass counts the range's keys and takes the range's positions with
+// one atomic on the shared counter (one per key serialised on it: 10 ms for
+// a million keys); the second, unless `count_only`, writes them, each
+// group's chain at its offset in the wave's running total.
+template <bool MAP>
+__global__ void __launch_bounds__(kBlock)
+    k_emit_new(GraphView G, const uint64_t *head_new, const uint64_t *slab_new,
+               const uint8_t *bucket_upd, uint32_t *osrc, uint32_t *odst, uint32_t *ow,
+               uint64_t cap, unsigned long long *count, uint32_t count_only) {
+  const uint32_t gl = group_lane(), grp = lane_id() / kGroupLanes;
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t W = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+  const uint64_t nb = G.total_buckets;
+  const uint64_t per = (nb + W - 1) / W;
+  const uint64_t b0 = gw * per, b1 = min(nb, b0 + per);
+  constexpr uint32_t kGroups = 64 / kGroupLanes;
+  uint32_t mine = 0;
+  for (uint64_t bb = b0; bb < b1; bb += kGroups) {
+    const uint64_t b = bb + grp;
+    if (b < b1 && bucket_upd[b])
+      mine += chain_new_keys<MAP>(G, head_new, slab_new, b, gl, false, 0, 0, nullptr, nullptr,
+                                  nullptr);
+  }
+  uint32_t total = 0;
+  (void)wave_excl_scan(mine, &total);
+  if (total == 0) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(count, (unsigned long long)total);
+  if (count_only) return;
+  uint64_t run = (uint64_t(read_lane(uint32_t(base >> 32), 0)) << 32) |
+                 read_lane(uint32_t(base), 0);
+  for (uint64_t bb = b0; bb < b1; bb += kGroups) {
+    const uint64_t b = bb + grp;
+    const bool act = b < b1 && bucket_upd[b];
+    const uint32_t c =
+        act ? chain_new_keys<MAP>(G, head_new, slab_new, b, gl, false, 0, 0, nullptr, nullptr,
+                                  nullptr)
+            : 0u;
+    uint32_t step = 0;
+    const uint32_t ex = wave_excl_scan(c, &step);
+    if (act)  // the whole group walks (the next-slab word is read across it)
+      chain_new_keys<MAP>(G, head_new, slab_new, b, gl, true, run + ex, cap, osrc, odst, ow);
+    run += step;
   }
 }
 
@@ -215,11 +269,13 @@ int mk_graph_updated_edges(mk_graph *g, uint32_t *src_dev, uint32_t *dst_dev, ui
   if (int r = c.ensure(8)) return r;
   unsigned long long *cnt = c.as<unsigned long long>();
   fill_u64(reinterpret_cast<uint64_t *>(cnt), 0ull, 1, g->own_stream);
-  const unsigned grid = blocks_for_threads(uint64_t(g->total_buckets) * kGroupLanes);
+  const unsigned grid = blocks_for_waves(std::min<uint64_t>(
+      uint64_t(grid_waves(g->device)), (uint64_t(g->total_buckets) + 63) / 64));
   auto k = g->weighted ? k_emit_new<true> : k_emit_new<false>;
   k<<<grid, kBlock, 0, g->own_stream>>>(g->view(), g->head_new, g->pool->slab_new,
                                          g->bucket_upd, src_dev, dst_dev,
-                                         g->weighted ? w_dev : nullptr, capacity, cnt);
+                                         g->weighted ? w_dev : nullptr, capacity, cnt,
+                                         capacity == 0 ? 1u : 0u);
   hipError_t e = hipGetLastError();
   uint64_t h = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, g->own_stream);

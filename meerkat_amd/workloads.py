@@ -8,6 +8,10 @@ kernel traces and counter passes (rocprofv3) beside bench.py's update step.
     python -m meerkat_amd.workloads static     # RMAT-24 weighted with the in-edge mirror:
                                                # static BFS and SSSP from 8 sources
     python -m meerkat_amd.workloads multi      # RMAT-24: 64-source BFS and SSSP, twice each
+    python -m meerkat_amd.workloads weights    # RMAT-24 weighted, mirror, update tracking:
+                                               # 3 x (1,000,000 weight changes + SSSP
+                                               # reweight), then 3 x (1,000,000 inserts,
+                                               # the graph's updated edges, incremental SSSP)
 
 Launch knobs (mk_tune_set) may follow as name=value, e.g. sssp_delta=64.
 
@@ -208,7 +212,72 @@ def multi():
             "region_s": round(el, 4), "kernels": kern}
 
 
-WORKLOADS = {"pagerank": pagerank, "tc_cc": tc_cc, "static": static, "multi": multi}
+def weights():
+    from . import gen
+    from .algorithms import SSSP
+    from .graph import DynamicGraph, SlabPool, device_u32
+
+    n, s, d, w = gen.rmat(24, 16, seed=9, weighted=True)
+    pool = SlabPool(16 << 30)
+    g = DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=True,
+                     in_degree_hints=gen.degree_hints(n, d), track_updates=True)
+    g.InsertEdges(s, d, w)
+    rng = np.random.default_rng(25)
+    deg = np.bincount(s, minlength=n)
+    source = int(rng.choice(np.nonzero(deg)[0]))
+    sp = SSSP(g, source)
+    sp.Static()
+    # weight batches: distinct present edges, new weights in [1, 100]
+    wb = []
+    for i in range(3):
+        idx = rng.choice(len(s), size=1_000_000, replace=False)
+        wb.append((device_u32(s[idx]), device_u32(d[idx]),
+                   device_u32(rng.integers(1, 101, len(idx)).astype(np.uint32))))
+    keys = gen.keys_of(s, d)
+    ib = []
+    for i in range(3):
+        bs, bd, bw = gen.insert_batch(keys, n, 1_000_000, seed=26 * 1000 + i, weighted=True,
+                                      wseed=27 + i)
+        ib.append((device_u32(bs), device_u32(bd), device_u32(bw)))
+        keys = np.union1d(keys, gen.keys_of(bs, bd))
+    g.UpdateSlabPointers()
+    _sync()
+
+    def wall(fn):
+        _sync()
+        t0 = time.perf_counter()
+        r = fn()
+        _sync()
+        return r, (time.perf_counter() - t0) * 1e3
+
+    def work():
+        rec = {"update_weights_ms": [], "reweight_ms": [], "insert_ms": [],
+               "updated_edges_ms": [], "updated_edges": [], "incremental_ms": [],
+               "update_slab_pointers_ms": []}
+        for bs, bd, bw in wb:
+            rec["update_weights_ms"].append(wall(lambda: g.UpdateEdges(bs, bd, bw))[1])
+            rec["reweight_ms"].append(sp.Reweight(bs, bd, bw))
+        for bs, bd, bw in ib:
+            rec["insert_ms"].append(wall(lambda: g.InsertEdges(bs, bd, bw))[1])
+            (us, ud, uw), t = wall(g.UpdatedEdges)
+            rec["updated_edges_ms"].append(t)
+            rec["updated_edges"].append(int(us.numel()))
+            rec["incremental_ms"].append(sp.Incremental(us, ud, uw))
+            rec["update_slab_pointers_ms"].append(wall(g.UpdateSlabPointers)[1])
+            del us, ud, uw
+        return rec
+
+    rec, el, kern = _region(work)
+    sp.close()
+    g.close()
+    pool.close()
+    return {"workload": "rmat24_weight_updates_and_updated_edges", "vertices": n,
+            "edges": len(s), "source": source, **rec, "region_s": round(el, 4),
+            "kernels": kern}
+
+
+WORKLOADS = {"pagerank": pagerank, "tc_cc": tc_cc, "static": static, "multi": multi,
+             "weights": weights}
 
 
 def main(argv):
