@@ -154,53 +154,77 @@ MK_DEV uint32_t wave_excl_scan(uint32_t x, uint32_t *total) {
   return inc - x;
 }
 
+// The group's bucket in a round over the flagged buckets of a 64-bucket
+// chunk (m: their bits, wave-uniform): group g takes the g-th lowest set bit;
+// the round's eight are then taken off m.  False when the group has none.
+MK_DEV bool round_bucket(uint64_t &m, uint32_t grp, uint32_t &j) {
+  uint64_t mm = m;
+  for (uint32_t t = 0; t < grp && mm; ++t) mm &= mm - 1;
+  const bool has = mm != 0;
+  j = has ? ffs64(mm) : 0u;
+  for (uint32_t t = 0; t < 64 / kGroupLanes && m; ++t) m &= m - 1;
+  return has;
+}
+
 // The update iterator: every live key of a flagged bucket whose slot is
 // marked new, as (owner, key[, value]) at out[position] while below `cap`
 // (the count includes the ones that did not fit).  Each wave takes a
-// contiguous range of buckets, eight at a time (one per 8-lane group): the
-// first pass counts the range's keys and takes the range's positions with
-// one atomic on the shared counter (one per key serialised on it: 10 ms for
-// a million keys); the second, unless `count_only`, writes them, each
-// group's chain at its offset in the wave's running total.
+// contiguous range of buckets in chunks of 64 (one flag per lane, the next
+// chunk's flags loaded ahead), its 8-lane groups walking the flagged buckets'
+// chains eight at a time.  The first pass counts the range's keys and takes
+// the range's positions with one atomic on the shared counter (one per key
+// serialised on it: 10 ms for a million keys); the second, unless
+// `count_only`, writes them, each group's chain at its offset in the wave's
+// running total.
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_emit_new(GraphView G, const uint64_t *head_new, const uint64_t *slab_new,
                const uint8_t *bucket_upd, uint32_t *osrc, uint32_t *odst, uint32_t *ow,
                uint64_t cap, unsigned long long *count, uint32_t count_only) {
-  const uint32_t gl = group_lane(), grp = lane_id() / kGroupLanes;
+  const uint32_t lane = lane_id(), gl = group_lane(), grp = lane / kGroupLanes;
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t W = (uint64_t(gridDim.x) * blockDim.x) >> 6;
   const uint64_t nb = G.total_buckets;
-  const uint64_t per = (nb + W - 1) / W;
+  const uint64_t per = ((nb + W - 1) / W + 63) / 64 * 64;  // whole chunks per wave
   const uint64_t b0 = gw * per, b1 = min(nb, b0 + per);
-  constexpr uint32_t kGroups = 64 / kGroupLanes;
+  auto flag = [&](uint64_t cb) { return cb + lane < b1 && bucket_upd[cb + lane] != 0; };
   uint32_t mine = 0;
-  for (uint64_t bb = b0; bb < b1; bb += kGroups) {
-    const uint64_t b = bb + grp;
-    if (b < b1 && bucket_upd[b])
-      mine += chain_new_keys<MAP>(G, head_new, slab_new, b, gl, false, 0, 0, nullptr, nullptr,
-                                  nullptr);
+  bool fnext = b0 < b1 && flag(b0);
+  for (uint64_t cb = b0; cb < b1; cb += 64) {
+    uint64_t m = ballot(fnext);
+    fnext = cb + 64 < b1 && flag(cb + 64);
+    while (m) {
+      uint32_t j;
+      if (round_bucket(m, grp, j))
+        mine += chain_new_keys<MAP>(G, head_new, slab_new, cb + j, gl, false, 0, 0, nullptr,
+                                    nullptr, nullptr);
+    }
   }
   uint32_t total = 0;
   (void)wave_excl_scan(mine, &total);
   if (total == 0) return;
   unsigned long long base = 0;
-  if (lane_id() == 0) base = atomicAdd(count, (unsigned long long)total);
+  if (lane == 0) base = atomicAdd(count, (unsigned long long)total);
   if (count_only) return;
   uint64_t run = (uint64_t(read_lane(uint32_t(base >> 32), 0)) << 32) |
                  read_lane(uint32_t(base), 0);
-  for (uint64_t bb = b0; bb < b1; bb += kGroups) {
-    const uint64_t b = bb + grp;
-    const bool act = b < b1 && bucket_upd[b];
-    const uint32_t c =
-        act ? chain_new_keys<MAP>(G, head_new, slab_new, b, gl, false, 0, 0, nullptr, nullptr,
-                                  nullptr)
-            : 0u;
-    uint32_t step = 0;
-    const uint32_t ex = wave_excl_scan(c, &step);
-    if (act)  // the whole group walks (the next-slab word is read across it)
-      chain_new_keys<MAP>(G, head_new, slab_new, b, gl, true, run + ex, cap, osrc, odst, ow);
-    run += step;
+  fnext = b0 < b1 && flag(b0);
+  for (uint64_t cb = b0; cb < b1; cb += 64) {
+    uint64_t m = ballot(fnext);
+    fnext = cb + 64 < b1 && flag(cb + 64);
+    while (m) {
+      uint32_t j;
+      const bool act = round_bucket(m, grp, j);
+      const uint32_t c = act ? chain_new_keys<MAP>(G, head_new, slab_new, cb + j, gl, false, 0,
+                                                   0, nullptr, nullptr, nullptr)
+                             : 0u;
+      uint32_t step = 0;
+      const uint32_t ex = wave_excl_scan(c, &step);
+      if (act)  // the whole group walks (the next-slab word is read across it)
+        chain_new_keys<MAP>(G, head_new, slab_new, cb + j, gl, true, run + ex, cap, osrc, odst,
+                            ow);
+      run += step;
+    }
   }
 }
 
@@ -269,8 +293,9 @@ int mk_graph_updated_edges(mk_graph *g, uint32_t *src_dev, uint32_t *dst_dev, ui
   if (int r = c.ensure(8)) return r;
   unsigned long long *cnt = c.as<unsigned long long>();
   fill_u64(reinterpret_cast<uint64_t *>(cnt), 0ull, 1, g->own_stream);
+  // 32 waves per CU: the chain walks are latency-bound
   const unsigned grid = blocks_for_waves(std::min<uint64_t>(
-      uint64_t(grid_waves(g->device)), (uint64_t(g->total_buckets) + 63) / 64));
+      uint64_t(device_cus(g->device)) * 32, (uint64_t(g->total_buckets) + 63) / 64));
   auto k = g->weighted ? k_emit_new<true> : k_emit_new<false>;
   k<<<grid, kBlock, 0, g->own_stream>>>(g->view(), g->head_new, g->pool->slab_new,
                                          g->bucket_upd, src_dev, dst_dev,
