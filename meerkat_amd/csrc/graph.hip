@@ -466,33 +466,59 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Per-bucket walk: live keys and tombstones added to the owner's counters;
-// overflow slab count and chain length maximum accumulated.
+// Per-bucket walk, one 8-lane group per bucket (grid-stride, eight chains in
+// flight per wave): live keys and tombstones added to the owner's counters;
+// the overflow slab count and the longest chain gathered per wave and added
+// with one atomic each (an atomic per bucket on one counter serialises: 0.3 s
+// over RMAT-24's 21M buckets).
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_count_slots(GraphView G, uint32_t *live, uint32_t *tomb, unsigned long long *overflow,
                   uint32_t *max_chain) {
-  const uint32_t lane = lane_id();
-  const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (b >= G.total_buckets) return;
-  const uint64_t km = key_mask<MAP>();
-  const uint32_t *slab = G.heads + size_t(b) * kSlabWords;
-  uint32_t nl = 0, nt = 0, chain = 1;
-  for (;;) {
-    const uint32_t v = slab[lane];
-    nl += uint32_t(__popcll(ballot(v < kTomb) & km));
-    nt += uint32_t(__popcll(ballot(v == kTomb) & km));
-    const uint32_t nx = read_lane(v, kNextLane);
-    if (nx == kEmpty) break;
-    slab = slab_ptr(G.pool, nx);
-    ++chain;
+  constexpr uint32_t kGroups = 64 / kGroupLanes;
+  const uint32_t gl = group_lane();
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t W = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+  uint64_t ovf = 0;
+  uint32_t mx = 0;
+  for (uint64_t b0 = gw * kGroups; b0 < G.total_buckets; b0 += W * kGroups) {
+    const uint64_t b = b0 + lane_id() / kGroupLanes;
+    if (b >= G.total_buckets) continue;  // the whole group
+    const uint32_t *slab = G.heads + b * kSlabWords;
+    uint32_t nl = 0, nt = 0, chain = 1;
+    for (;;) {
+      uint32_t v[kWordsPerLane];
+      load_words(slab + kWordsPerLane * gl, v);
+#pragma unroll
+      for (uint32_t k = 0; k < kWordsPerLane; ++k)
+        if (key_word<MAP>(kWordsPerLane * gl + k)) {
+          nl += v[k] < kTomb ? 1u : 0u;
+          nt += v[k] == kTomb ? 1u : 0u;
+        }
+      const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);
+      if (nx == kEmpty) break;
+      slab = slab_ptr(G.pool, nx);
+      ++chain;
+    }
+#pragma unroll
+    for (uint32_t o = kGroupLanes / 2; o >= 1; o >>= 1) {  // sums over the group
+      nl += uint32_t(__shfl_xor(int(nl), int(o), 64));
+      nt += uint32_t(__shfl_xor(int(nt), int(o), 64));
+    }
+    if (gl == 0) {
+      const uint32_t u = G.bucket_owner[b];
+      if (live && nl) atomicAdd(live + u, nl);
+      if (tomb && nt) atomicAdd(tomb + u, nt);
+      ovf += chain - 1;
+      mx = max(mx, chain);
+    }
   }
-  if (lane == 0) {
-    const uint32_t u = G.bucket_owner[b];
-    if (live && nl) atomicAdd(live + u, nl);
-    if (tomb && nt) atomicAdd(tomb + u, nt);
-    if (overflow && chain > 1) atomicAdd(overflow, (unsigned long long)(chain - 1));
-    if (max_chain) atomicMax(max_chain, chain);
+  ovf = wave_sum_u64(ovf);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mx = max(mx, uint32_t(__shfl_xor(int(mx), o, 64)));
+  if (lane_id() == 0) {
+    if (overflow && ovf) atomicAdd(overflow, (unsigned long long)ovf);
+    if (max_chain && mx) atomicMax(max_chain, mx);
   }
 }
 
@@ -1241,14 +1267,14 @@ int mk_graph_bucket_offsets(mk_graph *g, uint32_t *off) {
 
 static int count_slots(mk_graph *g, uint32_t *live_dev, uint32_t *tomb_dev,
                        unsigned long long *overflow_dev, uint32_t *max_dev) {
+  // 32 waves per CU of eight chain walks each
+  const unsigned grid = blocks_for_waves(std::max<uint64_t>(
+      1, std::min<uint64_t>(uint64_t(device_cus(g->device)) * 32,
+                            (uint64_t(g->total_buckets) + 7) / 8)));
   if (g->weighted)
-    k_count_slots<true><<<blocks_for_waves(g->total_buckets), kBlock>>>(g->view(), live_dev,
-                                                                       tomb_dev, overflow_dev,
-                                                                       max_dev);
+    k_count_slots<true><<<grid, kBlock>>>(g->view(), live_dev, tomb_dev, overflow_dev, max_dev);
   else
-    k_count_slots<false><<<blocks_for_waves(g->total_buckets), kBlock>>>(g->view(), live_dev,
-                                                                        tomb_dev, overflow_dev,
-                                                                        max_dev);
+    k_count_slots<false><<<grid, kBlock>>>(g->view(), live_dev, tomb_dev, overflow_dev, max_dev);
   MK_HIP(hipGetLastError());
   return MK_OK;
 }
