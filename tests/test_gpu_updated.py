@@ -9,9 +9,11 @@ first), minus those deleted since, with their weights in the graph.  Cases:
 duplicates and self-loops in a batch, edges already present, deletion of
 new and of old edges, re-insertion of a deleted edge into a reused
 (tombstoned) slot, hub chains growing overflow slabs after the marks were
-cleared, weight updates (not insertions).  Then the consumer the iteration
-is for: incremental BFS / SSSP driven by the graph's own updated edges after
-several insert batches, bit-exact against the oracle."""
+cleared, weight updates (not insertions), and a graph of over a million
+buckets (the iterator's waves then scan many 64-bucket chunks each).  Then
+the consumer the iteration is for: incremental BFS / SSSP driven by the
+graph's own updated edges after several insert batches, bit-exact against
+the oracle."""
 import numpy as np
 import pytest
 
@@ -164,5 +166,31 @@ def test_incremental_from_the_graphs_updated_edges(gpu, weighted):
         assert np.array_equal(dist, exp), int((dist != exp).sum())
         assert oracle.check_parents_fast(ref, dist, par, 0, weights=weighted) == 0
     algo.close()
+    g.close()
+    pool.close()
+
+
+def test_updated_edges_over_many_bucket_chunks(gpu):
+    # RMAT-20: over a million buckets, so each wave of the iterator scans
+    # several 64-bucket chunks; the iterator's count pass (no output) and its
+    # emit pass must agree, and report exactly the applied inserts with
+    # their weights, once each
+    n, s, d, w = gen.rmat(20, 8, seed=53, weighted=True)
+    pool = mk.SlabPool(1 << 30)
+    g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True,
+                        track_updates=True)
+    g.InsertEdges(s, d, w)
+    assert g.slab_stats()["head_slabs"] > 64 * 8192
+    g.UpdateSlabPointers()
+    keys = gen.keys_of(s, d)
+    bs, bd, bw = gen.insert_batch(keys, n, 200_000, seed=54, weighted=True)
+    g.InsertEdges(bs, bd, bw)
+    cnt = _lib.ctypes.c_uint64()
+    _lib.call("mk_graph_updated_edges", g.handle, None, None, None, 0, _lib.ctypes.byref(cnt))
+    assert cnt.value == len(bs)
+    got, wts = updated_set(g)
+    exp = gen.keys_of(bs, bd).tolist()
+    assert got == set(exp)
+    assert all(wts[k] == int(x) for k, x in zip(exp, bw))
     g.close()
     pool.close()
