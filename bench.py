@@ -435,9 +435,15 @@ def main():
         g = mk.DynamicGraph(n, pool, 0.7, gen.degree_hints(n, s), weighted=True, reverse=True,
                             in_degree_hints=gen.degree_hints(n, d))
         g.InsertEdges(s, d, w)
-        serving_out = run_serving(args, n, g, s, d, log)
-        g.close()
-        pool.close()
+        try:
+            serving_out = run_serving(args, n, g, s, d, log)
+        except RuntimeError as e:  # MeerkatError included
+            # the headline above stands; the line reports what stopped the phase
+            serving_out = {"error": repr(e)}
+            log(f"serving phase failed: {e!r}")
+        finally:
+            g.close()
+            pool.close()
 
     K = args.steps
     ms_per_step = elapsed / K * 1e3
