@@ -17,7 +17,8 @@ wait for its own outcome.
 
 value = edge updates per second over the timed steps, shortest paths
 maintained (B * K / seconds).  Also reported: the same steps with every call
-waiting for its own outcome (the reference's protocol) and replayed from a
+waiting for its own outcome, with the mutations launched without waiting and
+each SSSP call blocking (the reference's protocol), and replayed from a
 hipGraph-captured step; update-only throughput; incremental/decremental SSSP
 and incremental BFS traversed-edge rates; per-step latency percentiles; the
 roofline of the dominant kernel (timing inside the library over the timed
@@ -256,7 +257,7 @@ def main():
     # edges (each still present when deleted), insertions new edges
     # after the timed steps: breakdown steps with synchronous calls, first
     # for the per-call rates, then for the per-kernel table
-    steps_total = args.warmup + args.steps + 2 * args.breakdown_steps + args.graph_steps
+    steps_total = args.warmup + args.steps + 3 * args.breakdown_steps + args.graph_steps
     timed_end = args.warmup + args.steps
     half = args.batch // 2
     rng = np.random.default_rng(args.seed + 1)
@@ -348,9 +349,9 @@ def main():
     profiling.marker(dev)
     prof = profiling.totals()
     deferred = sp.deferred_stats()
-    # the reference's protocol: a few more steps in which every mutation and
-    # SSSP call waits for its own outcome (each call returns its time and
-    # edges read; only the loops timed)
+    # a few more steps in which every mutation and SSSP call waits for its
+    # own outcome (each call returns its time and edges read; only the loops
+    # timed)
     sp.set_deferred(False)
     g.set_deferred(False)
     calls = {k: [] for k in ("decr_ms", "incr_ms", "decr_edges", "incr_edges")}
@@ -361,11 +362,23 @@ def main():
         step(i, calls=calls)
     settle()
     sync_elapsed = time.perf_counter() - tsync
+    # the reference's own protocol: InsertEdges / DeleteEdges launch and
+    # return (no outcome; checks deferred to the call that follows), and each
+    # SSSP call blocks until its result
+    g.set_deferred(True)
+    ref_end = rate_end + args.breakdown_steps
+    torch.cuda.synchronize()
+    tref = time.perf_counter()
+    for i in range(rate_end, ref_end):
+        step(i)
+    settle()
+    ref_elapsed = time.perf_counter() - tref
+    g.set_deferred(False)
     # per-kernel table: a few more with every kernel timed
     profiling.reset()
     profiling.enable(True)
-    prof_end = rate_end + args.breakdown_steps
-    for i in range(rate_end, prof_end):
+    prof_end = ref_end + args.breakdown_steps
+    for i in range(ref_end, prof_end):
         step(i)
     settle()
     profiling.enable(False)
@@ -520,13 +533,17 @@ def main():
         # deferred checks: the mutation kernels' own time; synchronous: the calls'
         "update_only_edges_per_s": round(args.batch * K / (upd.sum() if args.sync_mutations else
                                          mut_s), 1),
-        # the reference's protocol (every mutation and SSSP call waits for its
-        # own outcome), wall clock over the synchronous breakdown steps
+        # every mutation and SSSP call waiting for its own outcome: wall clock
+        # over the synchronous breakdown steps
         "sync_protocol_ms_per_step": round(sync_elapsed / max(args.breakdown_steps, 1) * 1e3, 3),
         # the step replayed from a hipGraph-captured step, queued back to back
         "captured_step": captured,
         "sync_protocol_edge_updates_per_s": round(args.batch * args.breakdown_steps /
                                                   sync_elapsed, 1) if args.breakdown_steps else None,
+        # mutations launched without waiting (the reference's InsertEdges /
+        # DeleteEdges), each SSSP call blocking until its result
+        "reference_protocol_ms_per_step": round(ref_elapsed / max(args.breakdown_steps, 1) * 1e3,
+                                                3),
         # deferred SSSP calls of the warm-up and timed steps; a recomputed one
         # ran out of reserved loop steps and was recomputed from scratch
         "deferred_calls": deferred,
