@@ -289,8 +289,8 @@ int mk_sssp_sync(mk_sssp *s);
 int mk_sssp_deferred_stats(mk_sssp *s, uint64_t *settled, uint64_t *recomputed);
 
 /* Captured update steps (no reference counterpart: the reference launches
- * every kernel from the host; north_star's hipGraph-captured
- * update -> incremental-recompute step).  mk_*_capture_step records one step
+ * every kernel from the host; here the update -> incremental-recompute step
+ * can be replayed as one hipGraph).  mk_*_capture_step records one step
  * of the handle on its graph, DeleteEdges(del) -> Decremental(del) ->
  * InsertEdges(ins) -> Incremental(ins), into a hipGraph, reading the batch
  * buffers given here (device memory the caller refills between replays; the
