@@ -41,11 +41,13 @@ def replay_versions(n, s, d, w, reqs):
     return versions
 
 
-def test_queries_in_the_mix_equal_queries_alone(gpu):
+@pytest.mark.parametrize("lockstep", [False, True])
+def test_queries_in_the_mix_equal_queries_alone(gpu, lockstep):
+    # lockstep: both engines take a step, then the loop waits for both
     n, s, d, w, pool, g = build(12, 41)
     keys = gen.keys_of(s, d)
     reqs = serving.make_stream(n, keys, 150, rate=20000.0, seed=3, median=300, max_size=3000)
-    loop = serving.ServingLoop(g, keep_results=True)
+    loop = serving.ServingLoop(g, keep_results=True, lockstep=lockstep)
     done = loop.run(reqs, mode="batched")
     assert all(r.done is not None for r in done)
     versions = replay_versions(n, s, d, w, done)
