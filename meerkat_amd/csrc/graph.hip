@@ -296,11 +296,19 @@ __global__ void __launch_bounds__(kBlock)
   uint32_t pending = group_bits(ballot(act));
   uint32_t pool_fail = 0;
   const uint32_t seed_base = (second ? 0x9E3779B9u : 0u) + wave * 64u + group_first_lane();
+  uint32_t cur[kWordsPerLane];
+  if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, cur);
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
+    // the group's next edge's head slab is in flight while this edge's
+    // claim is
+    uint32_t nxt[kWordsPerLane] = {};
+    if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, nxt);
     const uint32_t sj = group_read(s, j), dj = group_read(d, j), wj = group_read(w, j);
-    const int r = slab_insert_group<MAP>(G, P, group_read(bk, j), dj, wj, seed_base + j);
+    const int r = slab_insert_group<MAP>(G, P, group_read(bk, j), dj, wj, seed_base + j, cur);
+#pragma unroll
+    for (uint32_t k = 0; k < kWordsPerLane; ++k) cur[k] = nxt[k];
     if (gl == j) {
       if (r == 1) st = MK_EDGE_APPLIED;
       else if (r < 0) st = MK_EDGE_POOL_FULL;  // rolled back by the host
@@ -363,11 +371,18 @@ __global__ void __launch_bounds__(kBlock)
   const GraphView &G = M.G;
   const uint32_t bk = act ? bucket_of(G, s, d) : 0u;
   uint32_t pending = group_bits(ballot(act));
+  uint32_t cur[kWordsPerLane];
+  if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, cur);
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
+    // the next edge's head slab is in flight while this edge's swap is
+    uint32_t nxt[kWordsPerLane] = {};
+    if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, nxt);
     const uint32_t sj = group_read(s, j), dj = group_read(d, j);
-    const int r = slab_delete_group<MAP>(G, group_read(bk, j), dj);
+    const int r = slab_delete_group<MAP>(G, group_read(bk, j), dj, cur);
+#pragma unroll
+    for (uint32_t k = 0; k < kWordsPerLane; ++k) cur[k] = nxt[k];
     if (gl == j && r == 1) st = MK_EDGE_APPLIED;
     if (r == 1 && gl == 0 && M.degrees == 1) atomicSub(G.vdeg + sj, 1u);
   }

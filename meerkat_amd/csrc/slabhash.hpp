@@ -303,18 +303,20 @@ MK_DEV uint32_t pool_alloc_group(const PoolView &P, uint32_t seed) {
 
 // slab_insert for one group (arguments group-uniform; `bucket` is
 // bucket_of(G, src, dst), computed by the caller for all its edges at once
-// so that this load leaves each operation's dependent chain).  Returns 1 if
+// so that this load leaves each operation's dependent chain).  `v` holds the
+// bucket's head slab as load_slab_coherent reads it, loaded by the caller
+// (so that it can be in flight during the previous operation; a view older
+// than the slab's current contents is safe: a claim made from it fails and
+// the walk is redone from fresh loads); it is overwritten.  Returns 1 if
 // inserted, 0 if already present, -1 if the pool is exhausted.
 template <bool MAP>
 MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t bucket, uint32_t dst,
-                             uint32_t w, uint32_t seed) {
+                             uint32_t w, uint32_t seed, uint32_t (&v)[kWordsPerLane]) {
   const uint32_t gl = group_lane();
   uint32_t *slab = G.heads + size_t(bucket) * kSlabWords;
   uint32_t *free_slab = nullptr;
   uint32_t free_word = 0, free_old = 0;
   for (;;) {
-    uint32_t v[kWordsPerLane];
-    load_slab_coherent(slab, v);
     bool hit = false;
     uint32_t fj = kWordsPerLane;  // this lane's first free key word
 #pragma unroll
@@ -337,6 +339,7 @@ MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t buc
     const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);  // word 63
     if (nx != kEmpty) {
       slab = slab_ptr(G.pool, nx);
+      load_slab_coherent(slab, v);
       continue;
     }
     if (free_slab != nullptr) {
@@ -350,6 +353,7 @@ MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t buc
       if (old == dst) return 0;
       slab = free_slab;  // lost the slot: re-walk from it
       free_slab = nullptr;
+      load_slab_coherent(slab, v);
       continue;
     }
     // chain full: link a fresh slab holding the key at its first slot
@@ -370,18 +374,20 @@ MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t buc
     if (old == kEmpty) return 1;
     if (gl == 0) pool_free_lane(P, ns);
     slab = slab_ptr(G.pool, old);
+    load_slab_coherent(slab, v);
   }
 }
 
-// slab_delete for one group: the key becomes a tombstone.  Returns 1 if
+// slab_delete for one group: the key becomes a tombstone.  `v` holds the
+// head slab, loaded by the caller as for slab_insert_group (an older view
+// is safe: the swap checks the key); it is overwritten.  Returns 1 if
 // deleted.
 template <bool MAP>
-MK_DEV int slab_delete_group(const GraphView &G, uint32_t bucket, uint32_t dst) {
+MK_DEV int slab_delete_group(const GraphView &G, uint32_t bucket, uint32_t dst,
+                             uint32_t (&v)[kWordsPerLane]) {
   const uint32_t gl = group_lane();
   uint32_t *slab = G.heads + size_t(bucket) * kSlabWords;
   for (;;) {
-    uint32_t v[kWordsPerLane];
-    load_slab_coherent(slab, v);
     uint32_t hj = kWordsPerLane;
 #pragma unroll
     for (int j = kWordsPerLane - 1; j >= 0; --j)
@@ -398,6 +404,7 @@ MK_DEV int slab_delete_group(const GraphView &G, uint32_t bucket, uint32_t dst) 
     const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);
     if (nx == kEmpty) return 0;
     slab = slab_ptr(G.pool, nx);
+    load_slab_coherent(slab, v);
   }
 }
 
