@@ -296,15 +296,17 @@ __global__ void __launch_bounds__(kBlock)
   uint32_t pending = group_bits(ballot(act));
   uint32_t pool_fail = 0;
   const uint32_t seed_base = (second ? 0x9E3779B9u : 0u) + wave * 64u + group_first_lane();
+  // head slabs with plain loads: an older view of a slab is safe (a claim
+  // made from it fails, and the walk is redone from agent-scope loads)
   uint32_t cur[kWordsPerLane];
-  if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, cur);
+  if (pending) load_slab_plain(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, cur);
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     // the group's next edge's head slab is in flight while this edge's
     // claim is
     uint32_t nxt[kWordsPerLane] = {};
-    if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, nxt);
+    if (pending) load_slab_plain(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, nxt);
     const uint32_t sj = group_read(s, j), dj = group_read(d, j), wj = group_read(w, j);
     const int r = slab_insert_group<MAP>(G, P, group_read(bk, j), dj, wj, seed_base + j, cur);
 #pragma unroll
@@ -372,13 +374,13 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t bk = act ? bucket_of(G, s, d) : 0u;
   uint32_t pending = group_bits(ballot(act));
   uint32_t cur[kWordsPerLane];
-  if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, cur);
+  if (pending) load_slab_plain(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, cur);
   while (pending) {
     const uint32_t j = ffs32(pending);
     pending &= pending - 1;
     // the next edge's head slab is in flight while this edge's swap is
     uint32_t nxt[kWordsPerLane] = {};
-    if (pending) load_slab_coherent(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, nxt);
+    if (pending) load_slab_plain(G.heads + size_t(group_read(bk, ffs32(pending))) * kSlabWords, nxt);
     const uint32_t sj = group_read(s, j), dj = group_read(d, j);
     const int r = slab_delete_group<MAP>(G, group_read(bk, j), dj, cur);
 #pragma unroll

@@ -113,6 +113,17 @@ MK_DEV void load_slab_coherent(const uint32_t *row, uint32_t (&v)[kWordsPerLane]
     v[2 * k + 1] = uint32_t(a >> 32);
   }
 }
+// The same words with plain loads (they may come from this CU's L1, which
+// is clean at kernel start, so a view from earlier in the kernel).
+MK_DEV void load_slab_plain(const uint32_t *row, uint32_t (&v)[kWordsPerLane]) {
+  const uint32_t gl = group_lane();
+#pragma unroll
+  for (uint32_t k = 0; k < kWordsPerLane / 2; ++k) {
+    const uint2 a = *reinterpret_cast<const uint2 *>(row + coh_word(gl, 2 * k));
+    v[2 * k] = a.x;
+    v[2 * k + 1] = a.y;
+  }
+}
 MK_DEV void load_words(const uint32_t *p, uint32_t (&v)[kWordsPerLane]) {
   const uint4 *q = reinterpret_cast<const uint4 *>(p);
 #pragma unroll
@@ -191,11 +202,14 @@ MK_DEV uint32_t pool_alloc_group(const PoolView &P, uint32_t seed) {
 // Insert (src -> dst [, w]) for one group (arguments group-uniform; `bucket` is
 // bucket_of(G, src, dst), computed by the caller for all its edges at once
 // so that this load leaves each operation's dependent chain).  `v` holds the
-// bucket's head slab as load_slab_coherent reads it, loaded by the caller
-// (so that it can be in flight during the previous operation; a view older
-// than the slab's current contents is safe: a claim made from it fails and
-// the walk is redone from fresh loads); it is overwritten.  Returns 1 if
-// inserted, 0 if already present, -1 if the pool is exhausted.
+// bucket's head slab in load_slab_coherent's layout, loaded by the caller so
+// that it can be in flight during the previous operation.  Plain loads will
+// do: during an insert kernel words only go from free to key and next
+// pointers from empty to a slab, so an older view can only show a taken
+// slot as free or a key as absent, and the claim made from it then fails
+// (the swap finds the slot taken) and the walk is redone from agent-scope
+// loads.  `v` is overwritten.  Returns 1 if inserted, 0 if already present,
+// -1 if the pool is exhausted.
 template <bool MAP>
 MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t bucket, uint32_t dst,
                              uint32_t w, uint32_t seed, uint32_t (&v)[kWordsPerLane]) {
@@ -266,9 +280,11 @@ MK_DEV int slab_insert_group(const GraphView &G, const PoolView &P, uint32_t buc
 }
 
 // Delete (src -> dst) for one group: the key becomes a tombstone.  `v` holds the
-// head slab, loaded by the caller as for slab_insert_group (an older view
-// is safe: the swap checks the key); it is overwritten.  Returns 1 if
-// deleted.
+// head slab in load_slab_coherent's layout, loaded by the caller (plain
+// loads will do: during a delete kernel words only go from key to
+// tombstone and chains do not change, so an older view can only show a key
+// already deleted, and the swap, which checks the key, then fails); it is
+// overwritten.  Returns 1 if deleted.
 template <bool MAP>
 MK_DEV int slab_delete_group(const GraphView &G, uint32_t bucket, uint32_t dst,
                              uint32_t (&v)[kWordsPerLane]) {
@@ -291,7 +307,7 @@ MK_DEV int slab_delete_group(const GraphView &G, uint32_t bucket, uint32_t dst,
     const uint32_t nx = group_read(v[kWordsPerLane - 1], kGroupLanes - 1);
     if (nx == kEmpty) return 0;
     slab = slab_ptr(G.pool, nx);
-    load_slab_coherent(slab, v);
+    load_slab_plain(slab, v);
   }
 }
 
