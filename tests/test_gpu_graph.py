@@ -107,6 +107,46 @@ def test_concurrent_duplicates_inserted_once(gpu):
         assert_same_adjacency(g, oracle.csr(n, s, d))
 
 
+@pytest.mark.parametrize("hints", [True, False])
+def test_repeated_deletes_and_reinserts_on_a_hub(gpu, hints):
+    # every key of one batch repeated, in random order, on a hub whose keys
+    # share few buckets (without hints: one chain of ~50 slabs): a repeated
+    # delete applies once; re-inserting deleted keys, each several times,
+    # applies each once into the freed slots, beside new keys
+    n, h = 5000, 3000
+    rng = np.random.default_rng(12)
+    s = np.r_[np.zeros(h, np.uint32), rng.integers(1, n, 20000).astype(np.uint32)]
+    d = np.r_[np.arange(1, h + 1, dtype=np.uint32), rng.integers(0, n, 20000).astype(np.uint32)]
+    keep = s != d
+    s, d = s[keep], d[keep]
+    pool, g = build(n, s, d, hints=hints)
+    ref = oracle.csr(n, s, d)
+    assert_same_adjacency(g, ref)
+    gone = np.arange(1, h + 1, 2, dtype=np.uint32)  # every other hub key
+    perm = rng.permutation(len(gone) * 6)
+    g.DeleteEdges(np.zeros(len(perm), np.uint32), np.tile(gone, 6)[perm])
+    st = g.last_status()
+    assert (st == _lib.EDGE_APPLIED).sum() == len(gone)
+    assert ((st == _lib.EDGE_APPLIED) | (st == _lib.EDGE_UNCHANGED)).all()
+    live = np.ones(len(s), bool)
+    live[:h][np.isin(d[:h], gone)] = False
+    assert_same_adjacency(g, oracle.csr(n, s[live], d[live]))
+    used = pool.stats()[0]
+    back = np.r_[np.tile(gone, 6), np.tile(np.arange(h + 1, h + 401, dtype=np.uint32), 4)]
+    perm = rng.permutation(len(back))
+    g.InsertEdges(np.zeros(len(back), np.uint32), back[perm])
+    assert (g.last_status() == _lib.EDGE_APPLIED).sum() == len(gone) + 400
+    s2 = np.r_[s, np.zeros(400, np.uint32)]
+    d2 = np.r_[d, np.arange(h + 1, h + 401, dtype=np.uint32)]
+    assert_same_adjacency(g, oracle.csr(n, s2, d2))
+    assert np.array_equal(g.degrees(), oracle.csr(n, s2, d2).degrees())
+    # the freed slots took the re-inserted keys: 400 new keys need at most
+    # ceil(400 / 63) more slabs in the hub's chains
+    assert pool.stats()[0] - used <= -(-400 // 63)
+    g.close()
+    pool.close()
+
+
 def test_weighted_pairs_and_keep_first_weight(gpu):
     n, s, d, w = gen.rmat(11, 16, seed=3, weighted=True)
     ref = oracle.csr(n, s, d, w)
