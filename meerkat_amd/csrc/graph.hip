@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <mutex>
 #include <string>
 
@@ -249,16 +250,21 @@ MK_DEV MutSide pick_side(bool second, const MutSide &A, const MutSide &B) {
 
 // Waves [0, waves_a) apply the batch to side A, the rest to side B, so the
 // forward graph and its mirror are updated by one launch with their
-// (latency-bound) chain walks overlapping.  Each wave takes `epw` (<= 64)
-// consecutive edges, lane i holding edge i; the eight 8-lane groups of the
-// wave apply their own lanes' edges one after another, each group walking
-// and claiming in its own slab chain (the reference runs one edge at a time
-// per warp, include/Graph/warp/InsertEdge.cuh:66-103).
+// (latency-bound) chain walks overlapping.  The batch is cut into chunks of
+// csz = ceil(epw / 8) consecutive edges (epw <= 64 edges per wave); group q
+// of a side (wave * 8 + group) takes chunk q * stride mod nchunks (stride
+// coprime with nchunks: every chunk once), group lane i holding the chunk's
+// edge i, and applies its edges one after another, walking and claiming in
+// its own slab chain (the reference runs one edge at a time per warp,
+// include/Graph/warp/InsertEdge.cuh:66-103).  Consecutive edges, which in a
+// batch sorted by source share buckets, so stay with one group, and the
+// groups in flight together work far-apart chunks instead of racing for
+// the same free slots.
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_insert(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b, PoolView P,
              const uint32_t *src, const uint32_t *dst, const uint32_t *wt, uint32_t n,
-             uint32_t epw, const uint32_t *bad, uint32_t epoch) {
+             uint32_t epw, uint32_t stride, const uint32_t *bad, uint32_t epoch) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -267,26 +273,33 @@ __global__ void __launch_bounds__(kBlock)
   if (second && wave_all - waves_a >= waves_b) return;
   const MutSide M = pick_side(second, A, B);
   const uint32_t wave = second ? wave_all - waves_a : wave_all;
-  const uint64_t base = uint64_t(wave) * epw;
-  if (base >= n) return;
-  const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  const uint32_t csz = (epw + kGroupsPerWave - 1) / kGroupsPerWave;
+  const uint64_t nchunks = (uint64_t(n) + csz - 1) / csz;
+  const uint64_t q = uint64_t(wave) * kGroupsPerWave;
+  if (q >= nchunks) return;
+  uint64_t base = 0;
+  uint32_t cnt = 0;  // this group's edges
+  if (q + (lane >> 3) < nchunks) {
+    base = ((q + (lane >> 3)) * stride % nchunks) * csz;
+    cnt = uint32_t(min<uint64_t>(csz, n - base));
+  }
   epoch += bad[1];  // the replay base of a captured step (0 otherwise)
   if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
-    if (lane < cnt) M.status[base + lane] = MK_EDGE_UNCHANGED;
+    if (gl < cnt) M.status[base + gl] = MK_EDGE_UNCHANGED;
     return;
   }
   uint32_t s = 0, d = 0, w = 0;
   bool act = false;
   uint8_t st = MK_EDGE_UNCHANGED;
-  if (lane < cnt) {
-    s = src[base + lane];
-    d = dst[base + lane];
+  if (gl < cnt) {
+    s = src[base + gl];
+    d = dst[base + gl];
     if (M.swap) {
       const uint32_t t = s;
       s = d;
       d = t;
     }
-    w = (MAP && wt) ? wt[base + lane] : 0u;
+    w = (MAP && wt) ? wt[base + gl] : 0u;
     act = s != d;
     if (!act) st = MK_EDGE_SELF_LOOP;
   }
@@ -322,7 +335,7 @@ __global__ void __launch_bounds__(kBlock)
       ++pool_fail;
     }
   }
-  if (lane < cnt) M.status[base + lane] = st;
+  if (gl < cnt) M.status[base + gl] = st;
   // pool failures (rare): one update per group that had any
   if (gl == 0 && pool_fail)
     add_epoch_count(reinterpret_cast<unsigned long long *>(M.counters + 2), epoch, pool_fail);
@@ -615,6 +628,15 @@ __global__ void __launch_bounds__(kBlock) k_release(GraphView G, PoolView P) {
     if (lane == 0) pool_free_lane(P, nx);
     nx = next;
   }
+}
+
+// A multiplier coprime with nchunks near 0.618 nchunks: q -> q * stride mod
+// nchunks visits every chunk once and sends neighbouring q far apart.
+static uint32_t chunk_stride(uint64_t nchunks) {
+  if (nchunks <= 2) return 1;
+  uint64_t p = uint64_t(double(nchunks) * 0.6180339887) | 1u;
+  while (std::gcd(p, nchunks) != 1) p += 2;
+  return uint32_t(p % nchunks);
 }
 
 static uint32_t edges_per_wave(uint32_t n, int device) {
@@ -947,7 +969,10 @@ static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
                           const uint32_t *w, uint32_t n, hipStream_t s, const char *name,
                           const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
-  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
+  const uint32_t csz = (epw + kGroupsPerWave - 1) / kGroupsPerWave;
+  const uint64_t nchunks = (uint64_t(n) + csz - 1) / csz;
+  const uint32_t stride = chunk_stride(nchunks);
+  const uint64_t waves = (nchunks + kGroupsPerWave - 1) / kGroupsPerWave;
   const MutSide a = side_of(g, false, g->counters),
                 b = g->rev ? side_of(g->rev, true, g->counters) : a;
   const uint64_t total = g->rev ? 2 * waves : waves;
@@ -958,7 +983,7 @@ static void launch_insert(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   prof_launch(name, bytes, g->weighted ? k_insert<true> : k_insert<false>,
               dim3(blocks_for_waves(total)), dim3(kBlock), 0u, s, a, b, uint32_t(waves),
               uint32_t(total - waves), g->pool->view, src, dst, g->weighted ? w : nullptr, n, epw,
-              bad, g->mut_epoch);
+              stride, bad, g->mut_epoch);
 }
 
 // Launch the delete kernel over g (and its mirror); only_status != 0xFF
