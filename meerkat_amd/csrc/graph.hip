@@ -342,13 +342,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Delete the batch edges whose status equals `only_status` (all edges when
-// only_status == 0xFF); records MK_EDGE_APPLIED / MK_EDGE_UNCHANGED.  Sides
-// and groups as in k_insert.
+// only_status == 0xFF); records MK_EDGE_APPLIED / MK_EDGE_UNCHANGED.  Sides,
+// chunks and groups as in k_insert (in a batch sorted by source, groups
+// in flight together would otherwise swap words of the same slabs).
 template <bool MAP>
 __global__ void __launch_bounds__(kBlock)
     k_delete(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b, const uint32_t *src,
-             const uint32_t *dst, uint32_t n, uint32_t epw, uint32_t only_status,
-             const uint32_t *bad, uint32_t epoch) {
+             const uint32_t *dst, uint32_t n, uint32_t epw, uint32_t stride,
+             uint32_t only_status, const uint32_t *bad, uint32_t epoch) {
   const uint32_t lane = lane_id();
   const uint32_t gl = group_lane();
   const uint32_t wave_all = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -357,27 +358,34 @@ __global__ void __launch_bounds__(kBlock)
   if (second && wave_all - waves_a >= waves_b) return;
   const MutSide M = pick_side(second, A, B);
   const uint32_t wave = second ? wave_all - waves_a : wave_all;
-  const uint64_t base = uint64_t(wave) * epw;
-  if (base >= n) return;
-  const uint32_t cnt = uint32_t(min<uint64_t>(epw, n - base));
+  const uint32_t csz = (epw + kGroupsPerWave - 1) / kGroupsPerWave;
+  const uint64_t nchunks = (uint64_t(n) + csz - 1) / csz;
+  const uint64_t q = uint64_t(wave) * kGroupsPerWave;
+  if (q >= nchunks) return;
+  uint64_t base = 0;
+  uint32_t cnt = 0;  // this group's edges
+  if (q + (lane >> 3) < nchunks) {
+    base = ((q + (lane >> 3)) * stride % nchunks) * csz;
+    cnt = uint32_t(min<uint64_t>(csz, n - base));
+  }
   epoch += bad[1];  // the replay base of a captured step (0 otherwise)
   if (*bad == epoch) {  // the batch failed the range check: leave the graph as it is
-    if (lane < cnt && only_status == 0xFFu) M.status[base + lane] = MK_EDGE_UNCHANGED;
+    if (gl < cnt && only_status == 0xFFu) M.status[base + gl] = MK_EDGE_UNCHANGED;
     return;
   }
   uint32_t s = 0, d = 0;
   bool act = false;
   uint8_t st = MK_EDGE_UNCHANGED;
-  if (lane < cnt) {
-    s = src[base + lane];
-    d = dst[base + lane];
+  if (gl < cnt) {
+    s = src[base + gl];
+    d = dst[base + gl];
     if (M.swap) {
       const uint32_t t = s;
       s = d;
       d = t;
     }
     if (only_status != 0xFFu) {
-      act = M.status[base + lane] == only_status;
+      act = M.status[base + gl] == only_status;
     } else {
       act = s != d;
       if (!act) st = MK_EDGE_SELF_LOOP;
@@ -402,9 +410,9 @@ __global__ void __launch_bounds__(kBlock)
     if (r == 1 && gl == 0 && M.degrees == 1) atomicSub(G.vdeg + sj, 1u);
   }
   if (only_status != 0xFFu) {
-    if (lane < cnt && act) M.status[base + lane] = MK_EDGE_UNCHANGED;
-  } else if (lane < cnt) {
-    M.status[base + lane] = st;
+    if (gl < cnt && act) M.status[base + gl] = MK_EDGE_UNCHANGED;
+  } else if (gl < cnt) {
+    M.status[base + gl] = st;
   }
 }
 
@@ -412,7 +420,9 @@ __global__ void __launch_bounds__(kBlock)
 // edge present gets the batch's weight (MK_EDGE_APPLIED); an absent edge is
 // not inserted (MK_EDGE_UNCHANGED); self-loops are skipped, as the
 // reference's UpdateEdgesKernel does (device/UpdateEdgeKernel.cuh:28).
-// Sides and groups as in k_insert.
+// Sides as in k_insert; each wave takes `epw` consecutive edges, lane i
+// holding edge i, and its eight groups apply their own lanes' edges one
+// after another.
 __global__ void __launch_bounds__(kBlock)
     k_update_weights(MutSide A, MutSide B, uint32_t waves_a, uint32_t waves_b,
                      const uint32_t *src, const uint32_t *dst, const uint32_t *wt, uint32_t n,
@@ -992,7 +1002,10 @@ static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst,
                           hipStream_t s, uint32_t only_status, const char *name,
                           const uint32_t *bad) {
   const uint32_t epw = edges_per_wave(n, g->device);
-  const uint64_t waves = (uint64_t(n) + epw - 1) / epw;
+  const uint32_t csz = (epw + kGroupsPerWave - 1) / kGroupsPerWave;
+  const uint64_t nchunks = (uint64_t(n) + csz - 1) / csz;
+  const uint32_t stride = chunk_stride(nchunks);
+  const uint64_t waves = (nchunks + kGroupsPerWave - 1) / kGroupsPerWave;
   const MutSide a = side_of(g, false, g->counters),
                 b = g->rev ? side_of(g->rev, true, g->counters) : a;
   const uint64_t total = g->rev ? 2 * waves : waves;
@@ -1001,7 +1014,7 @@ static void launch_delete(mk_graph *g, const uint32_t *src, const uint32_t *dst,
   prof_launch(name, double(n) * (g->rev ? 2 * 277 - 8 : 277),
               g->weighted ? k_delete<true> : k_delete<false>, dim3(blocks_for_waves(total)),
               dim3(kBlock), 0u, s, a, b, uint32_t(waves), uint32_t(total - waves), src, dst, n,
-              epw, only_status, bad, g->mut_epoch);
+              epw, stride, only_status, bad, g->mut_epoch);
 }
 
 // Launch the weight-update kernel over g (and its mirror).
